@@ -1,0 +1,88 @@
+"""ctypes mirrors of the plain-C descriptor types in include/afl_types.h."""
+import ctypes as C
+
+AFL_NODES_GAUSS_LOBATTO = 0
+AFL_NODES_EQUIDISTANT = 1
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("degree", C.c_int32), ("node_type", C.c_int32),
+        ("n_cells", C.c_int32 * 3),
+        ("x_min", C.c_double), ("x_max", C.c_double),
+        ("y_min", C.c_double), ("y_max", C.c_double),
+        ("z_top", C.c_double), ("hill_height", C.c_double), ("hill_width", C.c_double),
+        ("periodic", C.c_int32 * 2),
+    ]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [
+        ("gmres_eps", C.c_double), ("gmres_max_iter", C.c_int32), ("gmres_restart", C.c_int32),
+        ("fp_tol", C.c_double), ("fp_max_iter", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [
+        ("fp_iters", C.c_int32 * 2), ("gmres_iters", C.c_int32 * 2),
+        ("fp_converged", C.c_int32 * 2), ("pad_", C.c_int32 * 2),
+        ("schur_residual", C.c_double),
+    ]
+
+
+class HillState(C.Structure):
+    _fields_ = [
+        ("theta0", C.c_double), ("brunt", C.c_double), ("p0", C.c_double), ("u0", C.c_double),
+        ("theta_sigma", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class Sponge(C.Structure):
+    _fields_ = [
+        ("z_start", C.c_double), ("alpha_max", C.c_double), ("u_ref", C.c_double * 3),
+        ("enabled", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+def make_mesh(dim=2, degree=2, n_cells=(8, 1, 4), x_range=(-72e3, 72e3), y_range=(-72e3, 72e3),
+              z_top=25e3, hill_height=400.0, hill_width=1000.0, periodic=(1, 1),
+              node_type=AFL_NODES_GAUSS_LOBATTO):
+    m = MeshDesc()
+    m.dim, m.degree, m.node_type = dim, degree, node_type
+    for i in range(3):
+        m.n_cells[i] = int(n_cells[i])
+    m.x_min, m.x_max = x_range
+    m.y_min, m.y_max = y_range
+    m.z_top, m.hill_height, m.hill_width = z_top, hill_height, hill_width
+    m.periodic[0], m.periodic[1] = periodic
+    return m
+
+
+def make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01, seed=1234):
+    s = HillState()
+    s.theta0, s.brunt, s.p0, s.u0, s.theta_sigma, s.seed = theta0, brunt, p0, u0, theta_sigma, seed
+    return s
+
+
+def make_solver_params(gmres_eps=1e-10, gmres_max_iter=30000, gmres_restart=28, fp_tol=1e-10, fp_max_iter=100):
+    p = SolverParams()
+    p.gmres_eps, p.gmres_max_iter, p.gmres_restart = gmres_eps, gmres_max_iter, gmres_restart
+    p.fp_tol, p.fp_max_iter = fp_tol, fp_max_iter
+    return p
+
+
+def make_sponge(z_start=15e3, alpha_max=0.0, u_ref=(10.0, 0.0, 0.0), enabled=False):
+    s = Sponge()
+    s.z_start, s.alpha_max = z_start, alpha_max
+    for i in range(3):
+        s.u_ref[i] = u_ref[i]
+    s.enabled = 1 if enabled else 0
+    return s
+
+
+def n_per_field(mesh):
+    k1 = mesh.degree + 1
+    if mesh.dim == 2:
+        return mesh.n_cells[0] * mesh.n_cells[2] * k1 * k1
+    return mesh.n_cells[0] * mesh.n_cells[1] * mesh.n_cells[2] * k1 ** 3

@@ -1,0 +1,752 @@
+// C-ABI of libafl_hip.so (declarations and reference citations in include/afl.h).
+// Host side of the drop-in: EULEROperator state, NS_stage dispatch, the device-resident
+// Schur GMRES and the IMEX-ARK2 step driver. All vectors live on the device; the host only
+// sees reduction scalars (one small copy per Krylov iteration / fixed-point iteration).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/afl.h"
+#include "afl_internal.h"
+
+using namespace afl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int check_device(const char* where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(AFL_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+  return AFL_OK;
+}
+
+// ---- 1D tables (host) -------------------------------------------------------------------
+void legendre(int n, double x, double& p, double& dp) {
+  double p0 = 1.0, p1 = x;
+  if (n == 0) { p = 1.0; dp = 0.0; return; }
+  for (int j = 2; j <= n; ++j) {
+    const double p2 = ((2.0 * j - 1.0) * x * p1 - (j - 1.0) * p0) / j;
+    p0 = p1;
+    p1 = p2;
+  }
+  p = p1;
+  dp = n * (x * p1 - p0) / (x * x - 1.0);
+}
+
+// Gauss-Legendre points/weights on [0,1] (Newton on P_n from Chebyshev-like guesses)
+void gauss01(int n, double* x, double* w) {
+  for (int i = 0; i < n; ++i) {
+    double t = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p, dp;
+      legendre(n, t, p, dp);
+      const double d = p / dp;
+      t -= d;
+      if (std::fabs(d) < 1e-16) break;
+    }
+    double p, dp;
+    legendre(n, t, p, dp);
+    x[i] = 0.5 * (t + 1.0);
+    w[i] = 1.0 / ((1.0 - t * t) * dp * dp);
+  }
+}
+
+// support points: Gauss-Lobatto (end points + roots of P'_k) or equidistant, on [0,1]
+void nodes01(int k, int type, double* x) {
+  if (type == AFL_NODES_EQUIDISTANT) {
+    for (int i = 0; i <= k; ++i) x[i] = double(i) / k;
+    return;
+  }
+  x[0] = 0.0;
+  x[k] = 1.0;
+  for (int i = 1; i < k; ++i) {
+    double t = -std::cos(M_PI * i / k);
+    for (int it = 0; it < 100; ++it) {
+      double p, dp;
+      legendre(k, t, p, dp);
+      const double d2p = (2.0 * t * dp - k * (k + 1.0) * p) / (1.0 - t * t);
+      const double d = dp / d2p;
+      t -= d;
+      if (std::fabs(d) < 1e-16) break;
+    }
+    x[i] = 0.5 * (t + 1.0);
+  }
+}
+
+double lag(const double* xn, int n1, int i, double x) {
+  double v = 1.0;
+  for (int j = 0; j < n1; ++j)
+    if (j != i) v *= (x - xn[j]) / (xn[i] - xn[j]);
+  return v;
+}
+
+double lag_d(const double* xn, int n1, int i, double x) {
+  double s = 0.0;
+  for (int m = 0; m < n1; ++m) {
+    if (m == i) continue;
+    double v = 1.0 / (xn[i] - xn[m]);
+    for (int j = 0; j < n1; ++j)
+      if (j != i && j != m) v *= (x - xn[j]) / (xn[i] - xn[j]);
+    s += v;
+  }
+  return s;
+}
+
+void build_tab(int k, int node_type, Tab& T) {
+  std::memset(&T, 0, sizeof(T));
+  const int n1 = k + 1, nq = 2 * k + 1;
+  nodes01(k, node_type, T.ND);
+  gauss01(nq, T.QP, T.W);
+  for (int q = 0; q < nq; ++q)
+    for (int i = 0; i < n1; ++i) {
+      T.S[q][i] = lag(T.ND, n1, i, T.QP[q]);
+      T.D[q][i] = lag_d(T.ND, n1, i, T.QP[q]);
+    }
+  // inverse of the reference 1D mass matrix by Gauss-Jordan
+  double A[5][10];
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j) {
+      double m = 0.0;
+      for (int q = 0; q < nq; ++q) m += T.W[q] * T.S[q][i] * T.S[q][j];
+      A[i][j] = m;
+      A[i][n1 + j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < n1; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n1; ++r)
+      if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    for (int j = 0; j < 2 * n1; ++j) std::swap(A[c][j], A[piv][j]);
+    const double d = A[c][c];
+    for (int j = 0; j < 2 * n1; ++j) A[c][j] /= d;
+    for (int r = 0; r < n1; ++r)
+      if (r != c) {
+        const double f = A[r][c];
+        for (int j = 0; j < 2 * n1; ++j) A[r][j] -= f * A[c][j];
+      }
+  }
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j) T.MI[i][j] = A[i][n1 + j];
+}
+
+TableauC make_tableau() {
+  TableauC t{};
+  const double g = 2.0 - std::sqrt(2.0);
+  t.a21 = g; t.a22 = 0.0; t.a31 = 0.5; t.a32 = 0.5; t.a33 = 0.0;
+  t.at21 = 0.5 * g; t.at22 = 0.5 * g;
+  t.at31 = std::sqrt(2.0) / 4.0; t.at32 = std::sqrt(2.0) / 4.0; t.at33 = 1.0 - std::sqrt(2.0) / 2.0;
+  t.b1 = t.at31; t.b2 = t.at32; t.b3 = t.at33;
+  return t;
+}
+
+struct DevBuf {
+  double* p = nullptr;
+  int64_t n = 0;
+  int alloc(int64_t count) {
+    if (p && n >= count) return AFL_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, (size_t)(count > 0 ? count : 1) * sizeof(double)) != hipSuccess)
+      return fail(AFL_ERR_DEVICE, "hipMalloc failed");
+    n = count;
+    return AFL_OK;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct afl_operator {
+  OpCtx o;
+  unsigned ns = 1;
+  afl_solver_params sp{1e-8, 30000, 28, 1e-10, 100, 0};
+  afl_sponge sponge{15000.0, 0.0, {0.0, 0.0, 0.0}, 0, 0};
+  DevBuf rho_fix, p_fix, u_fix, inv_diag, dinv_D, red, dsc, wS, gV, gW, ydev;
+  int diag_comps = 0;
+  bool dinv_ready = false;
+  // IMEX work vectors
+  DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
+  double* host_sc = nullptr;  // pinned scalars
+  ~afl_operator() {
+    DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev,
+                     &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
+                     &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix};
+    for (DevBuf* b : all) b->release();
+    if (host_sc) hipHostFree(host_sc);
+  }
+  int64_t N() const { return o.N; }
+  int dim() const { return o.dim; }
+};
+
+namespace {
+
+constexpr int kMaxRestart = 60;
+// device scalar slots: [0,64) first CGS pass, [64,128) second pass, 128 norm, 129.. misc
+constexpr int kScH1 = 0, kScH2 = 64, kScNrm = 128, kScA = 129, kScB = 130, kScSize = 160;
+
+int ensure_common(afl_operator* op) {
+  int rc;
+  if ((rc = op->red.alloc(red_work_size(kMaxRestart + 2)))) return rc;
+  if ((rc = op->dsc.alloc(kScSize))) return rc;
+  if ((rc = op->wS.alloc(op->N() * op->dim()))) return rc;
+  if (!op->host_sc) {
+    if (hipHostMalloc(&op->host_sc, kScSize * sizeof(double), hipHostMallocDefault) != hipSuccess)
+      return fail(AFL_ERR_DEVICE, "hipHostMalloc failed");
+  }
+  return AFL_OK;
+}
+
+// fetch device scalars [off, off+cnt) to host_sc[off..]
+int fetch(afl_operator* op, int off, int cnt) {
+  if (hipMemcpyAsync(op->host_sc + off, op->dsc.p + off, cnt * sizeof(double), hipMemcpyDeviceToHost, op->o.stream) !=
+      hipSuccess)
+    return fail(AFL_ERR_DEVICE, "hipMemcpyAsync failed");
+  if (hipStreamSynchronize(op->o.stream) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
+  return AFL_OK;
+}
+
+// S p = D p - C A^{-1} B p, euler_operator.h:2374-2408 (A^{-1}: exact cell-local solve)
+void schur_apply(afl_operator* op, const double* p, double* y) {
+  launch_schur_w(op->o, op->rho_fix.p, p, op->wS.p);
+  launch_apply_C(op->o, op->p_fix.p, op->wS.p, p, y);
+}
+
+int ensure_dinv(afl_operator* op) {
+  int rc;
+  if ((rc = op->dinv_D.alloc(op->N()))) return rc;
+  if (!op->dinv_ready) {
+    launch_inv_diag(op->o, nullptr, 1.0 / (kGamma - 1.0), 1, op->dinv_D.p);
+    op->dinv_ready = true;
+  }
+  return AFL_OK;
+}
+
+// Right-preconditioned restarted GMRES, Jacobi(D), classical Gram-Schmidt applied twice.
+// (The reference's left-preconditioned deal.II GMRES compares the preconditioned residual
+// with eps*||rhs||; the Arnoldi estimate here is the true residual ||b - S x||.)
+int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+  int rc;
+  const int64_t N = op->N();
+  hipStream_t st = op->o.stream;
+  if (m < 1 || m > kMaxRestart) return fail(AFL_ERR_INVALID_ARGUMENT, "gmres_restart must be in 1..60");
+  if ((rc = ensure_common(op)) || (rc = ensure_dinv(op))) return rc;
+  if ((rc = op->gV.alloc((int64_t)(m + 1) * N)) || (rc = op->gW.alloc(2 * N)) || (rc = op->ydev.alloc(kMaxRestart)))
+    return rc;
+  double* V = op->gV.p;
+  double* w = op->gW.p;
+  double* z = op->gW.p + N;
+  double* sc = op->dsc.p;
+  double* hs = op->host_sc;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  int it = 0;
+  double res = 0.0;
+  while (true) {
+    schur_apply(op, x, w);
+    vec_copy(st, N, b, V);
+    vec_axpby(st, N, -1.0, w, 1.0, V);
+    vec_norm2(st, N, V, op->red.p, sc + kScNrm);
+    if ((rc = fetch(op, kScNrm, 1))) return rc;
+    const double beta = hs[kScNrm];
+    res = beta;
+    if (!(beta > tol)) break;
+    if (it >= max_it) {
+      *iters = it; *res_out = res;
+      return fail(AFL_ERR_NOT_CONVERGED, "Schur GMRES did not converge");
+    }
+    vec_scale_by_inv(st, N, sc + kScNrm, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    bool conv = false;
+    for (; j < m && it < max_it; ++j) {
+      ++it;
+      const double* vj = V + (int64_t)j * N;
+      double* vn = V + (int64_t)(j + 1) * N;
+      vec_mul(st, N, vj, op->dinv_D.p, z);
+      schur_apply(op, z, vn);
+      vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH1);
+      vec_maxpy(st, N, V, N, j + 1, sc + kScH1, -1.0, vn);
+      vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH2);
+      vec_maxpy(st, N, V, N, j + 1, sc + kScH2, -1.0, vn);
+      vec_norm2(st, N, vn, op->red.p, sc + kScNrm);
+      vec_scale_by_inv(st, N, sc + kScNrm, vn);
+      if ((rc = fetch(op, 0, kScNrm + 1))) return rc;
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hs[kScH1 + i] + hs[kScH2 + i];
+      H[(size_t)(j + 1) * m + j] = hs[kScNrm];
+      for (int i = 0; i < j; ++i) {
+        const double a0 = H[(size_t)i * m + j], a1 = H[(size_t)(i + 1) * m + j];
+        H[(size_t)i * m + j] = cs[i] * a0 + sn[i] * a1;
+        H[(size_t)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+      }
+      const double a0 = H[(size_t)j * m + j], a1 = H[(size_t)(j + 1) * m + j];
+      const double r = std::hypot(a0, a1);
+      cs[j] = a0 / r;
+      sn[j] = a1 / r;
+      H[(size_t)j * m + j] = r;
+      H[(size_t)(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      res = std::fabs(g[j + 1]);
+      if (res <= tol) { ++j; conv = true; break; }
+    }
+    for (int i = j - 1; i >= 0; --i) {
+      double v = g[i];
+      for (int l = i + 1; l < j; ++l) v -= H[(size_t)i * m + l] * y[l];
+      y[i] = v / H[(size_t)i * m + i];
+    }
+    if (j > 0) {
+      if (hipMemcpyAsync(op->ydev.p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(AFL_ERR_DEVICE, "hipMemcpyAsync failed");
+      vec_fill(st, N, 0.0, z);
+      vec_maxpy(st, N, V, N, j, op->ydev.p, 1.0, z);
+      vec_addmul(st, N, op->dinv_D.p, z, x);
+      // y must stay alive until the copy has been consumed
+      if (hipStreamSynchronize(st) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
+    }
+    if (conv) break;
+    if (it >= max_it) {
+      *iters = it; *res_out = res;
+      return fail(AFL_ERR_NOT_CONVERGED, "Schur GMRES did not converge");
+    }
+  }
+  *iters = it;
+  *res_out = res;
+  return check_device("gmres");
+}
+
+bool valid_op(afl_operator* op) { return op != nullptr; }
+
+}  // namespace
+
+// =========================================================================================
+extern "C" {
+
+const char* afl_last_error(void) { return g_err.c_str(); }
+
+int afl_create(const afl_mesh_desc* md, double mach, double froude, double dt, afl_operator** out) {
+  if (!out) return fail(AFL_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (!md) return fail(AFL_ERR_INVALID_ARGUMENT, "mesh is NULL");
+  if (md->dim != 2 && md->dim != 3) return fail(AFL_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
+  if (md->degree < 1 || md->degree > 4) return fail(AFL_ERR_INVALID_ARGUMENT, "degree must be 1..4");
+  if (md->node_type != AFL_NODES_GAUSS_LOBATTO && md->node_type != AFL_NODES_EQUIDISTANT)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "unknown node_type");
+  const int nx = md->n_cells[0], ny = md->dim == 3 ? md->n_cells[1] : 1, nz = md->n_cells[2];
+  if (nx < 1 || ny < 1 || nz < 1) return fail(AFL_ERR_INVALID_ARGUMENT, "cell counts must be positive");
+  if (!(md->x_max > md->x_min) || (md->dim == 3 && !(md->y_max > md->y_min)) || !(md->z_top > 0.0))
+    return fail(AFL_ERR_INVALID_ARGUMENT, "empty domain");
+  if (!(md->hill_width > 0.0) || !(md->hill_height < md->z_top) || md->hill_height < 0.0)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "hill must satisfy 0 <= h0 < z_top and a > 0");
+  if (!(dt > 0.0) || !(mach > 0.0) || !(froude > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "dt, Ma, Fr must be > 0");
+  const int64_t ncells = (int64_t)nx * ny * nz;
+  if (ncells > 2000000000LL) return fail(AFL_ERR_INVALID_ARGUMENT, "too many cells");
+  auto* op = new afl_operator;
+  OpCtx& o = op->o;
+  o.dim = md->dim;
+  o.k = md->degree;
+  build_tab(o.k, md->node_type, o.T);
+  Geo& g = o.geo;
+  g.nc[0] = nx; g.nc[1] = md->dim == 3 ? ny : nz; g.nc[2] = md->dim == 3 ? nz : 1;
+  g.ncells = (int)ncells;
+  g.per[0] = md->periodic[0] ? 1 : 0;
+  g.per[1] = (md->dim == 3 && md->periodic[1]) ? 1 : 0;
+  g.lo[0] = md->x_min; g.lo[1] = md->y_min;
+  g.hc[0] = (md->x_max - md->x_min) / nx;
+  if (md->dim == 3) { g.hc[1] = (md->y_max - md->y_min) / ny; g.hc[2] = md->z_top / nz; }
+  else { g.hc[1] = md->z_top / nz; g.hc[2] = 0.0; }
+  g.H = md->z_top; g.h0 = md->hill_height; g.inv_a2 = 1.0 / (md->hill_width * md->hill_width);
+  const int n1 = o.k + 1;
+  o.N = ncells * (md->dim == 2 ? n1 * n1 : n1 * n1 * n1);
+  o.Ma = mach; o.Fr = froude; o.dt = dt;
+  o.tab = make_tableau();
+  int rc;
+  if ((rc = op->rho_fix.alloc(o.N)) || (rc = op->p_fix.alloc(o.N)) || (rc = op->u_fix.alloc(o.N * o.dim)) ||
+      (rc = ensure_common(op))) {
+    delete op;
+    return rc;
+  }
+  vec_fill(o.stream, o.N, 1.0, op->rho_fix.p);
+  vec_fill(o.stream, o.N, 0.0, op->p_fix.p);
+  vec_fill(o.stream, o.N * o.dim, 0.0, op->u_fix.p);
+  if ((rc = check_device("afl_create"))) { delete op; return rc; }
+  *out = op;
+  return AFL_OK;
+}
+
+int afl_destroy(afl_operator* op) {
+  if (op) {
+    hipStreamSynchronize(op->o.stream);
+    delete op;
+  }
+  return AFL_OK;
+}
+
+int afl_set_stream(afl_operator* op, void* s) {
+  if (!valid_op(op)) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  op->o.stream = (hipStream_t)s;
+  return AFL_OK;
+}
+
+int afl_n_dofs(const afl_operator* op, int idx, int64_t* n) {
+  if (!op || !n) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (idx < 0 || idx > 2) return fail(AFL_ERR_INVALID_ARGUMENT, "dof_index must be 0, 1 or 2");
+  *n = idx == 0 ? op->o.N * op->o.dim : op->o.N;
+  return AFL_OK;
+}
+
+int afl_m(const afl_operator* op, int64_t* m) {
+  if (!op || !m) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  *m = (op->ns == 3 || op->ns == 5) ? op->o.N * op->o.dim : op->o.N;
+  return AFL_OK;
+}
+
+int afl_set_dt(afl_operator* op, double v) {
+  if (!op || !(v > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "dt must be > 0");
+  op->o.dt = v;
+  return AFL_OK;
+}
+int afl_set_mach(afl_operator* op, double v) {
+  if (!op || !(v > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "Ma must be > 0");
+  op->o.Ma = v;
+  return AFL_OK;
+}
+int afl_set_froude(afl_operator* op, double v) {
+  if (!op || !(v > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "Fr must be > 0");
+  op->o.Fr = v;
+  return AFL_OK;
+}
+int afl_set_hyperbolic_stage(afl_operator* op, unsigned s) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  if (s < 1 || s > 3) return fail(AFL_ERR_INVALID_ARGUMENT, "HYPERBOLIC_stage must be 1..3");
+  op->o.hyp = s;
+  return AFL_OK;
+}
+int afl_set_ns_stage(afl_operator* op, unsigned s) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  if (s < 1 || s > 6) return fail(AFL_ERR_INVALID_ARGUMENT, "NS_stage must be 1..6");
+  op->ns = s;
+  return AFL_OK;
+}
+int afl_set_rho_for_fixed(afl_operator* op, const double* v) {
+  if (!op || !v) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  vec_copy(op->o.stream, op->o.N, v, op->rho_fix.p);
+  return check_device("afl_set_rho_for_fixed");
+}
+int afl_set_pres_fixed(afl_operator* op, const double* v) {
+  if (!op || !v) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  vec_copy(op->o.stream, op->o.N, v, op->p_fix.p);
+  return check_device("afl_set_pres_fixed");
+}
+int afl_set_u_fixed(afl_operator* op, const double* v) {
+  if (!op || !v) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  vec_copy(op->o.stream, op->o.N * op->o.dim, v, op->u_fix.p);
+  return check_device("afl_set_u_fixed");
+}
+int afl_set_solver_params(afl_operator* op, const afl_solver_params* sp) {
+  if (!op || !sp) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (sp->gmres_restart < 1 || sp->gmres_restart > kMaxRestart) return fail(AFL_ERR_INVALID_ARGUMENT, "gmres_restart must be 1..60");
+  if (!(sp->gmres_eps > 0.0) || sp->gmres_max_iter < 1 || sp->fp_max_iter < 1 || !(sp->fp_tol > 0.0))
+    return fail(AFL_ERR_INVALID_ARGUMENT, "invalid solver parameters");
+  op->sp = *sp;
+  return AFL_OK;
+}
+int afl_set_sponge(afl_operator* op, const afl_sponge* s) {
+  if (!op || !s) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  op->sponge = *s;
+  return AFL_OK;
+}
+
+static int check_src(const double* const* src, int n_src, int need, const char* what) {
+  if (!src) return fail(AFL_ERR_INVALID_ARGUMENT, std::string(what) + ": src is NULL");
+  if (n_src != need)
+    return fail(AFL_ERR_INVALID_ARGUMENT, std::string(what) + ": expected " + std::to_string(need) +
+                                              " source vectors for this HYPERBOLIC_stage, got " + std::to_string(n_src));
+  for (int i = 0; i < n_src; ++i)
+    if (!src[i]) return fail(AFL_ERR_INVALID_ARGUMENT, std::string(what) + ": NULL source vector");
+  return AFL_OK;
+}
+
+int afl_vmult_rhs_rho_update(afl_operator* op, double* dst, const double* const* src, int n_src) {
+  if (!op || !dst) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int st = (int)op->o.hyp;
+  int rc;
+  if ((rc = check_src(src, n_src, 2 * st, "vmult_rhs_rho_update"))) return rc;
+  launch_rhs_rho(op->o, st, src, dst);
+  return check_device("vmult_rhs_rho_update");
+}
+
+int afl_vmult_rhs_velocity_fixed(afl_operator* op, double* dst, const double* const* src, int n_src) {
+  if (!op || !dst) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int st = (int)op->o.hyp;
+  int rc;
+  if ((rc = check_src(src, n_src, st == 1 ? 4 : (st == 2 ? 7 : 9), "vmult_rhs_velocity_fixed"))) return rc;
+  launch_rhs_u(op->o, st, src, dst);
+  return check_device("vmult_rhs_velocity_fixed");
+}
+
+int afl_vmult_rhs_pressure(afl_operator* op, double* dst, const double* const* src, int n_src) {
+  if (!op || !dst) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int st = (int)op->o.hyp;
+  int rc;
+  if ((rc = check_src(src, n_src, st == 1 ? 5 : (st == 2 ? 8 : 11), "vmult_rhs_pressure"))) return rc;
+  launch_rhs_p(op->o, st, src, dst);
+  return check_device("vmult_rhs_pressure");
+}
+
+int afl_vmult(afl_operator* op, double* dst, const double* src) {
+  if (!op || !dst || !src) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  switch (op->ns) {
+    case 1: case 4: launch_mass_apply(op->o, nullptr, 1.0, 1, src, dst); break;
+    case 2: {
+      int rc;
+      if ((rc = ensure_common(op))) return rc;
+      schur_apply(op, src, dst);
+      break;
+    }
+    case 3: case 5: launch_mass_apply(op->o, op->rho_fix.p, 1.0, op->o.dim, src, dst); break;
+    default: launch_mass_apply(op->o, nullptr, 1.0 / (kGamma - 1.0), 1, src, dst); break;
+  }
+  return check_device("vmult");
+}
+
+int afl_vmult_pressure(afl_operator* op, double* dst, const double* src) {
+  if (!op || !dst || !src) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_apply_B(op->o, src, dst);
+  return check_device("vmult_pressure");
+}
+
+int afl_vmult_enthalpy(afl_operator* op, double* dst, const double* src) {
+  if (!op || !dst || !src) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_apply_C(op->o, op->p_fix.p, src, nullptr, dst);
+  return check_device("vmult_enthalpy");
+}
+
+int afl_vmult_schur_velocity(afl_operator* op, double* w, const double* p) {
+  if (!op || !w || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_schur_w(op->o, op->rho_fix.p, p, w);
+  return check_device("vmult_schur_velocity");
+}
+
+int afl_compute_diagonal(afl_operator* op) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  const unsigned ns = op->ns;
+  const int comps = (ns == 3 || ns == 5) ? op->o.dim : 1;
+  int rc;
+  if ((rc = op->inv_diag.alloc(op->o.N * comps))) return rc;
+  if (ns == 1 || ns == 4) launch_inv_diag(op->o, nullptr, 1.0, 1, op->inv_diag.p);
+  else if (ns == 2 || ns == 6) launch_inv_diag(op->o, nullptr, 1.0 / (kGamma - 1.0), 1, op->inv_diag.p);
+  else launch_inv_diag(op->o, op->rho_fix.p, 1.0, comps, op->inv_diag.p);
+  op->diag_comps = comps;
+  return check_device("compute_diagonal");
+}
+
+int afl_get_matrix_diagonal_inverse(afl_operator* op, double* dst) {
+  if (!op || !dst) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (op->diag_comps == 0) return fail(AFL_ERR_STATE, "compute_diagonal has not been called");
+  vec_copy(op->o.stream, op->o.N * op->diag_comps, op->inv_diag.p, dst);
+  return check_device("get_matrix_diagonal_inverse");
+}
+
+int afl_solve_mass(afl_operator* op, double* x, const double* b) {
+  if (!op || !x || !b) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  switch (op->ns) {
+    case 1: case 4: launch_mass_solve(op->o, nullptr, 1.0, 1, b, x); break;
+    case 6: launch_mass_solve(op->o, nullptr, 1.0 / (kGamma - 1.0), 1, b, x); break;
+    case 3: case 5: launch_mass_solve(op->o, op->rho_fix.p, 1.0, op->o.dim, b, x); break;
+    default: return fail(AFL_ERR_STATE, "solve_mass: NS_stage 2 is the Schur system, use afl_schur_solve");
+  }
+  return check_device("solve_mass");
+}
+
+int afl_schur_solve(afl_operator* op, double* p, const double* rhs, int* iters, double* residual) {
+  if (!op || !p || !rhs) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  int rc;
+  if ((rc = ensure_common(op))) return rc;
+  vec_norm2(op->o.stream, op->o.N, rhs, op->red.p, op->dsc.p + kScA);
+  if ((rc = fetch(op, kScA, 1))) return rc;
+  const double tol = op->sp.gmres_eps * op->host_sc[kScA];
+  int it = 0;
+  double res = 0.0;
+  rc = gmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res);
+  if (iters) *iters = it;
+  if (residual) *residual = res;
+  return rc;
+}
+
+int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_info* info) {
+  if (!op || !rho || !u || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int64_t N = op->o.N, NU = N * op->o.dim;
+  hipStream_t st = op->o.stream;
+  int rc;
+  DevBuf* sc[] = {&op->rho2, &op->rho3, &op->rhoc, &op->p2, &op->p3, &op->pc, &op->rhs_r,
+                  &op->rhs_p, &op->tmp2, &op->pfix, &op->pfix_old};
+  for (DevBuf* b : sc)
+    if ((rc = b->alloc(N))) return rc;
+  DevBuf* vc[] = {&op->u2, &op->u3, &op->uc, &op->rhs_u, &op->tmp1, &op->Bp, &op->ufix};
+  for (DevBuf* b : vc)
+    if ((rc = b->alloc(NU))) return rc;
+  if ((rc = ensure_common(op))) return rc;
+  afl_step_info inf{};
+  OpCtx& o = op->o;
+
+  auto fixed_point = [&](int stage, const double* rho_new, const double* p_start, const double* u_start,
+                         double* p_out, double* u_out) -> int {
+    vec_copy(st, N, p_start, op->pfix_old.p);
+    vec_copy(st, NU, u_start, op->ufix.p);
+    int it = 0, conv = 0, gm_total = 0;
+    for (; it < op->sp.fp_max_iter; ++it) {
+      // pressure_fixed_point, Atmospheric_Flow.cc:545-673
+      vec_copy(st, N, op->pfix_old.p, op->p_fix.p);
+      vec_copy(st, NU, op->ufix.p, op->u_fix.p);
+      if (stage == 1) {
+        const double* sp_[5] = {rho, u, p, rho_new, op->ufix.p};
+        launch_rhs_p(o, 1, sp_, op->rhs_p.p);
+        const double* su_[4] = {rho, u, p, rho_new};
+        launch_rhs_u(o, 1, su_, op->rhs_u.p);
+      } else {
+        const double* sp_[8] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, rho_new, op->ufix.p};
+        launch_rhs_p(o, 2, sp_, op->rhs_p.p);
+        const double* su_[7] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, rho_new};
+        launch_rhs_u(o, 2, su_, op->rhs_u.p);
+      }
+      launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u.p, op->tmp1.p);
+      launch_apply_C(o, op->p_fix.p, op->tmp1.p, nullptr, op->tmp2.p);
+      vec_axpby(st, N, -1.0, op->tmp2.p, 1.0, op->rhs_p.p);
+      vec_copy(st, N, op->pfix_old.p, op->pfix.p);
+      int gi = 0;
+      double gr = 0.0;
+      const int r = afl_schur_solve(op, op->pfix.p, op->rhs_p.p, &gi, &gr);
+      gm_total += gi;
+      inf.schur_residual = gr;
+      if (r) return r;
+      // update_velocity, Atmospheric_Flow.cc:681-781
+      launch_apply_B(o, op->pfix.p, op->Bp.p);
+      vec_axpby(st, NU, -1.0, op->Bp.p, 1.0, op->rhs_u.p);
+      launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u.p, op->ufix.p);
+      // relative inf-norm change of the nodal pressure, Atmospheric_Flow.cc:1140-1150
+      vec_linf(st, N, op->pfix.p, op->red.p, op->dsc.p + kScA);
+      vec_linf_diff(st, N, op->pfix.p, op->pfix_old.p, op->red.p, op->dsc.p + kScB);
+      int rr;
+      if ((rr = fetch(op, kScA, 2))) return rr;
+      const double den = op->host_sc[kScA], num = op->host_sc[kScB];
+      if (num / (den + 1e-10) < op->sp.fp_tol) { conv = 1; ++it; break; }
+      vec_copy(st, N, op->pfix.p, op->pfix_old.p);
+    }
+    inf.fp_iters[stage - 1] = it;
+    inf.fp_converged[stage - 1] = conv;
+    inf.gmres_iters[stage - 1] = gm_total;
+    vec_copy(st, N, op->pfix.p, p_out);
+    vec_copy(st, NU, op->ufix.p, u_out);
+    return AFL_OK;
+  };
+
+  // stage 1
+  o.hyp = 1;
+  { const double* s[2] = {rho, u}; launch_rhs_rho(o, 1, s, op->rhs_r.p); }
+  launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho2.p);
+  vec_copy(st, N, op->rho2.p, op->rho_fix.p);
+  if ((rc = fixed_point(1, op->rho2.p, p, u, op->p2.p, op->u2.p))) return rc;
+  // stage 2
+  o.hyp = 2;
+  { const double* s[4] = {rho, u, op->rho2.p, op->u2.p}; launch_rhs_rho(o, 2, s, op->rhs_r.p); }
+  launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho3.p);
+  vec_copy(st, N, op->rho3.p, op->rho_fix.p);
+  if ((rc = fixed_point(2, op->rho3.p, op->p2.p, op->u2.p, op->p3.p, op->u3.p))) return rc;
+  // stage 3
+  o.hyp = 3;
+  { const double* s[6] = {rho, u, op->rho2.p, op->u2.p, op->rho3.p, op->u3.p}; launch_rhs_rho(o, 3, s, op->rhs_r.p); }
+  launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rhoc.p);
+  vec_copy(st, N, op->rhoc.p, op->rho_fix.p);
+  {
+    const double* s[9] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p};
+    launch_rhs_u(o, 3, s, op->rhs_u.p);
+  }
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u.p, op->uc.p);
+  {
+    const double* s[11] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p, op->rhoc.p, op->uc.p};
+    launch_rhs_p(o, 3, s, op->rhs_p.p);
+  }
+  launch_mass_solve(o, nullptr, 1.0 / (kGamma - 1.0), 1, op->rhs_p.p, op->pc.p);
+  vec_copy(st, N, op->rhoc.p, rho);
+  vec_copy(st, NU, op->uc.p, u);
+  vec_copy(st, N, op->pc.p, p);
+  if (op->sponge.enabled)
+    launch_sponge(o, op->sponge.z_start, op->sponge.alpha_max * o.dt, op->sponge.u_ref, u);
+  if (info) *info = inf;
+  return check_device("imex_step");
+}
+
+int afl_init_hill_state(afl_operator* op, const afl_hill_state* s, double* rho, double* u, double* p) {
+  if (!op || !s || !rho || !u || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(s->brunt > 0.0) || !(s->theta0 > 0.0) || !(s->p0 > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "bad background");
+  HillParams hp{s->theta0, s->brunt * s->brunt, s->p0, s->u0, s->theta_sigma, s->seed};
+  launch_init_state(op->o, hp, rho, u, p);
+  return check_device("init_hill_state");
+}
+
+int afl_background(afl_operator* op, const afl_hill_state* s, double* rho, double* p, double* th) {
+  if (!op || !s) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(s->brunt > 0.0) || !(s->theta0 > 0.0) || !(s->p0 > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "bad background");
+  int rc;
+  DevBuf tr, tp, tt;
+  double* r = rho; double* pp = p; double* t = th;
+  if (!r) { if ((rc = tr.alloc(op->o.N))) return rc; r = tr.p; }
+  if (!pp) { if ((rc = tp.alloc(op->o.N))) return rc; pp = tp.p; }
+  if (!t) { if ((rc = tt.alloc(op->o.N))) return rc; t = tt.p; }
+  HillParams hp{s->theta0, s->brunt * s->brunt, s->p0, s->u0, 0.0, 0};
+  launch_background(op->o, hp, r, pp, t);
+  rc = check_device("background");
+  hipStreamSynchronize(op->o.stream);
+  tr.release(); tp.release(); tt.release();
+  return rc;
+}
+
+}  // extern "C"
+
+// ---- layout conversion -------------------------------------------------------------------
+namespace {
+__global__ void k_pack(int64_t N, int npc, int ncomp, const double* __restrict__ src, const int64_t* __restrict__ perm,
+                       int64_t cs, int64_t as, int64_t ns, double* __restrict__ dst, bool to_afl) {
+  const int64_t total = N * ncomp;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int a = (int)(idx / N);
+    const int64_t r = idx % N;
+    const int64_t c = r / npc;
+    const int i = (int)(r % npc);
+    const int64_t pc = perm ? perm[c] : c;
+    const int64_t ci = pc * cs + a * as + i * ns;
+    if (to_afl) dst[idx] = src[ci];
+    else dst[ci] = src[idx];
+  }
+}
+}  // namespace
+
+extern "C" int afl_pack_dofs(afl_operator* op, int ncomp, const double* caller, const int64_t* perm, int64_t cs,
+                             int64_t as, int64_t ns, double* out) {
+  if (!op || !caller || !out || ncomp < 1) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int n1 = op->o.k + 1, npc = op->o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  int64_t b = (op->o.N * ncomp + AFL_NT - 1) / AFL_NT;
+  k_pack<<<dim3((unsigned)(b > 65536 ? 65536 : b)), dim3(AFL_NT), 0, op->o.stream>>>(op->o.N, npc, ncomp, caller, perm, cs, as, ns, out, true);
+  return check_device("pack_dofs");
+}
+
+extern "C" int afl_unpack_dofs(afl_operator* op, int ncomp, const double* in, const int64_t* perm, int64_t cs,
+                               int64_t as, int64_t ns, double* caller) {
+  if (!op || !caller || !in || ncomp < 1) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int n1 = op->o.k + 1, npc = op->o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  int64_t b = (op->o.N * ncomp + AFL_NT - 1) / AFL_NT;
+  k_pack<<<dim3((unsigned)(b > 65536 ? 65536 : b)), dim3(AFL_NT), 0, op->o.stream>>>(op->o.N, npc, ncomp, in, perm, cs, as, ns, caller, false);
+  return check_device("unpack_dofs");
+}

@@ -1,0 +1,553 @@
+// Shared device building blocks of the IMEX-DG kernels (gfx950).
+//
+// Work decomposition ("thread per quadrature line"): a cell is handled by LT = NQ^(dim-1)
+// threads; thread l owns the quadrature line parallel to the vertical axis through the
+// horizontal quadrature index l (2D: l = qx; 3D: l = qx + NQ*qy). Sum-factorised
+// interpolation runs along x (and y) with one LDS transpose per axis and along z inside
+// registers, so every thread ends up with the NQ values of its own line. Integration runs
+// the same sweeps transposed and leaves node row l (2D: z-row l; 3D: (iy, iz) = l) of the
+// cell's result in the registers of thread l, which then writes N1 contiguous doubles.
+//
+// Faces are computed cell-centrically from both sides (each cell evaluates the fluxes on
+// its own 2*dim faces with its own outward normal), so every output entry is written by
+// exactly one thread, without atomics. The fluxes of the reference are antisymmetric in
+// (+,-,n), so the two sides agree to the last bit except for the periodic wrap geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define AFL_NT 256
+
+namespace afl {
+
+constexpr double kGamma = 1.4;  // EquationData::Cp_Cv
+
+struct Tab {
+  double S[9][5];   // S[q][i]  = phi_i(x_q), Gauss point q of QGauss(2k+1) on [0,1]
+  double D[9][5];   // D[q][i]  = phi_i'(x_q)
+  double W[9];      // Gauss weights on [0,1]
+  double QP[9];     // Gauss points on [0,1]
+  double MI[5][5];  // inverse of the reference 1D mass matrix (cell-local preconditioner)
+  double ND[5];     // 1D support points
+};
+
+struct Geo {
+  int nc[3];        // cells along x, y, z (ny = 1 in 2D)
+  int ncells;
+  int per[2];       // periodic x, y
+  double lo[2];     // x_min, y_min
+  double hc[3];     // hx, hy, hz
+  double H, h0, inv_a2;
+};
+
+template <int DIM, int K>
+struct Cfg {
+  static constexpr int N1 = K + 1;
+  static constexpr int NQ = 2 * K + 1;
+  static constexpr int LT = (DIM == 2) ? NQ : NQ * NQ;        // threads per cell
+  static constexpr int CPB = AFL_NT / LT;                       // cells per block
+  static constexpr int NPC = (DIM == 2) ? N1 * N1 : N1 * N1 * N1;
+  static constexpr int NFQ = (DIM == 2) ? NQ : NQ * NQ;         // points per face
+  static constexpr int ROWS = (DIM == 2) ? N1 : N1 * N1;        // node rows per cell
+  // LDS doubles per cell for one evaluation (t1 + t2)
+  static constexpr int EV1 = (DIM == 2) ? N1 * NQ : N1 * N1 * NQ;
+  static constexpr int EV2 = (DIM == 2) ? 0 : N1 * NQ * NQ;
+  // LDS doubles per cell for one integration (2D: a,b [N1][NQ]; 3D: a,x,y [N1][NQ][NQ] + a2,x2 [N1][N1][NQ])
+  static constexpr int IN1 = (DIM == 2) ? 2 * N1 * NQ : 3 * N1 * NQ * NQ;
+  static constexpr int IN2 = (DIM == 2) ? 0 : 2 * N1 * N1 * NQ;
+};
+
+// Thread position inside the block.
+template <int DIM, int K>
+struct Slot {
+  int cb;        // cell slot in block
+  int l;         // line index in cell
+  int64_t cell;  // global cell index
+  bool act;      // slot maps to an existing cell
+  int ic[3];     // cell coordinates
+  __device__ Slot(const Geo& g) {
+    using C = Cfg<DIM, K>;
+    cb = threadIdx.x / C::LT;
+    l = threadIdx.x % C::LT;
+    cell = (int64_t)blockIdx.x * C::CPB + cb;
+    act = (cb < C::CPB) && (cell < g.ncells);
+    const int c = act ? (int)cell : 0;
+    ic[0] = c % g.nc[0];
+    if (DIM == 2) { ic[1] = c / g.nc[0]; ic[2] = 0; }
+    else { ic[1] = (c / g.nc[0]) % g.nc[1]; ic[2] = c / (g.nc[0] * g.nc[1]); }
+  }
+};
+
+template <int DIM>
+__device__ inline int vaxis() { return DIM - 1; }
+
+// Hill h(x,y) = h0 / (1 + r^2/a^2) and its horizontal derivatives.
+__device__ inline void hill(const Geo& g, double x, double y, double& h, double& hx, double& hy) {
+  const double s = 1.0 + (x * x + y * y) * g.inv_a2;
+  const double is = 1.0 / s;
+  h = g.h0 * is;
+  const double f = -2.0 * g.h0 * g.inv_a2 * is * is;
+  hx = f * x;
+  hy = f * y;
+}
+
+// Per-line geometry: everything that is constant along the thread's vertical line.
+template <int DIM, int K>
+struct LineGeo {
+  double c;      // hz (1 - h/H) = dz/dzeta
+  double dhx, dhy;
+  double wxy;    // horizontal quadrature weight product
+  __device__ void init(const Geo& g, const Tab& T, const int* ic, int l) {
+    using C = Cfg<DIM, K>;
+    double x, y = 0.0;
+    if (DIM == 2) {
+      x = g.lo[0] + (ic[0] + T.QP[l]) * g.hc[0];
+      wxy = T.W[l];
+    } else {
+      const int qx = l % C::NQ, qy = l / C::NQ;
+      x = g.lo[0] + (ic[0] + T.QP[qx]) * g.hc[0];
+      y = g.lo[1] + (ic[1] + T.QP[qy]) * g.hc[1];
+      wxy = T.W[qx] * T.W[qy];
+    }
+    double h;
+    hill(g, x, DIM == 3 ? y : 0.0, h, dhx, dhy);
+    c = g.hc[DIM - 1] * (1.0 - h / g.H);
+  }
+  // JxW at vertical quadrature index qz
+  __device__ double jxw(const Geo& g, const Tab& T, int qz) const {
+    return (DIM == 2 ? g.hc[0] : g.hc[0] * g.hc[1]) * c * wxy * T.W[qz];
+  }
+  // gradient contribution (physical G, already containing no JxW) -> reference coords * JxW
+  __device__ void gref(const Geo& g, const Tab& T, const int* ic, int qz, const double* G, double* R) const {
+    const double zeta = (ic[DIM - 1] + T.QP[qz]) * g.hc[DIM - 1];
+    const double fz = 1.0 - zeta / g.H;
+    const double w = wxy * T.W[qz];
+    if (DIM == 2) {
+      const double dx = dhx * fz * g.hc[0];
+      R[0] = G[0] * c * w;
+      R[1] = (-dx * G[0] + g.hc[0] * G[1]) * w;
+    } else {
+      const double hx = g.hc[0], hy = g.hc[1];
+      const double dx = dhx * fz * hx, dy = dhy * fz * hy;
+      R[0] = G[0] * hy * c * w;
+      R[1] = G[1] * hx * c * w;
+      R[2] = (-dx * hy * G[0] - dy * hx * G[1] + hx * hy * G[DIM - 1]) * w;
+    }
+  }
+};
+
+// Geometry of face f = 2*dir + side of cell ic at face point fq (2D: fq = q; 3D: fq = qa + NQ qb,
+// qa/qb along the two tangential axes in increasing order). Returns outward normal and JxW.
+template <int DIM, int K>
+__device__ inline void face_geo(const Geo& g, const Tab& T, const int* ic, int f, int fq, double* n, double& jxw) {
+  using C = Cfg<DIM, K>;
+  const int dir = f >> 1, side = f & 1;
+  const double sg = side ? 1.0 : -1.0;
+  if (DIM == 2) {
+    if (dir == 0) {
+      const double x = g.lo[0] + (ic[0] + side) * g.hc[0];
+      double h, hx, hy;
+      hill(g, x, 0.0, h, hx, hy);
+      n[0] = sg; n[1] = 0.0;
+      jxw = g.hc[1] * (1.0 - h / g.H) * T.W[fq];
+    } else {
+      const double x = g.lo[0] + (ic[0] + T.QP[fq]) * g.hc[0];
+      double h, hx, hy;
+      hill(g, x, 0.0, h, hx, hy);
+      const double zeta = (ic[1] + side) * g.hc[1];
+      const double vx = -hx * (1.0 - zeta / g.H);
+      const double nv = sqrt(1.0 + vx * vx);
+      n[0] = sg * vx / nv; n[1] = sg / nv;
+      jxw = g.hc[0] * nv * T.W[fq];
+    }
+  } else {
+    const int qa = fq % C::NQ, qb = fq / C::NQ;
+    const double w = T.W[qa] * T.W[qb];
+    double x, y, zeta;
+    if (dir == 0) {
+      x = g.lo[0] + (ic[0] + side) * g.hc[0];
+      y = g.lo[1] + (ic[1] + T.QP[qa]) * g.hc[1];
+      zeta = (ic[2] + T.QP[qb]) * g.hc[2];
+    } else if (dir == 1) {
+      x = g.lo[0] + (ic[0] + T.QP[qa]) * g.hc[0];
+      y = g.lo[1] + (ic[1] + side) * g.hc[1];
+      zeta = (ic[2] + T.QP[qb]) * g.hc[2];
+    } else {
+      x = g.lo[0] + (ic[0] + T.QP[qa]) * g.hc[0];
+      y = g.lo[1] + (ic[1] + T.QP[qb]) * g.hc[1];
+      zeta = (ic[2] + side) * g.hc[2];
+    }
+    double h, hx, hy;
+    hill(g, x, y, h, hx, hy);
+    const double c = g.hc[2] * (1.0 - h / g.H);
+    if (dir == 0) { n[0] = sg; n[1] = 0.0; n[2] = 0.0; jxw = g.hc[1] * c * w; }
+    else if (dir == 1) { n[0] = 0.0; n[1] = sg; n[2] = 0.0; jxw = g.hc[0] * c * w; }
+    else {
+      const double fz = 1.0 - zeta / g.H;
+      const double vx = -hx * fz, vy = -hy * fz;
+      const double nv = sqrt(1.0 + vx * vx + vy * vy);
+      n[0] = sg * vx / nv; n[1] = sg * vy / nv; n[2] = sg / nv;
+      jxw = g.hc[0] * g.hc[1] * nv * w;
+    }
+  }
+}
+
+// Neighbour across face f (-1 for a wall).
+template <int DIM>
+__device__ inline int64_t neighbour(const Geo& g, const int* ic, int f) {
+  const int dir = f >> 1, side = f & 1;
+  const int n = g.nc[dir];
+  int j[3] = {ic[0], ic[1], ic[2]};
+  int v = j[dir] + (side ? 1 : -1);
+  if (v < 0 || v >= n) {
+    if (dir < DIM - 1 && g.per[dir]) v = (v + n) % n;
+    else return -1;
+  }
+  j[dir] = v;
+  return DIM == 2 ? (int64_t)j[0] + (int64_t)g.nc[0] * j[1]
+                  : (int64_t)j[0] + (int64_t)g.nc[0] * (j[1] + (int64_t)g.nc[1] * j[2]);
+}
+
+// Node index of face node t (2D: t along the face; 3D: t = ta + N1 tb) on face (dir, side).
+template <int DIM, int K>
+__device__ inline int face_node(int dir, int side, int t) {
+  constexpr int N1 = K + 1;
+  const int s = side ? K : 0;
+  if (DIM == 2) return dir == 0 ? t * N1 + s : s * N1 + t;
+  const int ta = t % N1, tb = t / N1;
+  if (dir == 0) return s + N1 * (ta + N1 * tb);
+  if (dir == 1) return ta + N1 * (s + N1 * tb);
+  return ta + N1 * (tb + N1 * s);
+}
+
+// Trace of a scalar field of cell `cell` on face (dir, side) at face point fq (direct sum
+// over the face nodes; the 1D basis is nodal with end points, so only face nodes contribute).
+template <int DIM, int K>
+__device__ inline double trace(const double* __restrict__ f, int64_t cell, int dir, int side, int fq, const Tab& T) {
+  using C = Cfg<DIM, K>;
+  const double* u = f + cell * C::NPC;
+  double s = 0.0;
+  if (DIM == 2) {
+#pragma unroll
+    for (int t = 0; t < C::N1; ++t) s += T.S[fq][t] * u[face_node<DIM, K>(dir, side, t)];
+  } else {
+    const int qa = fq % C::NQ, qb = fq / C::NQ;
+#pragma unroll
+    for (int tb = 0; tb < C::N1; ++tb) {
+      double r = 0.0;
+#pragma unroll
+      for (int ta = 0; ta < C::N1; ++ta) r += T.S[qa][ta] * u[face_node<DIM, K>(dir, side, ta + C::N1 * tb)];
+      s += T.S[qb][tb] * r;
+    }
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Evaluate a scalar field (global dofs) at the quadrature points of the thread's line.
+// All threads of the block must call it (it contains barriers).
+template <int DIM, int K>
+__device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K>& s, double* t1, double* t2,
+                                 const Tab& T, double (&v)[2 * K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ;
+  const double* u = f + (s.act ? s.cell : 0) * C::NPC;
+  if (DIM == 2) {
+    if (s.act && s.l < N1) {
+      double d[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * d[i];
+        t1[s.l * NQ + q] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+      double c[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) c[i] = t1[i * NQ + s.l];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+        v[q] = a;
+      }
+    }
+    __syncthreads();
+  } else {
+    if (s.act && s.l < N1 * N1) {
+      double d[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * d[i];
+        t1[s.l * NQ + q] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+      double c[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) c[i] = t1[(iz * N1 + i) * NQ + qx];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+        t2[(iz * NQ + q) * NQ + qx] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+      double c[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) c[i] = t2[i * NQ * NQ + s.l];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+        v[q] = a;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Same as eval_line but the dofs come from an LDS copy of the cell (u = lds + cb*NPC).
+template <int DIM, int K>
+__device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, double* t1, double* t2,
+                                     const Tab& T, double (&v)[2 * K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ;
+  if (DIM == 2) {
+    if (s.act && s.l < N1) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[s.l * N1 + i];
+        t1[s.l * NQ + q] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * t1[i * NQ + s.l];
+        v[q] = a;
+      }
+    }
+    __syncthreads();
+  } else {
+    if (s.act && s.l < N1 * N1) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[s.l * N1 + i];
+        t1[s.l * NQ + q] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * t1[(iz * N1 + i) * NQ + qx];
+        t2[(iz * NQ + q) * NQ + qx] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * t2[i * NQ * NQ + s.l];
+        v[q] = a;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Integrate line data (values V and reference-gradient contributions G[d], all already
+// multiplied by JxW) against the test functions. Row `l` (l < ROWS) of the result ends up
+// in out[N1]. HAS_V / HAS_G select the terms. Contains barriers.
+template <int DIM, int K, bool HAS_V, bool HAS_G>
+__device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double* i2, const Tab& T,
+                                      const double (&V)[2 * K + 1], const double (&G)[DIM][2 * K + 1],
+                                      double (&out)[K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ;
+  if (DIM == 2) {
+    double* ta = i1;
+    double* tb = i1 + N1 * NQ;
+    if (s.act) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if (HAS_V) a += T.S[q][i] * V[q];
+          if (HAS_G) { a += T.D[q][i] * G[1][q]; b += T.S[q][i] * G[0][q]; }
+        }
+        ta[i * NQ + s.l] = a;
+        if (HAS_G) tb[i * NQ + s.l] = b;
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < N1) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          a += T.S[q][i] * ta[s.l * NQ + q];
+          if (HAS_G) a += T.D[q][i] * tb[s.l * NQ + q];
+        }
+        out[i] = a;
+      }
+    }
+    __syncthreads();
+  } else {
+    double* ta = i1;
+    double* tx = i1 + N1 * NQ * NQ;
+    double* ty = i1 + 2 * N1 * NQ * NQ;
+    double* ua = i2;
+    double* ux = i2 + N1 * N1 * NQ;
+    if (s.act) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if (HAS_V) a += T.S[q][i] * V[q];
+          if (HAS_G) { a += T.D[q][i] * G[2][q]; bx += T.S[q][i] * G[0][q]; by += T.S[q][i] * G[1][q]; }
+        }
+        ta[i * NQ * NQ + s.l] = a;
+        if (HAS_G) { tx[i * NQ * NQ + s.l] = bx; ty[i * NQ * NQ + s.l] = by; }
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0, bx = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int idx = (iz * NQ + q) * NQ + qx;
+          a += T.S[q][i] * ta[idx];
+          if (HAS_G) { a += T.D[q][i] * ty[idx]; bx += T.S[q][i] * tx[idx]; }
+        }
+        ua[(iz * N1 + i) * NQ + qx] = a;
+        if (HAS_G) ux[(iz * N1 + i) * NQ + qx] = bx;
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < N1 * N1) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          a += T.S[q][i] * ua[s.l * NQ + q];
+          if (HAS_G) a += T.D[q][i] * ux[s.l * NQ + q];
+        }
+        out[i] = a;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Add face-flux integrals to node row l. Fq points to the LDS block [2*DIM][NFQ] of this cell
+// (values already multiplied by JxW_f and oriented for this cell). Only rows on faces change.
+template <int DIM, int K>
+__device__ inline void add_faces(const Slot<DIM, K>& s, const double* Fq, const Tab& T, double (&out)[K + 1],
+                                 unsigned face_mask) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ, NFQ = C::NFQ;
+  if (!(s.act && s.l < C::ROWS)) return;
+  if (DIM == 2) {
+    const int iz = s.l;
+    // x faces: nodes (0, iz) and (K, iz)
+    if (face_mask & 1u) { double a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[0 * NFQ + q]; out[0] += a; }
+    if (face_mask & 2u) { double a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[1 * NFQ + q]; out[K] += a; }
+    const int fz = (iz == 0) ? 2 : ((iz == K) ? 3 : -1);
+    if (fz >= 0 && (face_mask & (1u << fz))) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i) { double a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][i] * Fq[fz * NFQ + q]; out[i] += a; }
+    }
+  } else {
+    const int iy = s.l % N1, iz = s.l / N1;
+    for (int side = 0; side < 2; ++side) {
+      if (!(face_mask & (1u << side))) continue;
+      const double* F = Fq + side * NFQ;
+      double a = 0.0;
+      for (int qb = 0; qb < NQ; ++qb) {
+        double r = 0.0;
+        for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][iy] * F[qb * NQ + qa];
+        a += T.S[qb][iz] * r;
+      }
+      out[side ? K : 0] += a;
+    }
+    const int fy = (iy == 0) ? 2 : ((iy == K) ? 3 : -1);
+    if (fy >= 0 && (face_mask & (1u << fy))) {
+      const double* F = Fq + fy * NFQ;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0;
+        for (int qb = 0; qb < NQ; ++qb) {
+          double r = 0.0;
+          for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][i] * F[qb * NQ + qa];
+          a += T.S[qb][iz] * r;
+        }
+        out[i] += a;
+      }
+    }
+    const int fz = (iz == 0) ? 4 : ((iz == K) ? 5 : -1);
+    if (fz >= 0 && (face_mask & (1u << fz))) {
+      const double* F = Fq + fz * NFQ;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        double a = 0.0;
+        for (int qb = 0; qb < NQ; ++qb) {
+          double r = 0.0;
+          for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][i] * F[qb * NQ + qa];
+          a += T.S[qb][iy] * r;
+        }
+        out[i] += a;
+      }
+    }
+  }
+}
+
+// Write node row l of a cell.
+template <int DIM, int K>
+__device__ inline void store_row(double* __restrict__ dst, const Slot<DIM, K>& s, const double (&out)[K + 1]) {
+  using C = Cfg<DIM, K>;
+  if (s.act && s.l < C::ROWS) {
+    double* o = dst + s.cell * C::NPC + s.l * C::N1;
+#pragma unroll
+    for (int i = 0; i < C::N1; ++i) o[i] = out[i];
+  }
+}
+
+}  // namespace afl

@@ -1,0 +1,62 @@
+// Host/device shared internals of libafl_hip.so (not part of the C-ABI).
+#pragma once
+
+#include "afl_common.h"
+
+namespace afl {
+
+// ARK2 tableau, euler_operator.h:214-240.
+struct TableauC {
+  double a21, a22, a31, a32, a33;
+  double at21, at22, at31, at32, at33;
+  double b1, b2, b3;
+};
+
+// Everything a launcher needs; a plain copy of the operator state.
+struct OpCtx {
+  int dim = 2, k = 1;
+  Geo geo{};
+  Tab T{};
+  int64_t N = 0;          // nodes per scalar field
+  double Ma = 1, Fr = 1, dt = 1;
+  unsigned hyp = 1;       // HYPERBOLIC_stage
+  TableauC tab{};
+  hipStream_t stream = nullptr;
+  double cell_tol = 1e-15;  // relative tolerance of the cell-local mass-type solves
+  int cell_maxit = 40;
+};
+
+// operator kernels (afl_ops.hip)
+void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double* dst);
+void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* dst);
+void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* dst);
+void launch_mass_apply(const OpCtx& o, const double* rho, double scale, int ncomp, const double* src, double* dst);
+void launch_inv_diag(const OpCtx& o, const double* rho, double scale, int ncomp, double* dst);
+void launch_apply_B(const OpCtx& o, const double* p, double* dst);
+void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst);
+void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x);
+void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* w);
+
+// vector kernels (afl_blas.hip); reductions leave their result in device memory
+void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);
+void vec_axpby(hipStream_t st, int64_t n, double a, const double* x, double b, double* y);  // y = a x + b y
+void vec_sub_mul(hipStream_t st, int64_t n, const double* b, const double* y, const double* d, double* out);  // (b-y)*d
+void vec_mul(hipStream_t st, int64_t n, const double* x, const double* d, double* out);
+void vec_addmul(hipStream_t st, int64_t n, const double* d, const double* z, double* x);  // x += d*z
+void vec_scale_by_inv(hipStream_t st, int64_t n, const double* s, double* x);  // x /= *s
+void vec_fill(hipStream_t st, int64_t n, double v, double* x);
+// h[j] = <V_j, w> for j < m (V_j = V + j*ldv); partial sums use `work` (>= red_work_size(m) doubles)
+void vec_mdot(hipStream_t st, int64_t n, const double* V, int64_t ldv, int m, const double* w, double* work, double* h);
+// w -= sum_j h[j] V_j  (h on device);  x += sum_j y[j] V_j with y on device
+void vec_maxpy(hipStream_t st, int64_t n, const double* V, int64_t ldv, int m, const double* h, double sign, double* w);
+void vec_norm2(hipStream_t st, int64_t n, const double* x, double* work, double* out);  // sqrt(<x,x>)
+void vec_linf(hipStream_t st, int64_t n, const double* x, double* work, double* out);
+void vec_linf_diff(hipStream_t st, int64_t n, const double* x, const double* y, double* work, double* out);
+int64_t red_work_size(int m);
+// state initialisation and sponge (afl_state.hip)
+struct HillParams { double theta0, N2, p0, u0, sigma; uint64_t seed; };
+void launch_init_state(const OpCtx& o, const HillParams& hp, double* rho, double* u, double* p);
+void launch_background(const OpCtx& o, const HillParams& hp, double* rho, double* p, double* th);
+void launch_sponge(const OpCtx& o, double z_start, double alpha_dt, const double* uref, double* u);
+
+}  // namespace afl

@@ -1,0 +1,179 @@
+"""Host-side mirror of the reference's EULEROperator (euler_operator.h:19-98) over the C-ABI
+of libafl_hip.so. Device memory and streams come from PyTorch (ROCm); every compute call
+goes to the hand-written HIP kernels. There is no CPU fallback: constructing an operator
+without the built library raises.
+
+Method names, argument meaning and stage semantics follow the reference:
+  set_dt / set_Mach / set_Froude / set_HYPERBOLIC_stage / set_NS_stage      (:34-53)
+  set_rho_for_fixed / set_pres_fixed / set_u_fixed                           (:310-339)
+  vmult_rhs_rho_update / vmult_rhs_velocity_fixed / vmult_rhs_pressure       (:55-62)
+  vmult (apply_add dispatch on NS_stage), vmult_pressure (B), vmult_enthalpy (C)
+  compute_diagonal / get_matrix_diagonal_inverse                             (:2589-2626)
+Vectors are torch.float64 CUDA tensors in the layout documented in include/afl.h.
+"""
+import ctypes as C
+import os
+
+from ._abi import MeshDesc, SolverParams, StepInfo, HillState, Sponge
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafl_hip.so")
+_lib = None
+
+
+class AflError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Load libafl_hip.so (built in-tree by atmospheric_flows_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AflError(f"{LIB_PATH} is missing: build it with `python -m atmospheric_flows_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, i64p = C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)
+    L.afl_create.argtypes = [C.POINTER(MeshDesc), C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]
+    L.afl_destroy.argtypes = [vp]
+    L.afl_last_error.restype = C.c_char_p
+    L.afl_set_stream.argtypes = [vp, C.c_void_p]
+    L.afl_n_dofs.argtypes = [vp, C.c_int, i64p]
+    L.afl_m.argtypes = [vp, i64p]
+    for n in ("afl_set_dt", "afl_set_mach", "afl_set_froude"):
+        getattr(L, n).argtypes = [vp, C.c_double]
+    for n in ("afl_set_hyperbolic_stage", "afl_set_ns_stage"):
+        getattr(L, n).argtypes = [vp, C.c_uint]
+    for n in ("afl_set_rho_for_fixed", "afl_set_pres_fixed", "afl_set_u_fixed"):
+        getattr(L, n).argtypes = [vp, dp]
+    L.afl_set_solver_params.argtypes = [vp, C.POINTER(SolverParams)]
+    L.afl_set_sponge.argtypes = [vp, C.POINTER(Sponge)]
+    for n in ("afl_vmult_rhs_rho_update", "afl_vmult_rhs_velocity_fixed", "afl_vmult_rhs_pressure"):
+        getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.c_int]
+    for n in ("afl_vmult", "afl_vmult_pressure", "afl_vmult_enthalpy", "afl_solve_mass", "afl_vmult_schur_velocity"):
+        getattr(L, n).argtypes = [vp, dp, dp]
+    L.afl_compute_diagonal.argtypes = [vp]
+    L.afl_get_matrix_diagonal_inverse.argtypes = [vp, dp]
+    L.afl_schur_solve.argtypes = [vp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.afl_imex_step.argtypes = [vp, dp, dp, dp, C.POINTER(StepInfo)]
+    L.afl_init_hill_state.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
+    L.afl_background.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
+    for n in ("afl_pack_dofs", "afl_unpack_dofs"):
+        getattr(L, n).argtypes = [vp, C.c_int, dp, dp, C.c_int64, C.c_int64, C.c_int64, dp]
+    _lib = L
+    return L
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8 and t.is_contiguous()):
+        raise AflError("expected a contiguous float64 device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class EulerOperator:
+    """EULEROperator on one MI355X; one handle per host thread (not reentrant)."""
+
+    def __init__(self, mesh: MeshDesc, mach=1.0, froude=1.0 / 9.80616 ** 0.5, dt=1.0, stream=None):
+        import torch
+        self.torch = torch
+        self.L = load_library()
+        self.mesh = mesh
+        h = C.c_void_p()
+        self._chk(self.L.afl_create(C.byref(mesh), mach, froude, dt, C.byref(h)))
+        self.h = h
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self._chk(self.L.afl_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+        self.dim = mesh.dim
+        n = C.c_int64()
+        self._chk(self.L.afl_n_dofs(self.h, 1, C.byref(n)))
+        self.N = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.afl_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise AflError(f"afl error {rc}: {self.L.afl_last_error().decode()}")
+
+    # ---- allocation helpers -----------------------------------------------------------------
+    def new_scalar(self):
+        return self.torch.zeros(self.N, dtype=self.torch.float64, device="cuda")
+
+    def new_vector(self):
+        return self.torch.zeros(self.dim * self.N, dtype=self.torch.float64, device="cuda")
+
+    # ---- setters -----------------------------------------------------------------------------
+    def set_dt(self, v): self._chk(self.L.afl_set_dt(self.h, v))
+    def set_Mach(self, v): self._chk(self.L.afl_set_mach(self.h, v))
+    def set_Froude(self, v): self._chk(self.L.afl_set_froude(self.h, v))
+    def set_HYPERBOLIC_stage(self, s): self._chk(self.L.afl_set_hyperbolic_stage(self.h, s))
+    def set_NS_stage(self, s): self._chk(self.L.afl_set_ns_stage(self.h, s))
+    def set_rho_for_fixed(self, t): self._chk(self.L.afl_set_rho_for_fixed(self.h, _ptr(t)))
+    def set_pres_fixed(self, t): self._chk(self.L.afl_set_pres_fixed(self.h, _ptr(t)))
+    def set_u_fixed(self, t): self._chk(self.L.afl_set_u_fixed(self.h, _ptr(t)))
+    def set_solver_params(self, sp: SolverParams): self._chk(self.L.afl_set_solver_params(self.h, C.byref(sp)))
+    def set_sponge(self, s: Sponge): self._chk(self.L.afl_set_sponge(self.h, C.byref(s)))
+
+    def n_dofs(self, dof_index):
+        n = C.c_int64()
+        self._chk(self.L.afl_n_dofs(self.h, dof_index, C.byref(n)))
+        return n.value
+
+    def m(self):
+        n = C.c_int64()
+        self._chk(self.L.afl_m(self.h, C.byref(n)))
+        return n.value
+
+    # ---- explicit-stage applies ---------------------------------------------------------------
+    def _rhs(self, fn, dst, srcs):
+        arr = (C.c_void_p * len(srcs))(*[t.data_ptr() for t in srcs])
+        for t in srcs:
+            _ptr(t)
+        self._chk(fn(self.h, _ptr(dst), arr, len(srcs)))
+        return dst
+
+    def vmult_rhs_rho_update(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_rho_update, dst, srcs)
+    def vmult_rhs_velocity_fixed(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_velocity_fixed, dst, srcs)
+    def vmult_rhs_pressure(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_pressure, dst, srcs)
+
+    # ---- linear blocks -------------------------------------------------------------------------
+    def vmult(self, dst, src): self._chk(self.L.afl_vmult(self.h, _ptr(dst), _ptr(src))); return dst
+    def vmult_pressure(self, dst, src): self._chk(self.L.afl_vmult_pressure(self.h, _ptr(dst), _ptr(src))); return dst
+    def vmult_enthalpy(self, dst, src): self._chk(self.L.afl_vmult_enthalpy(self.h, _ptr(dst), _ptr(src))); return dst
+    def vmult_schur_velocity(self, w, p):
+        self._chk(self.L.afl_vmult_schur_velocity(self.h, _ptr(w), _ptr(p)))
+        return w
+
+    def compute_diagonal(self): self._chk(self.L.afl_compute_diagonal(self.h))
+
+    def get_matrix_diagonal_inverse(self, dst):
+        self._chk(self.L.afl_get_matrix_diagonal_inverse(self.h, _ptr(dst)))
+        return dst
+
+    # ---- solves --------------------------------------------------------------------------------
+    def solve_mass(self, x, b): self._chk(self.L.afl_solve_mass(self.h, _ptr(x), _ptr(b))); return x
+
+    def schur_solve(self, p, rhs):
+        it = C.c_int(0)
+        res = C.c_double(0.0)
+        self._chk(self.L.afl_schur_solve(self.h, _ptr(p), _ptr(rhs), C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    def imex_step(self, rho, u, p):
+        info = StepInfo()
+        self._chk(self.L.afl_imex_step(self.h, _ptr(rho), _ptr(u), _ptr(p), C.byref(info)))
+        return info
+
+    def init_hill_state(self, st: HillState):
+        rho, u, p = self.new_scalar(), self.new_vector(), self.new_scalar()
+        self._chk(self.L.afl_init_hill_state(self.h, C.byref(st), _ptr(rho), _ptr(u), _ptr(p)))
+        return rho, u, p
+
+    def background(self, st: HillState):
+        rho, p, th = self.new_scalar(), self.new_scalar(), self.new_scalar()
+        self._chk(self.L.afl_background(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(th)))
+        return rho, p, th

@@ -1,0 +1,125 @@
+/*
+ * C-ABI of libafl_hip.so: the MI355X (gfx950) drop-in for the matrix-free cell and face
+ * loops of the reference's EULEROperator and for the solves around them.
+ *
+ * Reference interface (letiziabottani/Atmospheric_Flows):
+ *   class EULEROperator : MatrixFreeOperators::Base<dim, Vec>    euler_operator.h:19-98
+ *   consumed as a matrix by SolverCG / SolverGMRES / PreconditionJacobi / mg::Matrix
+ *   (Atmospheric_Flow.cc:518, 527, 642, 672).
+ *
+ * Conventions
+ *  - Every vector argument is a DEVICE pointer (hipMalloc'ed or any device allocation).
+ *    Scalar fields hold N = cells*(k+1)^dim doubles, cell-major, nodes lexicographic with
+ *    x fastest; vector fields (velocity) hold dim consecutive blocks of N doubles.
+ *    afl_pack_dofs / afl_unpack_dofs convert from/to a caller layout with explicit strides
+ *    and a cell permutation (e.g. deal.II's FESystem DoF order).
+ *  - Calls are ordered on the handle's stream (afl_set_stream; default: the null stream).
+ *    Solver entry points synchronise that stream where they need a scalar on the host.
+ *  - A handle is not reentrant (the reference's NS_stage is mutable, euler_operator.h:91):
+ *    use one handle per host thread.
+ *  - Errors never cross the ABI as exceptions: every call returns an afl_status and
+ *    afl_last_error() returns a thread-local message.
+ *  - Setters copy their vector into device memory owned by the handle (the reference
+ *    deep-copies, euler_operator.h:312-339); the caller keeps ownership of its buffers.
+ */
+#ifndef AFL_H
+#define AFL_H
+
+#include <stdint.h>
+
+#include "afl_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum afl_status {
+  AFL_OK = 0,
+  AFL_ERR_INVALID_ARGUMENT = 1,
+  AFL_ERR_NOT_CONVERGED = 2, /* SolverControl::NoConvergence in the reference */
+  AFL_ERR_DEVICE = 3,
+  AFL_ERR_STATE = 4          /* call not valid for the current NS_stage / setup */
+} afl_status;
+
+typedef struct afl_operator afl_operator;
+
+/* ---- lifetime ------------------------------------------------------------------------ */
+/* Replaces EULEROperator(data) (euler_operator.h:228-240) + MatrixFree::reinit
+ * (Atmospheric_Flow.cc:325-348): mesh/basis descriptor, Ma, Fr, dt. */
+int afl_create(const afl_mesh_desc* mesh, double mach, double froude, double dt, afl_operator** out);
+int afl_destroy(afl_operator* op);
+const char* afl_last_error(void);
+int afl_set_stream(afl_operator* op, void* hip_stream);
+/* Number of unknowns of DoFHandler index 0 (velocity), 1 (pressure), 2 (density),
+ * as MatrixFree::initialize_dof_vector (Atmospheric_Flow.cc:351-375). */
+int afl_n_dofs(const afl_operator* op, int dof_index, int64_t* n);
+/* Base::m(): rows of the operator selected by NS_stage. */
+int afl_m(const afl_operator* op, int64_t* m);
+
+/* ---- setters, euler_operator.h:34-53 / 243-339 ------------------------------------- */
+int afl_set_dt(afl_operator* op, double dt);
+int afl_set_mach(afl_operator* op, double mach);
+int afl_set_froude(afl_operator* op, double froude);
+int afl_set_hyperbolic_stage(afl_operator* op, unsigned stage); /* 1..3, :283-288 */
+int afl_set_ns_stage(afl_operator* op, unsigned stage);         /* 1..6, :296-301 */
+int afl_set_rho_for_fixed(afl_operator* op, const double* rho); /* :310-315 */
+int afl_set_pres_fixed(afl_operator* op, const double* p);      /* :321-327 */
+int afl_set_u_fixed(afl_operator* op, const double* u);         /* :333-339 */
+int afl_set_solver_params(afl_operator* op, const afl_solver_params* sp);
+int afl_set_sponge(afl_operator* op, const afl_sponge* sponge);
+
+/* ---- explicit-stage applies (rows E1-E3), euler_operator.h:55-62 ---------------------
+ * src holds device pointers in the reference's positional order
+ * (Atmospheric_Flow.cc:438-450, 555-568, 696-698, 796-799); n_src is checked per stage. */
+int afl_vmult_rhs_rho_update(afl_operator* op, double* dst, const double* const* src, int n_src);     /* :697-712 */
+int afl_vmult_rhs_velocity_fixed(afl_operator* op, double* dst, const double* const* src, int n_src); /* :2310-2325 */
+int afl_vmult_rhs_pressure(afl_operator* op, double* dst, const double* const* src, int n_src);       /* :1366-1381 */
+
+/* ---- linear blocks ------------------------------------------------------------------- */
+/* Base::vmult -> apply_add (euler_operator.h:2370-2416): NS 1/4 M, 2 Schur S = D - C A^-1 B,
+ * 3/5 A (rho_for_fixed), 6 D. The A^-1 inside S is an exact cell-local solve. */
+int afl_vmult(afl_operator* op, double* dst, const double* src);
+int afl_vmult_pressure(afl_operator* op, double* dst, const double* src); /* B, :2420-2435 */
+int afl_vmult_enthalpy(afl_operator* op, double* dst, const double* src); /* C, :2438-2453 (uses pres_fixed) */
+/* First half of the Schur apply, w = A^{-1} B p (euler_operator.h:2380-2397: vmult_pressure
+ * followed by the inner CG on A), one fused kernel with an exact cell-local A solve. */
+int afl_vmult_schur_velocity(afl_operator* op, double* w, const double* p);
+/* compute_diagonal / get_matrix_diagonal_inverse (euler_operator.h:2589-2626) */
+int afl_compute_diagonal(afl_operator* op);
+int afl_get_matrix_diagonal_inverse(afl_operator* op, double* dst);
+
+/* ---- solves -------------------------------------------------------------------------- */
+/* Exact inverse of the block-diagonal mass-type operator selected by NS_stage (1/4 M,
+ * 3/5 A, 6 D). Replaces SolverCG + PreconditionMG on those systems
+ * (Atmospheric_Flow.cc:452-535, 571-642, 701-779, 801-870). */
+int afl_solve_mass(afl_operator* op, double* x, const double* b);
+/* Restarted GMRES on the Schur system S p = rhs with Jacobi(D) (Atmospheric_Flow.cc:652-672),
+ * right-preconditioned so that the stopping test ||rhs - S p|| <= gmres_eps * ||rhs|| is on the
+ * true residual. p holds the initial guess on entry; restart and iteration cap from
+ * afl_set_solver_params. */
+int afl_schur_solve(afl_operator* op, double* p, const double* rhs, int* iters, double* residual);
+/* One full IMEX-ARK2 step (Atmospheric_Flow.cc:1110-1251): density solves, fixed-point
+ * Schur loops of stages 1-2, explicit stage 3, then the sponge. Updates rho, u, p in place. */
+int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_info* info);
+
+/* ---- problem set-up helpers ---------------------------------------------------------- */
+/* Hydrostatic constant-N background + seeded nodal theta noise (replaces
+ * EulerSolver::initialize, Atmospheric_Flow.cc:417-423, for the hill problems). */
+int afl_init_hill_state(afl_operator* op, const afl_hill_state* st, double* rho, double* u, double* p);
+/* Noise-free background (rho_bar, p_bar, theta_bar); any output may be NULL. */
+int afl_background(afl_operator* op, const afl_hill_state* st, double* rho_bar, double* p_bar, double* theta_bar);
+
+/* ---- layout conversion ---------------------------------------------------------------
+ * dst[c*NPC*ncomp + ...] style caller layouts: caller index of (cell c, component a,
+ * node i) = cell_perm[c]*cell_stride + a*comp_stride + i*node_stride (cell_perm may be
+ * NULL for the identity; it is a device array of int64). */
+int afl_pack_dofs(afl_operator* op, int ncomp, const double* caller, const int64_t* cell_perm, int64_t cell_stride,
+                  int64_t comp_stride, int64_t node_stride, double* afl_layout);
+int afl_unpack_dofs(afl_operator* op, int ncomp, const double* afl_layout, const int64_t* cell_perm,
+                    int64_t cell_stride, int64_t comp_stride, int64_t node_stride, double* caller);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFL_H */

@@ -1,0 +1,84 @@
+/*
+ * Plain-C descriptor types shared by the C-ABI (afl.h) and its callers.
+ *
+ * The mesh descriptor replaces what the reference builds through deal.II's
+ * Triangulation + MappingQ + MatrixFree::reinit (Atmospheric_Flow.cc:280-348):
+ * a structured, terrain-following mesh over a Witch-of-Agnesi hill, with the
+ * exact analytic mapping
+ *     x = x_min + (ix + xi) hx,   y = y_min + (iy + eta) hy,
+ *     zeta = (iz + tau) hz,       z = zeta + h(x, y) (1 - zeta / z_top),
+ *     h(x, y) = h0 / (1 + (x^2 + y^2) / a^2)          (y absent in 2D).
+ * Cells are numbered x fastest, then y, then z; nodes inside a cell are
+ * lexicographic (x fastest) on the 1D support points of FE_DGQ(k).
+ */
+#ifndef AFL_TYPES_H
+#define AFL_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1D support points of the nodal DG basis (FE_DGQ(k) in the reference). */
+enum afl_node_type {
+  AFL_NODES_GAUSS_LOBATTO = 0, /* FE_DGQArbitraryNodes(QGaussLobatto(k+1)) */
+  AFL_NODES_EQUIDISTANT = 1    /* equispaced support points incl. end points */
+};
+
+typedef struct afl_mesh_desc {
+  int32_t dim;          /* 2 or 3 */
+  int32_t degree;       /* polynomial degree k, 1..4; quadrature has 2k+1 Gauss points per direction */
+  int32_t node_type;    /* enum afl_node_type */
+  int32_t n_cells[3];   /* cells along x, y, z (n_cells[1] ignored in 2D) */
+  double x_min, x_max;  /* horizontal extent [m] */
+  double y_min, y_max;  /* second horizontal extent [m], 3D only */
+  double z_top;         /* height H of the model top [m] */
+  double hill_height;   /* h0 [m] */
+  double hill_width;    /* a [m] */
+  int32_t periodic[2];  /* 1: periodic in x (resp. y); 0: wall. Bottom and top are walls. */
+} afl_mesh_desc;
+
+/* Solver controls of one IMEX step (reference values in brackets). */
+typedef struct afl_solver_params {
+  double gmres_eps;        /* relative tolerance of the Schur GMRES [prm eps = 1e-8] */
+  int32_t gmres_max_iter;  /* [prm max_iterations = 30000] */
+  int32_t gmres_restart;   /* Krylov basis size before restart [deal.II default 28] */
+  double fp_tol;           /* fixed-point tolerance on the relative inf-norm change of p [1e-10] */
+  int32_t fp_max_iter;     /* [100] */
+  int32_t pad_;
+} afl_solver_params;
+
+/* Per-step diagnostics (the reference prints these through pcout). */
+typedef struct afl_step_info {
+  int32_t fp_iters[2];      /* fixed-point iterations in stages 1 and 2 */
+  int32_t gmres_iters[2];   /* total Schur GMRES iterations in stages 1 and 2 */
+  int32_t fp_converged[2];  /* 1 if the fixed point met fp_tol */
+  int32_t pad_[2];
+  double schur_residual;    /* last GMRES residual estimate */
+} afl_step_info;
+
+/* Hydrostatic constant-N background plus seeded theta noise. */
+typedef struct afl_hill_state {
+  double theta0;      /* surface potential temperature [K] (280) */
+  double brunt;       /* Brunt-Vaisala frequency N [1/s] (0.01) */
+  double p0;          /* surface pressure [Pa] (1e5) */
+  double u0;          /* uniform inflow [m/s] (10) */
+  double theta_sigma; /* std-dev of the nodal theta perturbation [K] (0.01) */
+  uint64_t seed;      /* noise seed */
+} afl_hill_state;
+
+/* Rayleigh sponge applied to the velocity after every step. */
+typedef struct afl_sponge {
+  double z_start;     /* bottom of the sponge [m] (15 km) */
+  double alpha_max;   /* relaxation rate at the top [1/s] */
+  double u_ref[3];    /* velocity the sponge relaxes to */
+  int32_t enabled;
+  int32_t pad_;
+} afl_sponge;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFL_TYPES_H */

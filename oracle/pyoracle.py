@@ -1,0 +1,165 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of oracle/liboracle.so, the CPU FP64 restatement of the reference's
+EULEROperator (euler_operator.h) and IMEX stage logic (Atmospheric_Flow.cc). Only the
+tests, the smoke check and the bench's cpu_baseline leg import this module, and only as
+the checker / CPU baseline; the product path never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from atmospheric_flows_amd._abi import MeshDesc, SolverParams, StepInfo, HillState, Sponge
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_P = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(MeshDesc), C.c_double, C.c_double, C.c_double]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        for name in ("orc_set_dt", "orc_set_mach", "orc_set_froude"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_double]
+        for name in ("orc_set_hyperbolic_stage", "orc_set_ns_stage"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_uint]
+        for name in ("orc_set_rho_for_fixed", "orc_set_pres_fixed", "orc_set_u_fixed"):
+            getattr(L, name).argtypes = [C.c_void_p, _P]
+        L.orc_set_solver_params.argtypes = [C.c_void_p, C.POINTER(SolverParams)]
+        L.orc_set_sponge.argtypes = [C.c_void_p, C.POINTER(Sponge)]
+        L.orc_n_dofs.restype = C.c_int64
+        L.orc_n_dofs.argtypes = [C.c_void_p, C.c_int]
+        for name in ("orc_vmult_rhs_rho_update", "orc_vmult_rhs_velocity_fixed", "orc_vmult_rhs_pressure"):
+            getattr(L, name).argtypes = [C.c_void_p, _P, C.POINTER(_P), C.c_int]
+        for name in ("orc_vmult", "orc_vmult_pressure", "orc_vmult_enthalpy", "orc_solve_mass"):
+            getattr(L, name).argtypes = [C.c_void_p, _P, _P]
+        L.orc_compute_diagonal.argtypes = [C.c_void_p]
+        L.orc_get_matrix_diagonal_inverse.argtypes = [C.c_void_p, _P]
+        L.orc_schur_solve.argtypes = [C.c_void_p, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.orc_imex_step.argtypes = [C.c_void_p, _P, _P, _P, C.POINTER(StepInfo)]
+        L.orc_init_hill_state.argtypes = [C.c_void_p, C.POINTER(HillState), _P, _P, _P]
+        L.orc_background.argtypes = [C.c_void_p, C.POINTER(HillState), _P, _P, _P]
+        L.orc_basis_tables.argtypes = [C.c_void_p, _P, _P, _P, _P, _P]
+        L.orc_node_coords.argtypes = [C.c_void_p, _P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_P)
+
+
+class OracleOperator:
+    """Host-array mirror of EULEROperator on the CPU restatement."""
+
+    def __init__(self, mesh: MeshDesc, mach=1.0, froude=1.0 / np.sqrt(9.80616), dt=1.0):
+        self.L = lib()
+        self.mesh = mesh
+        self.h = self.L.orc_create(C.byref(mesh), mach, froude, dt)
+        if not self.h:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        self.dim = mesh.dim
+        self.N = int(self.L.orc_n_dofs(self.h, 1))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"oracle error {rc}: {self.L.orc_last_error().decode()}")
+        return rc
+
+    # setters (euler_operator.h:34-53)
+    def set_dt(self, v): self._chk(self.L.orc_set_dt(self.h, v))
+    def set_mach(self, v): self._chk(self.L.orc_set_mach(self.h, v))
+    def set_froude(self, v): self._chk(self.L.orc_set_froude(self.h, v))
+    def set_hyperbolic_stage(self, s): self._chk(self.L.orc_set_hyperbolic_stage(self.h, s))
+    def set_ns_stage(self, s): self._chk(self.L.orc_set_ns_stage(self.h, s))
+    def set_rho_for_fixed(self, a): self._chk(self.L.orc_set_rho_for_fixed(self.h, _ptr(np.ascontiguousarray(a))))
+    def set_pres_fixed(self, a): self._chk(self.L.orc_set_pres_fixed(self.h, _ptr(np.ascontiguousarray(a))))
+    def set_u_fixed(self, a): self._chk(self.L.orc_set_u_fixed(self.h, _ptr(np.ascontiguousarray(a))))
+    def set_solver_params(self, sp): self._chk(self.L.orc_set_solver_params(self.h, C.byref(sp)))
+    def set_sponge(self, s): self._chk(self.L.orc_set_sponge(self.h, C.byref(s)))
+
+    def _rhs(self, fn, n_out, srcs):
+        srcs = [np.ascontiguousarray(s, dtype=np.float64) for s in srcs]
+        arr = (_P * len(srcs))(*[_ptr(s) for s in srcs])
+        dst = np.zeros(n_out)
+        self._chk(fn(self.h, _ptr(dst), arr, len(srcs)))
+        return dst
+
+    def vmult_rhs_rho_update(self, srcs): return self._rhs(self.L.orc_vmult_rhs_rho_update, self.N, srcs)
+    def vmult_rhs_velocity_fixed(self, srcs): return self._rhs(self.L.orc_vmult_rhs_velocity_fixed, self.dim * self.N, srcs)
+    def vmult_rhs_pressure(self, srcs): return self._rhs(self.L.orc_vmult_rhs_pressure, self.N, srcs)
+
+    def _lin(self, fn, n_out, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros(n_out)
+        self._chk(fn(self.h, _ptr(dst), _ptr(src)))
+        return dst
+
+    def vmult(self, src, n_out=None):
+        return self._lin(self.L.orc_vmult, n_out or src.size, src)
+
+    def vmult_pressure(self, p): return self._lin(self.L.orc_vmult_pressure, self.dim * self.N, p)
+    def vmult_enthalpy(self, w): return self._lin(self.L.orc_vmult_enthalpy, self.N, w)
+    def solve_mass(self, b): return self._lin(self.L.orc_solve_mass, b.size, b)
+
+    def compute_diagonal(self): self._chk(self.L.orc_compute_diagonal(self.h))
+
+    def get_matrix_diagonal_inverse(self, n):
+        out = np.zeros(n)
+        self._chk(self.L.orc_get_matrix_diagonal_inverse(self.h, _ptr(out)))
+        return out
+
+    def schur_solve(self, p0, rhs):
+        p = np.ascontiguousarray(p0, dtype=np.float64).copy()
+        it = C.c_int(0); res = C.c_double(0)
+        self._chk(self.L.orc_schur_solve(self.h, _ptr(p), _ptr(np.ascontiguousarray(rhs)), C.byref(it), C.byref(res)))
+        return p, it.value, res.value
+
+    def imex_step(self, rho, u, p):
+        info = StepInfo()
+        self._chk(self.L.orc_imex_step(self.h, _ptr(rho), _ptr(u), _ptr(p), C.byref(info)))
+        return info
+
+    def init_hill_state(self, st: HillState):
+        rho = np.zeros(self.N); u = np.zeros(self.dim * self.N); p = np.zeros(self.N)
+        self._chk(self.L.orc_init_hill_state(self.h, C.byref(st), _ptr(rho), _ptr(u), _ptr(p)))
+        return rho, u, p
+
+    def background(self, st: HillState):
+        rho = np.zeros(self.N); p = np.zeros(self.N); th = np.zeros(self.N)
+        self._chk(self.L.orc_background(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(th)))
+        return rho, p, th
+
+    def basis_tables(self):
+        k = self.mesh.degree
+        n1, nq = k + 1, 2 * k + 1
+        nodes, qp, qw = np.zeros(n1), np.zeros(nq), np.zeros(nq)
+        S, D = np.zeros(nq * n1), np.zeros(nq * n1)
+        self._chk(self.L.orc_basis_tables(self.h, _ptr(nodes), _ptr(qp), _ptr(qw), _ptr(S), _ptr(D)))
+        return nodes, qp, qw, S.reshape(nq, n1), D.reshape(nq, n1)
+
+    def node_coords(self):
+        X = np.zeros(self.dim * self.N)
+        self._chk(self.L.orc_node_coords(self.h, _ptr(X)))
+        return X.reshape(self.dim, self.N)
