@@ -324,6 +324,145 @@ __device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K
   }
 }
 
+// Neighbour cells of the thread's cell, face order f = 2*dir + side (-1 for walls).
+template <int DIM>
+struct Nbrs {
+  int64_t c[2 * DIM];
+  __device__ Nbrs(const Geo& g, const int* ic) {
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) c[f] = neighbour<DIM>(g, ic, f);
+  }
+};
+
+// LDS doubles per cell for one field's face nodes: [face][own|neighbour][face node]
+template <int DIM, int K>
+struct FaceLds {
+  static constexpr int N1 = K + 1;
+  static constexpr int NFN = DIM == 2 ? N1 : N1 * N1;
+  static constexpr int FN = 2 * 2 * DIM * NFN;
+};
+
+// eval_line plus the traces of the same field at face point l of every face of the cell:
+// tr[f][0] own side, tr[f][1] neighbour side (at walls the neighbour entry is the own trace).
+// Row owners copy their own face nodes and fetch the neighbour's matching face nodes (2 scalar
+// loads per face direction) into fn during the x sweep; face-point threads interpolate after the
+// first barrier, so no barrier is added. fn: FaceLds::FN doubles per cell.
+template <int DIM, int K>
+__device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
+                                       double* t1, double* t2, double* fn, const Tab& T, double (&v)[2 * K + 1],
+                                       double (&tr)[2 * DIM][2]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN;
+  const double* u = f + (s.act ? s.cell : 0) * NPC;
+  auto FN = [&](int face, int side, int t) -> double& { return fn[(face * 2 + side) * NFN + t]; };
+  if (s.act && s.l < C::ROWS) {
+    double d[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double a = 0.0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) a += T.S[q][i] * d[i];
+      t1[s.l * NQ + q] = a;
+    }
+    // x faces: node (0|K, row l); neighbour's opposite node of the same row
+    FN(0, 0, s.l) = d[0];
+    FN(1, 0, s.l) = d[K];
+    FN(0, 1, s.l) = nb.c[0] >= 0 ? f[nb.c[0] * NPC + s.l * N1 + K] : d[0];
+    FN(1, 1, s.l) = nb.c[1] >= 0 ? f[nb.c[1] * NPC + s.l * N1] : d[K];
+    if (DIM == 2) {
+      // z faces: own rows 0 / K; neighbour rows K (below) / 0 (above), one node per row owner
+      if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
+      if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
+      if (nb.c[2] >= 0) FN(2, 1, s.l) = f[nb.c[2] * NPC + K * N1 + s.l];
+      if (nb.c[3] >= 0) FN(3, 1, s.l) = f[nb.c[3] * NPC + s.l];
+    } else {
+      const int iy = s.l % N1, iz = s.l / N1;
+      // y faces: own nodes (ix, 0|K, iz) -> t = ix + N1 iz; neighbour node (ix = iy, K|0, iz)
+      if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
+      if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
+      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = f[nb.c[2] * NPC + (iz * N1 + K) * N1 + iy];
+      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = f[nb.c[3] * NPC + (iz * N1) * N1 + iy];
+      // z faces: own nodes (ix, iy, 0|K) -> t = ix + N1 iy; neighbour node (ix = iy, iy = iz, K|0)
+      if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
+      if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
+      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = f[nb.c[4] * NPC + (K * N1 + iz) * N1 + iy];
+      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
+    }
+  }
+  __syncthreads();
+  // traces at face point l (every thread of an active cell owns one face point per face)
+  if (s.act) {
+#pragma unroll
+    for (int fc = 0; fc < 2 * DIM; ++fc)
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd) {
+        double a = 0.0;
+        if (DIM == 2) {
+#pragma unroll
+          for (int t = 0; t < N1; ++t) a += T.S[s.l][t] * FN(fc, sd, t);
+        } else {
+          const int qa = s.l % NQ, qb = s.l / NQ;
+#pragma unroll
+          for (int tb = 0; tb < N1; ++tb) {
+            double r = 0.0;
+#pragma unroll
+            for (int ta = 0; ta < N1; ++ta) r += T.S[qa][ta] * FN(fc, sd, ta + N1 * tb);
+            a += T.S[qb][tb] * r;
+          }
+        }
+        tr[fc][sd] = a;
+      }
+#pragma unroll
+    for (int fc = 0; fc < 2 * DIM; ++fc)
+      if (nb.c[fc] < 0) tr[fc][1] = tr[fc][0];
+  }
+  if (DIM == 2) {
+    if (s.act) {
+      double c[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) c[i] = t1[i * NQ + s.l];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+        v[q] = a;
+      }
+    }
+    __syncthreads();
+  } else {
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+      double c[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) c[i] = t1[(iz * N1 + i) * NQ + qx];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+        t2[(iz * NQ + q) * NQ + qx] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+      double c[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) c[i] = t2[i * NQ * NQ + s.l];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+        v[q] = a;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Same as eval_line but the dofs come from an LDS copy of the cell (u = lds + cb*NPC).
 template <int DIM, int K>
 __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, double* t1, double* t2,
@@ -381,6 +520,172 @@ __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, dou
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * t2[i * NQ * NQ + s.l];
         v[q] = a;
       }
+    }
+    __syncthreads();
+  }
+}
+
+// Batched evaluation of NF scalar fields whose cell dofs sit in LDS (field f at u + f*NPC).
+// t1 needs NF*EV1 doubles, t2 NF*EV2 doubles per cell. Contains barriers.
+template <int DIM, int K, int NF>
+__device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, double* t1, double* t2, const Tab& T,
+                                      double (&v)[NF][2 * K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, E1 = C::EV1, E2 = C::EV2;
+  if (DIM == 2) {
+    if (s.act && s.l < N1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          double a = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[f * NPC + s.l * N1 + i];
+          t1[f * E1 + s.l * NQ + q] = a;
+        }
+    }
+    __syncthreads();
+    if (s.act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double c[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) c[i] = t1[f * E1 + i * NQ + s.l];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          double a = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+          v[f][q] = a;
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+    if (s.act && s.l < N1 * N1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          double a = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[f * NPC + s.l * N1 + i];
+          t1[f * E1 + s.l * NQ + q] = a;
+        }
+    }
+    __syncthreads();
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double c[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) c[i] = t1[f * E1 + (iz * N1 + i) * NQ + qx];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          double a = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+          t2[f * E2 + (iz * NQ + q) * NQ + qx] = a;
+        }
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double c[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) c[i] = t2[f * E2 + i * NQ * NQ + s.l];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          double a = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
+          v[f][q] = a;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Batched value-only integration of NF line fields; row l of field f ends in out[f].
+// i1 needs NF*I1V doubles, i2 NF*I2V doubles per cell (2D: I1V = N1*NQ, I2V = 0;
+// 3D: I1V = N1*NQ*NQ, I2V = N1*N1*NQ). Contains barriers.
+template <int DIM, int K>
+struct IntV {
+  static constexpr int N1 = K + 1, NQ = 2 * K + 1;
+  static constexpr int I1 = DIM == 2 ? N1 * NQ : N1 * NQ * NQ;
+  static constexpr int I2 = DIM == 2 ? 0 : N1 * N1 * NQ;
+};
+
+template <int DIM, int K, int NF>
+__device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, double* i2, const Tab& T,
+                                         const double (&V)[NF][2 * K + 1], double (&out)[NF][K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ, A1 = IntV<DIM, K>::I1, A2 = IntV<DIM, K>::I2;
+  if (DIM == 2) {
+    if (s.act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) a += T.S[q][i] * V[f][q];
+          i1[f * A1 + i * NQ + s.l] = a;
+        }
+    }
+    __syncthreads();
+    if (s.act && s.l < N1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) a += T.S[q][i] * i1[f * A1 + s.l * NQ + q];
+          out[f][i] = a;
+        }
+    }
+    __syncthreads();
+  } else {
+    if (s.act) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) a += T.S[q][i] * V[f][q];
+          i1[f * A1 + i * NQ * NQ + s.l] = a;
+        }
+    }
+    __syncthreads();
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) a += T.S[q][i] * i1[f * A1 + (iz * NQ + q) * NQ + qx];
+          i2[f * A2 + (iz * N1 + i) * NQ + qx] = a;
+        }
+    }
+    __syncthreads();
+    if (s.act && s.l < N1 * N1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) a += T.S[q][i] * i2[f * A2 + s.l * NQ + q];
+          out[f][i] = a;
+        }
     }
     __syncthreads();
   }

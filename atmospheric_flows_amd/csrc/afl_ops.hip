@@ -14,6 +14,8 @@ struct Lds {
   static constexpr int IN = C::IN1 + C::IN2;
   static constexpr int W = EV > IN ? EV : IN;   // eval / integrate work area per cell
   static constexpr int F = 2 * DIM * C::NFQ;    // face flux block per cell and component
+  static constexpr int FN = FaceLds<DIM, K>::FN;
+  static constexpr int WF = W + FN;             // work area + one field's face nodes
 };
 
 #define AFL_SLOT_PTRS                                                      \
@@ -22,11 +24,28 @@ struct Lds {
   (void)N1;                                                                \
   Slot<DIM, K> s(g);                                                       \
   const int cbs = s.cb < C::CPB ? s.cb : 0;                                \
-  double* t1 = shw + cbs * Lds<DIM, K>::W;                                 \
+  double* t1 = shw + cbs * Lds<DIM, K>::WF;                                \
   double* t2 = t1 + C::EV1;                                                \
   double* i1 = t1;                                                         \
   double* i2 = t1 + C::IN1;                                                \
-  (void)t2; (void)i2;
+  double* fn = t1 + Lds<DIM, K>::W;                                        \
+  (void)t2; (void)i2; (void)fn;
+
+// Face geometry of the thread's face point on all 2*dim faces (outward normals, JxW).
+template <int DIM, int K>
+struct FacePts {
+  double n[2 * DIM][DIM];
+  double jxw[2 * DIM];
+  __device__ FacePts(const Geo& g, const Tab& T, const Slot<DIM, K>& s) {
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      double nn[3];
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nn, jxw[f]);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) n[f][d] = nn[d];
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------------------
 // E1: vmult_rhs_rho_update (euler_operator.h:342-473 cell, 476-692 faces, boundary none).
@@ -34,25 +53,44 @@ struct Lds {
 template <int DIM, int K, int NS>
 __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, Tab T, Srcs src, double dt, double w0, double w1,
                                                      double w2, int64_t N, double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   AFL_SLOT_PTRS
   const double wj[3] = {w0, w1, w2};
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
-  double V[NQ], G[DIM][NQ];
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
+  double V[NQ], G[DIM][NQ], F[2 * DIM];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) { V[q] = 0.0; for (int d = 0; d < DIM; ++d) G[d][q] = 0.0; }
 #pragma unroll
+  for (int f = 0; f < 2 * DIM; ++f) F[f] = 0.0;
+#pragma unroll
   for (int j = 0; j < NS; ++j) {
-    double r[NQ];
-    eval_line<DIM, K>(src.p[2 * j], s, t1, t2, T, r);
+    double r[NQ], trr[2 * DIM][2], tru[DIM][2 * DIM][2];
+    eval_line_faces<DIM, K>(src.p[2 * j], s, nb, t1, t2, fn, T, r, trr);
     if (j == 0) for (int q = 0; q < NQ; ++q) V[q] = r[q];
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       double u[NQ];
-      eval_line<DIM, K>(src.p[2 * j + 1] + d * N, s, t1, t2, T, u);
+      eval_line_faces<DIM, K>(src.p[2 * j + 1] + d * N, s, nb, t1, t2, fn, T, u, tru[d]);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) G[d][q] += wj[j] * dt * r[q] * u[q];
+    }
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      if (nb.c[f] < 0) continue;
+      const double rp = trr[f][0], rm = trr[f][1];
+      double avg = 0.0, unp = 0.0, unm = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        avg += 0.5 * (rp * tru[d][f][0] + rm * tru[d][f][1]) * fp.n[f][d];
+        unp += tru[d][f][0] * fp.n[f][d];
+        unm += tru[d][f][1] * fp.n[f][d];
+      }
+      const double lam = fmax(fabs(unp), fabs(unm));
+      F[f] += wj[j] * dt * (avg + 0.5 * lam * (rp - rm));
     }
   }
 #pragma unroll
@@ -63,34 +101,10 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, Tab T, Srcs src, doub
     for (int d = 0; d < DIM; ++d) G[d][q] = R[d];
     V[q] *= lg.jxw(g, T, q);
   }
-  // faces: thread l handles face point l of every face of its cell
   double* Fq = shF + cbs * Lds<DIM, K>::F;
-  if (s.act && s.l < C::NFQ) {
-    for (int f = 0; f < 2 * DIM; ++f) {
-      const int dir = f >> 1, side = f & 1;
-      const int64_t nb = neighbour<DIM>(g, s.ic, f);
-      double F = 0.0;
-      if (nb >= 0) {
-        double n[3], jxw;
-        face_geo<DIM, K>(g, T, s.ic, f, s.l, n, jxw);
-        for (int j = 0; j < NS; ++j) {
-          const double rp = trace<DIM, K>(src.p[2 * j], s.cell, dir, side, s.l, T);
-          const double rm = trace<DIM, K>(src.p[2 * j], nb, dir, 1 - side, s.l, T);
-          double avg = 0.0, unp = 0.0, unm = 0.0;
-          for (int d = 0; d < DIM; ++d) {
-            const double up = trace<DIM, K>(src.p[2 * j + 1] + d * N, s.cell, dir, side, s.l, T);
-            const double um = trace<DIM, K>(src.p[2 * j + 1] + d * N, nb, dir, 1 - side, s.l, T);
-            avg += 0.5 * (rp * up + rm * um) * n[d];
-            unp += up * n[d]; unm += um * n[d];
-          }
-          const double lam = fmax(fabs(unp), fabs(unm));
-          F += wj[j] * dt * (avg + 0.5 * lam * (rp - rm));
-        }
-        F = -F * jxw;
-      }
-      Fq[f * C::NFQ + s.l] = F;
-    }
-  }
+  if (s.act)
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = -F[f] * fp.jxw[f];
   __syncthreads();
   double out[N1];
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
@@ -107,13 +121,15 @@ struct E2Coef { double w[3], wp[3], gw[3]; };
 template <int DIM, int K, int NE, int NG>
 __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, Tab T, Srcs src, E2Coef cf, double dt, double iMa2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F * DIM];
   AFL_SLOT_PTRS
   constexpr int NSYM = DIM * (DIM + 1) / 2;
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
-  double V[DIM][NQ], Sym[NSYM][NQ], P[NQ];
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
+  double V[DIM][NQ], Sym[NSYM][NQ], P[NQ], F[2 * DIM][DIM];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     for (int d = 0; d < DIM; ++d) V[d][q] = 0.0;
@@ -121,12 +137,16 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, Tab T, Srcs src, E2Coef
     P[q] = 0.0;
   }
 #pragma unroll
+  for (int f = 0; f < 2 * DIM; ++f)
+    for (int a = 0; a < DIM; ++a) F[f][a] = 0.0;
+#pragma unroll
   for (int j = 0; j < NE; ++j) {
-    double r[NQ], u[DIM][NQ];
-    eval_line<DIM, K>(src.p[3 * j], s, t1, t2, T, r);
-    for (int d = 0; d < DIM; ++d) eval_line<DIM, K>(src.p[3 * j + 1] + d * N, s, t1, t2, T, u[d]);
-    double pj[NQ];
-    eval_line<DIM, K>(src.p[3 * j + 2], s, t1, t2, T, pj);
+    double r[NQ], u[DIM][NQ], pj[NQ];
+    double trr[2 * DIM][2], tru[DIM][2 * DIM][2], trp[2 * DIM][2];
+    eval_line_faces<DIM, K>(src.p[3 * j], s, nb, t1, t2, fn, T, r, trr);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src.p[3 * j + 1] + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
+    eval_line_faces<DIM, K>(src.p[3 * j + 2], s, nb, t1, t2, fn, T, pj, trp);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       if (j == 0) for (int d = 0; d < DIM; ++d) V[d][q] = r[q] * u[d][q];
@@ -134,6 +154,28 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, Tab T, Srcs src, E2Coef
       for (int a = 0; a < DIM; ++a)
         for (int b = a; b < DIM; ++b) Sym[m++][q] += cf.w[j] * dt * r[q] * u[a][q] * u[b][q];
       P[q] += cf.wp[j] * dt * iMa2 * pj[q];
+    }
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      const double pp = trp[f][0];
+      if (nb.c[f] >= 0) {
+        const double rp = trr[f][0], rm = trr[f][1], pm = trp[f][1];
+        double unp = 0.0, unm = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { unp += tru[d][f][0] * fp.n[f][d]; unm += tru[d][f][1] * fp.n[f][d]; }
+        const double lam = fmax(fabs(unp), fabs(unm));
+        const double avg_p = 0.5 * (pp + pm);
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          const double avg_tn = 0.5 * (rp * tru[a][f][0] * unp + rm * tru[a][f][1] * unm);
+          const double jump = rp * tru[a][f][0] - rm * tru[a][f][1];
+          F[f][a] += cf.w[j] * dt * avg_tn + cf.wp[j] * dt * iMa2 * avg_p * fp.n[f][a] + cf.w[j] * dt * 0.5 * lam * jump;
+        }
+      } else {
+        // wall: p_D = p, flux (p + p_D)/2 n (euler_operator.h:2213-2218)
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) F[f][a] += cf.wp[j] * dt * iMa2 * (0.5 * (pp + pp)) * fp.n[f][a];
+      }
     }
   }
 #pragma unroll
@@ -143,43 +185,12 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, Tab T, Srcs src, E2Coef
 #pragma unroll
     for (int q = 0; q < NQ; ++q) V[DIM - 1][q] -= cf.gw[j] * dt * iFr2 * r[q];
   }
-  // faces (all components)
   double* Fq = shF + cbs * Lds<DIM, K>::F * DIM;
-  if (s.act && s.l < C::NFQ) {
-    for (int f = 0; f < 2 * DIM; ++f) {
-      const int dir = f >> 1, side = f & 1;
-      const int64_t nb = neighbour<DIM>(g, s.ic, f);
-      double n[3], jxw;
-      face_geo<DIM, K>(g, T, s.ic, f, s.l, n, jxw);
-      double F[DIM];
-      for (int a = 0; a < DIM; ++a) F[a] = 0.0;
-      for (int j = 0; j < NE; ++j) {
-        const double pp = trace<DIM, K>(src.p[3 * j + 2], s.cell, dir, side, s.l, T);
-        if (nb >= 0) {
-          const double rp = trace<DIM, K>(src.p[3 * j], s.cell, dir, side, s.l, T);
-          const double rm = trace<DIM, K>(src.p[3 * j], nb, dir, 1 - side, s.l, T);
-          const double pm = trace<DIM, K>(src.p[3 * j + 2], nb, dir, 1 - side, s.l, T);
-          double up[DIM], um[DIM], unp = 0.0, unm = 0.0;
-          for (int d = 0; d < DIM; ++d) {
-            up[d] = trace<DIM, K>(src.p[3 * j + 1] + d * N, s.cell, dir, side, s.l, T);
-            um[d] = trace<DIM, K>(src.p[3 * j + 1] + d * N, nb, dir, 1 - side, s.l, T);
-            unp += up[d] * n[d]; unm += um[d] * n[d];
-          }
-          const double lam = fmax(fabs(unp), fabs(unm));
-          const double avg_p = 0.5 * (pp + pm);
-          for (int a = 0; a < DIM; ++a) {
-            const double avg_tn = 0.5 * (rp * up[a] * unp + rm * um[a] * unm);
-            const double jump = rp * up[a] - rm * um[a];
-            F[a] += cf.w[j] * dt * avg_tn + cf.wp[j] * dt * iMa2 * avg_p * n[a] + cf.w[j] * dt * 0.5 * lam * jump;
-          }
-        } else {
-          // wall: p_D = p, flux (p + p_D)/2 n (euler_operator.h:2213-2218)
-          for (int a = 0; a < DIM; ++a) F[a] += cf.wp[j] * dt * iMa2 * (0.5 * (pp + pp)) * n[a];
-        }
-      }
-      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + f) * C::NFQ + s.l] = -F[a] * jxw;
-    }
-  }
+  if (s.act)
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + f) * C::NFQ + s.l] = -F[f][a] * fp.jxw[f];
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -212,21 +223,33 @@ struct E3Coef { double w[3], wt[3], gw[3]; };
 template <int DIM, int K, int NE, int NG, int STAGE>
 __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, Tab T, Srcs src, E3Coef cf, double dt, double Ma2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   AFL_SLOT_PTRS
   constexpr double gm1 = kGamma - 1.0;
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
-  double V[NQ], G[DIM][NQ];
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
+  double V[NQ], G[DIM][NQ], F[2 * DIM];
+  double tu0[DIM][2 * DIM][2];  // u_old traces (stage-2 kinetic jump as written)
 #pragma unroll
   for (int q = 0; q < NQ; ++q) { V[q] = 0.0; for (int d = 0; d < DIM; ++d) G[d][q] = 0.0; }
 #pragma unroll
+  for (int f = 0; f < 2 * DIM; ++f) F[f] = 0.0;
+#pragma unroll
   for (int j = 0; j < NE; ++j) {
     double r[NQ], u[DIM][NQ], p[NQ];
-    eval_line<DIM, K>(src.p[3 * j], s, t1, t2, T, r);
-    for (int d = 0; d < DIM; ++d) eval_line<DIM, K>(src.p[3 * j + 1] + d * N, s, t1, t2, T, u[d]);
-    eval_line<DIM, K>(src.p[3 * j + 2], s, t1, t2, T, p);
+    double trr[2 * DIM][2], tru[DIM][2 * DIM][2], trp[2 * DIM][2];
+    eval_line_faces<DIM, K>(src.p[3 * j], s, nb, t1, t2, fn, T, r, trr);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src.p[3 * j + 1] + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
+    eval_line_faces<DIM, K>(src.p[3 * j + 2], s, nb, t1, t2, fn, T, p, trp);
+    if (STAGE == 2 && j == 0)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) { tu0[d][f][0] = tru[d][f][0]; tu0[d][f][1] = tru[d][f][1]; }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       double uu = 0.0;
@@ -236,6 +259,43 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, Tab T, Srcs src, E3Coef
       const double kin = 0.5 * cf.w[j] * dt * Ma2 * uu * r[q];
       const double ent = cf.wt[j] * dt * (r[q] * (E - 0.5 * Ma2 * uu) + p[q]);
       for (int d = 0; d < DIM; ++d) G[d][q] += (kin + ent) * u[d][q];
+    }
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      if (nb.c[f] < 0) continue;
+      const double rp = trr[f][0], rm = trr[f][1], pp = trp[f][0], pm = trp[f][1];
+      double uup = 0.0, uum = 0.0, unp = 0.0, unm = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        uup += tru[d][f][0] * tru[d][f][0]; uum += tru[d][f][1] * tru[d][f][1];
+        unp += tru[d][f][0] * fp.n[f][d]; unm += tru[d][f][1] * fp.n[f][d];
+      }
+      const double Ep = 1.0 / gm1 * pp / rp + 0.5 * Ma2 * uup;
+      const double Em = 1.0 / gm1 * pm / rm + 0.5 * Ma2 * uum;
+      double avg_kin = 0.0, avg_ent = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        avg_kin += 0.5 * (0.5 * uup * rp * tru[d][f][0] + 0.5 * uum * rm * tru[d][f][1]) * fp.n[f][d];
+        avg_ent += 0.5 * (((Ep - 0.5 * Ma2 * uup) * rp + pp) * tru[d][f][0] +
+                          ((Em - 0.5 * Ma2 * uum) * rm + pm) * tru[d][f][1]) * fp.n[f][d];
+      }
+      const double lam = fmax(fabs(unp), fabs(unm));
+      if (STAGE == 3) {
+        const double jump = rp * Ep - rm * Em;
+        F[f] += cf.w[j] * dt * Ma2 * avg_kin + cf.w[j] * dt * avg_ent + cf.w[j] * dt * 0.5 * lam * jump;
+      } else {
+        double jk;
+        if (STAGE == 2 && j == 1) {  // euler_operator.h:1182-1183, u_tmp_2 . u_old as written
+          double d0p = 0.0, d0m = 0.0;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) { d0p += tru[d][f][0] * tu0[d][f][0]; d0m += tru[d][f][1] * tu0[d][f][1]; }
+          jk = rp * 0.5 * d0p - rm * 0.5 * d0m;
+        } else {
+          jk = rp * 0.5 * uup - rm * 0.5 * uum;
+        }
+        const double je = rp * (Ep - 0.5 * Ma2 * uup) - rm * (Em - 0.5 * Ma2 * uum);
+        F[f] += cf.w[j] * dt * Ma2 * (avg_kin + 0.5 * lam * jk) + cf.wt[j] * dt * (avg_ent + 0.5 * lam * je);
+      }
     }
   }
   {
@@ -268,61 +328,9 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, Tab T, Srcs src, E3Coef
     V[q] *= lg.jxw(g, T, q);
   }
   double* Fq = shF + cbs * Lds<DIM, K>::F;
-  if (s.act && s.l < C::NFQ) {
-    for (int f = 0; f < 2 * DIM; ++f) {
-      const int dir = f >> 1, side = f & 1;
-      const int64_t nb = neighbour<DIM>(g, s.ic, f);
-      double F = 0.0;
-      if (nb >= 0) {
-        double n[3], jxw;
-        face_geo<DIM, K>(g, T, s.ic, f, s.l, n, jxw);
-        double u0p[DIM], u0m[DIM];
-        if (STAGE == 2)
-          for (int d = 0; d < DIM; ++d) {
-            u0p[d] = trace<DIM, K>(src.p[1] + d * N, s.cell, dir, side, s.l, T);
-            u0m[d] = trace<DIM, K>(src.p[1] + d * N, nb, dir, 1 - side, s.l, T);
-          }
-        for (int j = 0; j < NE; ++j) {
-          const double rp = trace<DIM, K>(src.p[3 * j], s.cell, dir, side, s.l, T);
-          const double rm = trace<DIM, K>(src.p[3 * j], nb, dir, 1 - side, s.l, T);
-          const double pp = trace<DIM, K>(src.p[3 * j + 2], s.cell, dir, side, s.l, T);
-          const double pm = trace<DIM, K>(src.p[3 * j + 2], nb, dir, 1 - side, s.l, T);
-          double up[DIM], um[DIM], uup = 0.0, uum = 0.0, unp = 0.0, unm = 0.0;
-          for (int d = 0; d < DIM; ++d) {
-            up[d] = trace<DIM, K>(src.p[3 * j + 1] + d * N, s.cell, dir, side, s.l, T);
-            um[d] = trace<DIM, K>(src.p[3 * j + 1] + d * N, nb, dir, 1 - side, s.l, T);
-            uup += up[d] * up[d]; uum += um[d] * um[d];
-            unp += up[d] * n[d]; unm += um[d] * n[d];
-          }
-          const double Ep = 1.0 / gm1 * pp / rp + 0.5 * Ma2 * uup;
-          const double Em = 1.0 / gm1 * pm / rm + 0.5 * Ma2 * uum;
-          double avg_kin = 0.0, avg_ent = 0.0;
-          for (int d = 0; d < DIM; ++d) {
-            avg_kin += 0.5 * (0.5 * uup * rp * up[d] + 0.5 * uum * rm * um[d]) * n[d];
-            avg_ent += 0.5 * (((Ep - 0.5 * Ma2 * uup) * rp + pp) * up[d] + ((Em - 0.5 * Ma2 * uum) * rm + pm) * um[d]) * n[d];
-          }
-          const double lam = fmax(fabs(unp), fabs(unm));
-          if (STAGE == 3) {
-            const double jump = rp * Ep - rm * Em;
-            F += cf.w[j] * dt * Ma2 * avg_kin + cf.w[j] * dt * avg_ent + cf.w[j] * dt * 0.5 * lam * jump;
-          } else {
-            double jk;
-            if (STAGE == 2 && j == 1) {  // euler_operator.h:1182-1183, u_tmp_2 . u_old as written
-              double d0p = 0.0, d0m = 0.0;
-              for (int d = 0; d < DIM; ++d) { d0p += up[d] * u0p[d]; d0m += um[d] * u0m[d]; }
-              jk = rp * 0.5 * d0p - rm * 0.5 * d0m;
-            } else {
-              jk = rp * 0.5 * uup - rm * 0.5 * uum;
-            }
-            const double je = rp * (Ep - 0.5 * Ma2 * uup) - rm * (Em - 0.5 * Ma2 * uum);
-            F += cf.w[j] * dt * Ma2 * (avg_kin + 0.5 * lam * jk) + cf.wt[j] * dt * (avg_ent + 0.5 * lam * je);
-          }
-        }
-        F = -F * jxw;
-      }
-      Fq[f * C::NFQ + s.l] = F;
-    }
-  }
+  if (s.act)
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = -F[f] * fp.jxw[f];
   __syncthreads();
   double out[N1];
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
@@ -337,7 +345,7 @@ template <int DIM, int K>
 __global__ __launch_bounds__(AFL_NT) void k_mass_apply(Geo g, Tab T, const double* __restrict__ rho, double scale,
                                                         int ncomp, int64_t N, const double* __restrict__ src,
                                                         double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   AFL_SLOT_PTRS
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
@@ -360,7 +368,7 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_apply(Geo g, Tab T, const doubl
 template <int DIM, int K>
 __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, Tab T, const double* __restrict__ rho, double scale,
                                                       int ncomp, int64_t N, double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   AFL_SLOT_PTRS
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
@@ -415,25 +423,22 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, Tab T, const double*
 
 // ---------------------------------------------------------------------------------------
 // B: vmult_pressure (euler_operator.h:1510-1617). Volume -f p div v, faces +f {p} n . v
-// (own side), walls +f p n . v, with f = coeff dt / Ma^2.
+// (own side), walls +f p n . v, with f = coeff dt / Ma^2. fn: face-node LDS of this cell.
 template <int DIM, int K>
 __device__ inline void apply_B_rows(const Geo& g, const Tab& T, const Slot<DIM, K>& s, const LineGeo<DIM, K>& lg,
-                                    const double* __restrict__ p, double f, double* t1, double* t2, double* i1,
-                                    double* i2, double* Fq, double (&out)[DIM][K + 1]) {
+                                    const Nbrs<DIM>& nb, const FacePts<DIM, K>& fp, const double* __restrict__ p,
+                                    double f, double* t1, double* t2, double* i1, double* i2, double* fn, double* Fq,
+                                    double (&out)[DIM][K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ;
-  double pq[NQ];
-  eval_line<DIM, K>(p, s, t1, t2, T, pq);
-  if (s.act && s.l < C::NFQ) {
+  double pq[NQ], trp[2 * DIM][2];
+  eval_line_faces<DIM, K>(p, s, nb, t1, t2, fn, T, pq, trp);
+  if (s.act) {
+#pragma unroll
     for (int fc = 0; fc < 2 * DIM; ++fc) {
-      const int dir = fc >> 1, side = fc & 1;
-      const int64_t nb = neighbour<DIM>(g, s.ic, fc);
-      double n[3], jxw;
-      face_geo<DIM, K>(g, T, s.ic, fc, s.l, n, jxw);
-      const double pp = trace<DIM, K>(p, s.cell, dir, side, s.l, T);
-      const double pm = nb >= 0 ? trace<DIM, K>(p, nb, dir, 1 - side, s.l, T) : pp;
-      const double avg = 0.5 * (pp + pm);
-      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + fc) * C::NFQ + s.l] = f * avg * n[a] * jxw;
+      const double avg = 0.5 * (trp[fc][0] + trp[fc][1]);  // walls: trp[fc][1] == trp[fc][0] (p_D = p)
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + fc) * C::NFQ + s.l] = f * avg * fp.n[fc][a] * fp.jxw[fc];
     }
   }
   __syncthreads();
@@ -455,13 +460,16 @@ __device__ inline void apply_B_rows(const Geo& g, const Tab& T, const Slot<DIM, 
 template <int DIM, int K>
 __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, Tab T, double f, int64_t N, const double* __restrict__ p,
                                                      double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F * DIM];
   AFL_SLOT_PTRS
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
   double out[DIM][N1];
-  apply_B_rows<DIM, K>(g, T, s, lg, p, f, t1, t2, i1, i2, shF + cbs * Lds<DIM, K>::F * DIM, out);
+  apply_B_rows<DIM, K>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, shF + cbs * Lds<DIM, K>::F * DIM, out);
+#pragma unroll
   for (int a = 0; a < DIM; ++a) store_row<DIM, K>(dst + a * N, s, out[a]);
 }
 
@@ -473,18 +481,22 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, Tab T, double cdt, in
                                                      const double* __restrict__ pfix, const double* __restrict__ w,
                                                      const double* __restrict__ pD, double scaleD,
                                                      double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   AFL_SLOT_PTRS
   constexpr double gr = kGamma / (kGamma - 1.0), igm1 = 1.0 / (kGamma - 1.0);
   const double sgn = WITH_D ? -1.0 : 1.0;
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
   double pq[NQ], G[DIM][NQ], V[NQ];
-  eval_line<DIM, K>(pfix, s, t1, t2, T, pq);
+  double trp[2 * DIM][2], trw[DIM][2 * DIM][2];
+  eval_line_faces<DIM, K>(pfix, s, nb, t1, t2, fn, T, pq, trp);
+#pragma unroll
   for (int d = 0; d < DIM; ++d) {
     double wq[NQ];
-    eval_line<DIM, K>(w + d * N, s, t1, t2, T, wq);
+    eval_line_faces<DIM, K>(w + d * N, s, nb, t1, t2, fn, T, wq, trw[d]);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) G[d][q] = sgn * (-cdt * gr * pq[q] * wq[q]);
   }
@@ -501,25 +513,21 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, Tab T, double cdt, in
     for (int d = 0; d < DIM; ++d) G[d][q] = R[d];
   }
   double* Fq = shF + cbs * Lds<DIM, K>::F;
-  if (s.act && s.l < C::NFQ) {
+  if (s.act) {
+#pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
-      const int dir = f >> 1, side = f & 1;
-      const int64_t nb = neighbour<DIM>(g, s.ic, f);
       double F = 0.0;
-      if (nb >= 0) {
-        double n[3], jxw;
-        face_geo<DIM, K>(g, T, s.ic, f, s.l, n, jxw);
-        const double pp = trace<DIM, K>(pfix, s.cell, dir, side, s.l, T);
-        const double pm = trace<DIM, K>(pfix, nb, dir, 1 - side, s.l, T);
+      if (nb.c[f] >= 0) {
+        const double pp = trp[f][0], pm = trp[f][1];
         double avg = 0.0, wnp = 0.0, wnm = 0.0;
+#pragma unroll
         for (int d = 0; d < DIM; ++d) {
-          const double wp = trace<DIM, K>(w + d * N, s.cell, dir, side, s.l, T);
-          const double wm = trace<DIM, K>(w + d * N, nb, dir, 1 - side, s.l, T);
-          avg += 0.5 * gr * (pp * wp + pm * wm) * n[d];
-          wnp += wp * n[d]; wnm += wm * n[d];
+          avg += 0.5 * gr * (pp * trw[d][f][0] + pm * trw[d][f][1]) * fp.n[f][d];
+          wnp += trw[d][f][0] * fp.n[f][d];
+          wnm += trw[d][f][1] * fp.n[f][d];
         }
         const double lam = fmax(fabs(wnp), fabs(wnm));
-        F = sgn * cdt * (avg + 0.5 * lam * igm1 * (pp - pm)) * jxw;
+        F = sgn * cdt * (avg + 0.5 * lam * igm1 * (pp - pm)) * fp.jxw[f];
       }
       Fq[f * C::NFQ + s.l] = F;
     }
@@ -532,182 +540,236 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, Tab T, double cdt, in
 }
 
 // ---------------------------------------------------------------------------------------
-// Cell-local preconditioned CG for the block-diagonal mass-type matrices (exact inverse up to
-// the stopping tolerance). Matrix: sum_q co(q) phi_i phi_j with co = scale rho JxW on the
-// thread's line; preconditioner: (MI x MI (x MI)) / mean(co / w_ref). b rows in, x rows out.
-template <int DIM, int K, int NC>
-__device__ inline void cell_pcg(const Geo& g, const Tab& T, const Slot<DIM, K>& s, const double (&co)[2 * K + 1],
-                                double* cell_lds /* NPC */, double* t1, double* t2, double* i1, double* i2,
-                                double* red /* CPB*ROWS */, int* flag, double tol2, int maxit,
-                                const double (&b)[NC][K + 1], double (&x)[NC][K + 1]) {
+// Cell-local solver for the block-diagonal mass-type matrices A = sum_q co(q) phi_i phi_j
+// (co = scale rho JxW): Chebyshev iteration (Saad, Iterative Methods, Alg. 12.1) preconditioned
+// with P = mean(co) * (M1 x M1 (x M1)), M1 the reference 1D mass matrix. The spectrum of
+// P^{-1} A lies in [min_q co/(mean w_q), max_q co/(mean w_q)] (w_q reference weights), so the
+// interval and the iteration count for the requested A-norm tolerance (2 sigma^n <= tol) are
+// known per cell up front: no inner products, no convergence flags. All NC components share
+// every barrier. b rows in, x rows out.
+template <int DIM, int K>
+struct SolvLds {
   using C = Cfg<DIM, K>;
-  constexpr int NQ = C::NQ, N1 = C::N1, ROWS = C::ROWS;
+  static constexpr int EVF = C::EV1 + C::EV2;
+  static constexpr int INF = IntV<DIM, K>::I1 + IntV<DIM, K>::I2;
+  static constexpr int WF = EVF > INF ? EVF : INF;  // per component work area
+  static constexpr int PER = WF + C::NPC;            // + the component's dofs
+};
+
+template <int DIM, int K, int NC>
+__device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const LineGeo<DIM, K>& lg,
+                                 const double (&co)[2 * K + 1], double* sl /* NC*PER per cell */,
+                                 double tol, int maxit, const double (&b)[NC][K + 1], double (&x)[NC][K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int NQ = C::NQ, N1 = C::N1, ROWS = C::ROWS, NPC = C::NPC;
+  constexpr int WF = SolvLds<DIM, K>::WF;
+  double* dl = sl;                 // NC*NPC dofs
+  double* wk = sl + NC * NPC;      // NC*WF work
+  double* t1 = wk;
+  double* t2 = wk + NC * C::EV1;
+  double* i1 = wk;
+  double* i2 = wk + NC * IntV<DIM, K>::I1;
   const bool row_owner = s.act && s.l < ROWS;
-  double* rd = red + (s.cb < C::CPB ? s.cb : 0) * ROWS;
-  // mean coefficient relative to the reference weights: sum_q co / sum_q w_ref = sum_q co
-  // (reference weights sum to 1); each line sums its own points, rows reduce.
-  double lsum = 0.0;
+  __shared__ double red[3 * AFL_NT];
+  __shared__ int nit_blk;
+  // cell mean and spectral bounds of P^{-1}A
+  {
+    double lsum = 0.0, lmin = 1e300, lmax = 0.0;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) lsum += co[q];
-  // reduce over the LT lines of the cell through LDS
-  __shared__ double red_lines[AFL_NT];
-  red_lines[threadIdx.x] = lsum;
-  __syncthreads();
-  double mean = 1.0;
-  if (s.act) {
-    double a = 0.0;
-    const int base = s.cb * C::LT;
-    for (int t = 0; t < C::LT; ++t) a += red_lines[base + t];
-    mean = a;
+    for (int q = 0; q < NQ; ++q) {
+      const double r = co[q] / (lg.wxy * T.W[q]);
+      lsum += co[q];
+      lmin = fmin(lmin, r);
+      lmax = fmax(lmax, r);
+    }
+    red[threadIdx.x] = lsum;
+    red[AFL_NT + threadIdx.x] = lmin;
+    red[2 * AFL_NT + threadIdx.x] = lmax;
+    if (threadIdx.x == 0) nit_blk = 1;
   }
   __syncthreads();
+  double mean = 1.0, cmin = 1.0, cmax = 1.0;
+  if (s.act) {
+    const int base = s.cb * C::LT;
+    double a = 0.0, mn = 1e300, mx = 0.0;
+    for (int t = 0; t < C::LT; ++t) {
+      a += red[base + t];
+      mn = fmin(mn, red[AFL_NT + base + t]);
+      mx = fmax(mx, red[2 * AFL_NT + base + t]);
+    }
+    mean = a; cmin = mn; cmax = mx;
+  }
+  const double alpha = cmin / mean, beta = cmax / mean;
+  const double theta = 0.5 * (alpha + beta);
+  const double delta = fmax(0.5 * (beta - alpha), 1e-14 * theta);
+  const double sigma1 = theta / delta;
+  {
+    const double sk = sqrt(beta / alpha);
+    const double sg = (sk - 1.0) / (sk + 1.0);
+    int n = 1;
+    if (sg > 1e-300) n = (int)ceil(log(0.5 * tol) / log(sg));
+    n = n < 1 ? 1 : (n > maxit ? maxit : n);
+    if (s.act && s.l == 0) atomicMax(&nit_blk, n);
+  }
+  __syncthreads();
+  const int nit = nit_blk;
   const double pinv = 1.0 / mean;
 
-  // per-cell reduction of row partial sums; result identical for all threads of the cell
-  auto cell_sum = [&](double part) -> double {
-    if (row_owner) rd[s.l] = part;
-    __syncthreads();
-    double a = 0.0;
-    if (s.act) for (int t = 0; t < ROWS; ++t) a += rd[t];
-    __syncthreads();
-    return a;
-  };
-  // z = P^{-1} r for one component (rows in registers), tensor-product MI sweeps through LDS
-  auto precond = [&](const double (&r)[N1], double (&z)[N1]) {
-    double tmp[N1];
+  // z = P^{-1} r for all components (rows in registers)
+  auto precond = [&](const double (&r)[NC][N1], double (&z)[NC][N1]) {
     if (row_owner) {
-      for (int i = 0; i < N1; ++i) {
-        double a = 0.0;
-        for (int j = 0; j < N1; ++j) a += T.MI[i][j] * r[j];
-        tmp[i] = a;
-      }
-      for (int i = 0; i < N1; ++i) cell_lds[s.l * N1 + i] = tmp[i];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * r[c][j];
+          dl[c * NPC + s.l * N1 + i] = a;
+        }
     }
     __syncthreads();
     if (DIM == 2) {
-      // along z: thread l < N1 handles column ix = l
       if (s.act && s.l < N1) {
-        double c[N1];
-        for (int j = 0; j < N1; ++j) c[j] = cell_lds[j * N1 + s.l];
-        for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
-          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * c[j];
-          t1[i * N1 + s.l] = a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double v[N1];
+#pragma unroll
+          for (int j = 0; j < N1; ++j) v[j] = dl[c * NPC + j * N1 + s.l];
+#pragma unroll
+          for (int i = 0; i < N1; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+            t1[c * NPC + i * N1 + s.l] = a;
+          }
         }
       }
       __syncthreads();
-      if (row_owner) for (int i = 0; i < N1; ++i) z[i] = pinv * t1[s.l * N1 + i];
-      __syncthreads();
+      if (row_owner)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int i = 0; i < N1; ++i) z[c][i] = pinv * t1[c * NPC + s.l * N1 + i];
     } else {
-      // along y: thread (ix, iz) = (l % N1, l / N1)
       if (s.act && s.l < N1 * N1) {
         const int ix = s.l % N1, iz = s.l / N1;
-        double c[N1];
-        for (int j = 0; j < N1; ++j) c[j] = cell_lds[(iz * N1 + j) * N1 + ix];
-        for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
-          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * c[j];
-          t1[(iz * N1 + i) * N1 + ix] = a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double v[N1];
+#pragma unroll
+          for (int j = 0; j < N1; ++j) v[j] = dl[c * NPC + (iz * N1 + j) * N1 + ix];
+#pragma unroll
+          for (int i = 0; i < N1; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+            t1[c * NPC + (iz * N1 + i) * N1 + ix] = a;
+          }
         }
       }
       __syncthreads();
-      // along z: thread (ix, iy)
       if (s.act && s.l < N1 * N1) {
         const int ix = s.l % N1, iy = s.l / N1;
-        double c[N1];
-        for (int j = 0; j < N1; ++j) c[j] = t1[(j * N1 + iy) * N1 + ix];
-        for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
-          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * c[j];
-          cell_lds[(i * N1 + iy) * N1 + ix] = a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double v[N1];
+#pragma unroll
+          for (int j = 0; j < N1; ++j) v[j] = t1[c * NPC + (j * N1 + iy) * N1 + ix];
+#pragma unroll
+          for (int i = 0; i < N1; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+            dl[c * NPC + (i * N1 + iy) * N1 + ix] = a;
+          }
         }
       }
       __syncthreads();
-      if (row_owner) for (int i = 0; i < N1; ++i) z[i] = pinv * cell_lds[s.l * N1 + i];
-      __syncthreads();
-    }
-  };
-  // y = A v for one component: v rows -> LDS -> evaluate -> * co -> integrate
-  auto apply = [&](const double (&v)[N1], double (&y)[N1]) {
-    if (row_owner) for (int i = 0; i < N1; ++i) cell_lds[s.l * N1 + i] = v[i];
-    __syncthreads();
-    double vq[NQ], G[DIM][NQ];
-    eval_line_lds<DIM, K>(cell_lds, s, t1, t2, T, vq);
+      if (row_owner)
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) vq[q] *= co[q];
-    integrate_line<DIM, K, true, false>(s, i1, i2, T, vq, G, y);
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int i = 0; i < N1; ++i) z[c][i] = pinv * dl[c * NPC + s.l * N1 + i];
+    }
+    __syncthreads();
+  };
+  // y = A v for all components
+  auto apply = [&](const double (&v)[NC][N1], double (&y)[NC][N1]) {
+    if (row_owner)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) dl[c * NPC + s.l * N1 + i] = v[c][i];
+    __syncthreads();
+    double vq[NC][NQ];
+    eval_lines_lds<DIM, K, NC>(dl, s, t1, t2, T, vq);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) vq[c][q] *= co[q];
+    integrate_lines_v<DIM, K, NC>(s, i1, i2, T, vq, y);
   };
 
-  double r[NC][N1], z[NC][N1], p[NC][N1], ap[N1];
-  double rz = 0.0, bb = 0.0;
-  for (int c = 0; c < NC; ++c) {
+  double r[NC][N1], z[NC][N1], d[NC][N1];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
     for (int i = 0; i < N1; ++i) { x[c][i] = 0.0; r[c][i] = row_owner ? b[c][i] : 0.0; }
-    precond(r[c], z[c]);
-    double prz = 0.0, pbb = 0.0;
-    if (row_owner) for (int i = 0; i < N1; ++i) { p[c][i] = z[c][i]; prz += r[c][i] * z[c][i]; pbb += b[c][i] * b[c][i]; }
-    rz += cell_sum(prz);
-    bb += cell_sum(pbb);
-  }
-  bool done = !(bb > 0.0);
-  for (int it = 0; it < maxit; ++it) {
-    // block-wide termination check
-    if (threadIdx.x == 0) *flag = 0;
-    __syncthreads();
-    if (s.act && !done) *flag = 1;
-    __syncthreads();
-    if (*flag == 0) break;
-    double pap = 0.0;
-    double apv[NC][N1];
-    for (int c = 0; c < NC; ++c) {
-      apply(p[c], ap);
-      double part = 0.0;
-      if (row_owner) for (int i = 0; i < N1; ++i) { apv[c][i] = ap[i]; part += p[c][i] * ap[i]; }
-      pap += cell_sum(part);
-    }
-    const double alpha = (done || pap == 0.0) ? 0.0 : rz / pap;
-    double rr = 0.0;
-    for (int c = 0; c < NC; ++c) {
-      double part = 0.0;
-      if (row_owner)
-        for (int i = 0; i < N1; ++i) { x[c][i] += alpha * p[c][i]; r[c][i] -= alpha * apv[c][i]; part += r[c][i] * r[c][i]; }
-      rr += cell_sum(part);
-    }
-    if (rr <= tol2 * bb) done = true;
-    double rz_new = 0.0;
-    for (int c = 0; c < NC; ++c) {
-      precond(r[c], z[c]);
-      double part = 0.0;
-      if (row_owner) for (int i = 0; i < N1; ++i) part += r[c][i] * z[c][i];
-      rz_new += cell_sum(part);
-    }
-    const double beta = (done || rz == 0.0) ? 0.0 : rz_new / rz;
-    rz = rz_new;
+  precond(r, z);
+  double rho = 1.0 / sigma1;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int i = 0; i < N1; ++i) d[c][i] = z[c][i] / theta;
+  for (int k = 0; k < nit; ++k) {
+#pragma unroll
     for (int c = 0; c < NC; ++c)
-      if (row_owner) for (int i = 0; i < N1; ++i) p[c][i] = z[c][i] + beta * p[c][i];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) x[c][i] += d[c][i];
+    if (k == nit - 1) break;
+    double ad[NC][N1];
+    apply(d, ad);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) r[c][i] -= ad[c][i];
+    precond(r, z);
+    const double rn = 1.0 / (2.0 * sigma1 - rho);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) d[c][i] = rn * rho * d[c][i] + (2.0 * rn / delta) * z[c][i];
+    rho = rn;
   }
 }
 
 template <int DIM, int K, int NC>
 __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const double* __restrict__ rho, double scale,
                                                         int64_t N, const double* __restrict__ b,
-                                                        double* __restrict__ x, double tol2, int maxit) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
-  __shared__ double shc[Cfg<DIM, K>::CPB * Cfg<DIM, K>::NPC];
-  __shared__ double shr[Cfg<DIM, K>::CPB * Cfg<DIM, K>::ROWS];
-  __shared__ int flag;
-  AFL_SLOT_PTRS
+                                                        double* __restrict__ x, double tol, int maxit) {
+  constexpr int PC = Lds<DIM, K>::W > NC * SolvLds<DIM, K>::PER ? Lds<DIM, K>::W : NC * SolvLds<DIM, K>::PER;
+  __shared__ double shw[Cfg<DIM, K>::CPB * PC];
+  using C = Cfg<DIM, K>;
+  constexpr int NQ = C::NQ, N1 = C::N1;
+  Slot<DIM, K> s(g);
+  const int cbs = s.cb < C::CPB ? s.cb : 0;
+  double* base = shw + cbs * PC;
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
   double co[NQ];
-  if (rho) eval_line<DIM, K>(rho, s, t1, t2, T, co);
+  if (rho) eval_line<DIM, K>(rho, s, base, base + C::EV1, T, co);
   else for (int q = 0; q < NQ; ++q) co[q] = 1.0;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) co[q] *= scale * lg.jxw(g, T, q);
   double bb[NC][N1], xx[NC][N1];
+#pragma unroll
   for (int c = 0; c < NC; ++c)
+#pragma unroll
     for (int i = 0; i < N1; ++i)
       bb[c][i] = (s.act && s.l < C::ROWS) ? b[c * N + s.cell * C::NPC + s.l * N1 + i] : 0.0;
-  cell_pcg<DIM, K, NC>(g, T, s, co, shc + cbs * C::NPC, t1, t2, i1, i2, shr, &flag, tol2, maxit, bb, xx);
+  cell_cheb<DIM, K, NC>(T, s, lg, co, base, tol, maxit, bb, xx);
+#pragma unroll
   for (int c = 0; c < NC; ++c) store_row<DIM, K>(x + c * N, s, xx[c]);
 }
 
@@ -715,23 +777,35 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const doubl
 template <int DIM, int K>
 __global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, Tab T, const double* __restrict__ rho, double f,
                                                      int64_t N, const double* __restrict__ p, double* __restrict__ w,
-                                                     double tol2, int maxit) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::W];
-  __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F * DIM];
-  __shared__ double shc[Cfg<DIM, K>::CPB * Cfg<DIM, K>::NPC];
-  __shared__ double shr[Cfg<DIM, K>::CPB * Cfg<DIM, K>::ROWS];
-  __shared__ int flag;
-  AFL_SLOT_PTRS
+                                                     double tol, int maxit) {
+  constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;  // B phase: work + face nodes + face fluxes
+  constexpr int PS = DIM * SolvLds<DIM, K>::PER;               // solver phase
+  constexpr int PC = PB > PS ? PB : PS;
+  __shared__ double shw[Cfg<DIM, K>::CPB * PC];
+  using C = Cfg<DIM, K>;
+  constexpr int NQ = C::NQ, N1 = C::N1;
+  Slot<DIM, K> s(g);
+  const int cbs = s.cb < C::CPB ? s.cb : 0;
+  double* base = shw + cbs * PC;
+  double* t1 = base;
+  double* t2 = base + C::EV1;
+  double* i1 = base;
+  double* i2 = base + C::IN1;
+  double* fn = base + Lds<DIM, K>::W;
+  double* Fq = base + Lds<DIM, K>::WF;
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
   double bp[DIM][N1];
-  apply_B_rows<DIM, K>(g, T, s, lg, p, f, t1, t2, i1, i2, shF + cbs * Lds<DIM, K>::F * DIM, bp);
+  apply_B_rows<DIM, K>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, Fq, bp);
   double co[NQ];
   eval_line<DIM, K>(rho, s, t1, t2, T, co);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) co[q] *= lg.jxw(g, T, q);
   double xx[DIM][N1];
-  cell_pcg<DIM, K, DIM>(g, T, s, co, shc + cbs * C::NPC, t1, t2, i1, i2, shr, &flag, tol2, maxit, bp, xx);
+  cell_cheb<DIM, K, DIM>(T, s, lg, co, base, tol, maxit, bp, xx);
+#pragma unroll
   for (int c = 0; c < DIM; ++c) store_row<DIM, K>(w + c * N, s, xx[c]);
 }
 
@@ -868,7 +942,7 @@ void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const d
 }
 
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x) {
-  const double tol2 = o.cell_tol * o.cell_tol;
+  const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
     if (ncomp == 1)
       k_mass_solve<D_, K_, 1><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, scale, o.N, b, x, tol2, o.cell_maxit);
@@ -880,7 +954,7 @@ void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncom
 void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* w) {
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double f = coeff * o.dt / (o.Ma * o.Ma);
-  const double tol2 = o.cell_tol * o.cell_tol;
+  const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
     k_schur_w<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, f, o.N, p, w, tol2, o.cell_maxit);
   });
