@@ -10,6 +10,7 @@
 
 #include "../../include/afl.h"
 #include "afl_internal.h"
+#include "afl_mg.h"
 
 using namespace afl;
 
@@ -171,7 +172,14 @@ struct DevBuf {
 struct afl_operator {
   OpCtx o;
   unsigned ns = 1;
-  afl_solver_params sp{1e-8, 30000, 28, 1e-10, 100, 0};
+  afl_solver_params sp{1e-8, 30000, 28, 1e-10, 100, 0, AFL_SCHUR_JACOBI, 3, 16, 1, 15.0, 0.05};
+  // p-multigrid preconditioner state
+  MgT<float> mg32;
+  MgT<double> mg64;
+  int mg_built = 0;          // precision of the built hierarchy (0: none)
+  bool mg_eig_stale = true;  // rho_for_fixed changed since the last eigenvalue estimate
+  Tab tabs_by_k[5]{};        // 1D tables per degree (same node family)
+  DevBuf gZ, mgw;
   afl_sponge sponge{15000.0, 0.0, {0.0, 0.0, 0.0}, 0, 0};
   DevBuf rho_fix, p_fix, u_fix, inv_diag, dinv_D, red, dsc, wS, gV, gW, ydev;
   int diag_comps = 0;
@@ -180,7 +188,7 @@ struct afl_operator {
   DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
   double* host_sc = nullptr;  // pinned scalars
   ~afl_operator() {
-    DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev,
+    DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
                      &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix};
     for (DevBuf* b : all) b->release();
@@ -326,6 +334,146 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
   return check_device("gmres");
 }
 
+// p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
+MgConfig mg_config(const afl_solver_params& sp) {
+  MgConfig c;
+  if (sp.mg_smoother_degree > 0) c.smoother_degree = sp.mg_smoother_degree;
+  if (sp.mg_coarse_degree > 0) c.coarse_degree = sp.mg_coarse_degree;
+  if (sp.mg_levels > 0) c.max_levels = sp.mg_levels;
+  if (sp.mg_smoothing_range > 1.0) c.smoothing_range = sp.mg_smoothing_range;
+  if (sp.mg_cell_tol > 0.0) c.cell_tol = sp.mg_cell_tol;
+  return c;
+}
+
+int ensure_mg(afl_operator* op) {
+  const int want = op->sp.schur_precond;
+  int rc;
+  if (op->mg_built != want) {
+    const MgConfig cfg = mg_config(op->sp);
+    rc = want == AFL_SCHUR_MG_FP32 ? op->mg32.setup_levels(op->o, cfg, op->tabs_by_k)
+                                   : op->mg64.setup_levels(op->o, cfg, op->tabs_by_k);
+    if (rc) return fail(AFL_ERR_DEVICE, "multigrid set-up: hipMalloc failed");
+    op->mg_built = want;
+    op->mg_eig_stale = true;
+  }
+  {  // smoothing degrees and range take effect without a rebuild
+    const MgConfig cfg = mg_config(op->sp);
+    auto upd = [&](auto& mg) {
+      mg.smoother_degree = cfg.smoother_degree;
+      mg.coarse_degree = cfg.coarse_degree;
+      mg.smoothing_range = cfg.smoothing_range;
+    };
+    if (want == AFL_SCHUR_MG_FP32) upd(op->mg32);
+    else upd(op->mg64);
+  }
+  if ((rc = op->mgw.alloc(op->N() + red_work_size(1) + 16))) return rc;
+  // stage coefficients (rho_for_fixed, pres_fixed) on every level
+  if (want == AFL_SCHUR_MG_FP32) op->mg32.set_coefficients(op->rho_fix.p, op->p_fix.p);
+  else op->mg64.set_coefficients(op->rho_fix.p, op->p_fix.p);
+  if (op->mg_eig_stale) {
+    const int it = mg_config(op->sp).eig_iters;
+    rc = want == AFL_SCHUR_MG_FP32
+             ? op->mg32.estimate_eigenvalues(op->o, it, op->mgw.p, op->dsc.p + kScA + 4, op->host_sc + kScA + 4)
+             : op->mg64.estimate_eigenvalues(op->o, it, op->mgw.p, op->dsc.p + kScA + 4, op->host_sc + kScA + 4);
+    if (rc) return fail(AFL_ERR_DEVICE, "multigrid eigenvalue estimate failed");
+    op->mg_eig_stale = false;
+  }
+  return check_device("multigrid set-up");
+}
+
+// Flexible GMRES (Saad 1993) with the p-multigrid V-cycle as variable right preconditioner:
+// z_j = MG(v_j) is stored, x += Z y. Outer arithmetic, Arnoldi basis and residual in FP64.
+int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+  int rc;
+  const int64_t N = op->N();
+  hipStream_t st = op->o.stream;
+  if (m < 1 || m > kMaxRestart) return fail(AFL_ERR_INVALID_ARGUMENT, "gmres_restart must be in 1..60");
+  if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
+  if ((rc = op->gV.alloc((int64_t)(m + 1) * N)) || (rc = op->gZ.alloc((int64_t)m * N)) || (rc = op->gW.alloc(2 * N)) ||
+      (rc = op->ydev.alloc(kMaxRestart)))
+    return rc;
+  double* V = op->gV.p;
+  double* Z = op->gZ.p;
+  double* w = op->gW.p;
+  double* sc = op->dsc.p;
+  double* hs = op->host_sc;
+  const bool f32 = op->sp.schur_precond == AFL_SCHUR_MG_FP32;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  int it = 0;
+  double res = 0.0;
+  while (true) {
+    schur_apply(op, x, w);
+    vec_copy(st, N, b, V);
+    vec_axpby(st, N, -1.0, w, 1.0, V);
+    vec_norm2(st, N, V, op->red.p, sc + kScNrm);
+    if ((rc = fetch(op, kScNrm, 1))) return rc;
+    const double beta = hs[kScNrm];
+    res = beta;
+    if (!(beta > tol)) break;
+    if (it >= max_it) {
+      *iters = it; *res_out = res;
+      return fail(AFL_ERR_NOT_CONVERGED, "Schur FGMRES did not converge");
+    }
+    vec_scale_by_inv(st, N, sc + kScNrm, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    bool conv = false;
+    for (; j < m && it < max_it; ++j) {
+      ++it;
+      const double* vj = V + (int64_t)j * N;
+      double* zj = Z + (int64_t)j * N;
+      double* vn = V + (int64_t)(j + 1) * N;
+      if (f32) op->mg32.apply(op->o, vj, zj);
+      else op->mg64.apply(op->o, vj, zj);
+      schur_apply(op, zj, vn);
+      vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH1);
+      vec_maxpy(st, N, V, N, j + 1, sc + kScH1, -1.0, vn);
+      vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH2);
+      vec_maxpy(st, N, V, N, j + 1, sc + kScH2, -1.0, vn);
+      vec_norm2(st, N, vn, op->red.p, sc + kScNrm);
+      vec_scale_by_inv(st, N, sc + kScNrm, vn);
+      if ((rc = fetch(op, 0, kScNrm + 1))) return rc;
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hs[kScH1 + i] + hs[kScH2 + i];
+      H[(size_t)(j + 1) * m + j] = hs[kScNrm];
+      for (int i = 0; i < j; ++i) {
+        const double a0 = H[(size_t)i * m + j], a1 = H[(size_t)(i + 1) * m + j];
+        H[(size_t)i * m + j] = cs[i] * a0 + sn[i] * a1;
+        H[(size_t)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+      }
+      const double a0 = H[(size_t)j * m + j], a1 = H[(size_t)(j + 1) * m + j];
+      const double r = std::hypot(a0, a1);
+      cs[j] = a0 / r;
+      sn[j] = a1 / r;
+      H[(size_t)j * m + j] = r;
+      H[(size_t)(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      res = std::fabs(g[j + 1]);
+      if (res <= tol) { ++j; conv = true; break; }
+    }
+    for (int i = j - 1; i >= 0; --i) {
+      double v = g[i];
+      for (int l = i + 1; l < j; ++l) v -= H[(size_t)i * m + l] * y[l];
+      y[i] = v / H[(size_t)i * m + i];
+    }
+    if (j > 0) {
+      if (hipMemcpyAsync(op->ydev.p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(AFL_ERR_DEVICE, "hipMemcpyAsync failed");
+      vec_maxpy(st, N, Z, N, j, op->ydev.p, 1.0, x);
+      if (hipStreamSynchronize(st) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
+    }
+    if (conv) break;
+    if (it >= max_it) {
+      *iters = it; *res_out = res;
+      return fail(AFL_ERR_NOT_CONVERGED, "Schur FGMRES did not converge");
+    }
+  }
+  *iters = it;
+  *res_out = res;
+  return check_device("fgmres");
+}
+
 bool valid_op(afl_operator* op) { return op != nullptr; }
 
 }  // namespace
@@ -357,6 +505,7 @@ int afl_create(const afl_mesh_desc* md, double mach, double froude, double dt, a
   o.dim = md->dim;
   o.k = md->degree;
   build_tab(o.k, md->node_type, o.T);
+  for (int kk = 1; kk <= 4; ++kk) build_tab(kk, md->node_type, op->tabs_by_k[kk]);
   Geo& g = o.geo;
   g.nc[0] = nx; g.nc[1] = md->dim == 3 ? ny : nz; g.nc[2] = md->dim == 3 ? nz : 1;
   g.ncells = (int)ncells;
@@ -442,6 +591,7 @@ int afl_set_ns_stage(afl_operator* op, unsigned s) {
 int afl_set_rho_for_fixed(afl_operator* op, const double* v) {
   if (!op || !v) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   vec_copy(op->o.stream, op->o.N, v, op->rho_fix.p);
+  op->mg_eig_stale = true;
   return check_device("afl_set_rho_for_fixed");
 }
 int afl_set_pres_fixed(afl_operator* op, const double* v) {
@@ -459,7 +609,14 @@ int afl_set_solver_params(afl_operator* op, const afl_solver_params* sp) {
   if (sp->gmres_restart < 1 || sp->gmres_restart > kMaxRestart) return fail(AFL_ERR_INVALID_ARGUMENT, "gmres_restart must be 1..60");
   if (!(sp->gmres_eps > 0.0) || sp->gmres_max_iter < 1 || sp->fp_max_iter < 1 || !(sp->fp_tol > 0.0))
     return fail(AFL_ERR_INVALID_ARGUMENT, "invalid solver parameters");
+  if (sp->schur_precond < AFL_SCHUR_JACOBI || sp->schur_precond > AFL_SCHUR_MG_FP64)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "unknown schur_precond");
+  if (sp->mg_smoother_degree > 12 || sp->mg_coarse_degree > 64 || sp->mg_levels > 4)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "multigrid parameters out of range");
+  const bool rebuild = sp->schur_precond != op->sp.schur_precond || sp->mg_levels != op->sp.mg_levels ||
+                       sp->mg_cell_tol != op->sp.mg_cell_tol;
   op->sp = *sp;
+  if (rebuild) op->mg_built = 0;
   return AFL_OK;
 }
 int afl_set_sponge(afl_operator* op, const afl_sponge* s) {
@@ -579,7 +736,10 @@ int afl_schur_solve(afl_operator* op, double* p, const double* rhs, int* iters, 
   const double tol = op->sp.gmres_eps * op->host_sc[kScA];
   int it = 0;
   double res = 0.0;
-  rc = gmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res);
+  if (op->sp.schur_precond == AFL_SCHUR_JACOBI)
+    rc = gmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res);
+  else
+    rc = fgmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res);
   if (iters) *iters = it;
   if (residual) *residual = res;
   return rc;
@@ -657,12 +817,14 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
   { const double* s[2] = {rho, u}; launch_rhs_rho(o, 1, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho2.p);
   vec_copy(st, N, op->rho2.p, op->rho_fix.p);
+  op->mg_eig_stale = true;
   if ((rc = fixed_point(1, op->rho2.p, p, u, op->p2.p, op->u2.p))) return rc;
   // stage 2
   o.hyp = 2;
   { const double* s[4] = {rho, u, op->rho2.p, op->u2.p}; launch_rhs_rho(o, 2, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho3.p);
   vec_copy(st, N, op->rho3.p, op->rho_fix.p);
+  op->mg_eig_stale = true;
   if ((rc = fixed_point(2, op->rho3.p, op->p2.p, op->u2.p, op->p3.p, op->u3.p))) return rc;
   // stage 3
   o.hyp = 3;

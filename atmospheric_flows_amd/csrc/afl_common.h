@@ -23,14 +23,16 @@ namespace afl {
 
 constexpr double kGamma = 1.4;  // EquationData::Cp_Cv
 
-struct Tab {
-  double S[9][5];   // S[q][i]  = phi_i(x_q), Gauss point q of QGauss(2k+1) on [0,1]
-  double D[9][5];   // D[q][i]  = phi_i'(x_q)
-  double W[9];      // Gauss weights on [0,1]
-  double QP[9];     // Gauss points on [0,1]
-  double MI[5][5];  // inverse of the reference 1D mass matrix (cell-local preconditioner)
-  double ND[5];     // 1D support points
+template <typename R>
+struct TabT {
+  R S[9][5];   // S[q][i]  = phi_i(x_q), Gauss point q of QGauss(2k+1) on [0,1]
+  R D[9][5];   // D[q][i]  = phi_i'(x_q)
+  R W[9];      // Gauss weights on [0,1]
+  R QP[9];     // Gauss points on [0,1]
+  R MI[5][5];  // inverse of the reference 1D mass matrix (cell-local preconditioner)
+  R ND[5];     // 1D support points
 };
+using Tab = TabT<double>;
 
 struct Geo {
   int nc[3];        // cells along x, y, z (ny = 1 in 2D)
@@ -82,113 +84,120 @@ struct Slot {
 template <int DIM>
 __device__ inline int vaxis() { return DIM - 1; }
 
-// Hill h(x,y) = h0 / (1 + r^2/a^2) and its horizontal derivatives.
-__device__ inline void hill(const Geo& g, double x, double y, double& h, double& hx, double& hy) {
-  const double s = 1.0 + (x * x + y * y) * g.inv_a2;
-  const double is = 1.0 / s;
-  h = g.h0 * is;
-  const double f = -2.0 * g.h0 * g.inv_a2 * is * is;
+// Hill h(x,y) = h0 / (1 + r^2/a^2) and its horizontal derivatives (arithmetic in R).
+template <typename R>
+__device__ inline void hill(const Geo& g, R x, R y, R& h, R& hx, R& hy) {
+  const R s = R(1) + (x * x + y * y) * R(g.inv_a2);
+  const R is = R(1) / s;
+  h = R(g.h0) * is;
+  const R f = R(-2) * R(g.h0) * R(g.inv_a2) * is * is;
   hx = f * x;
   hy = f * y;
 }
 
 // Per-line geometry: everything that is constant along the thread's vertical line.
-template <int DIM, int K>
+template <int DIM, int K, typename R = double>
 struct LineGeo {
-  double c;      // hz (1 - h/H) = dz/dzeta
-  double dhx, dhy;
-  double wxy;    // horizontal quadrature weight product
-  __device__ void init(const Geo& g, const Tab& T, const int* ic, int l) {
+  R c;      // hz (1 - h/H) = dz/dzeta
+  R dhx, dhy;
+  R wxy;    // horizontal quadrature weight product
+  R hx, hy, hz, iH, zeta0;
+  __device__ void init(const Geo& g, const TabT<R>& T, const int* ic, int l) {
     using C = Cfg<DIM, K>;
-    double x, y = 0.0;
+    hx = R(g.hc[0]);
+    hy = DIM == 3 ? R(g.hc[1]) : R(1);
+    hz = R(g.hc[DIM - 1]);
+    iH = R(1.0 / g.H);
+    zeta0 = R(ic[DIM - 1] * g.hc[DIM - 1]);
+    R x, y = R(0);
     if (DIM == 2) {
-      x = g.lo[0] + (ic[0] + T.QP[l]) * g.hc[0];
+      x = R(g.lo[0] + (ic[0] + (double)T.QP[l]) * g.hc[0]);
       wxy = T.W[l];
     } else {
       const int qx = l % C::NQ, qy = l / C::NQ;
-      x = g.lo[0] + (ic[0] + T.QP[qx]) * g.hc[0];
-      y = g.lo[1] + (ic[1] + T.QP[qy]) * g.hc[1];
+      x = R(g.lo[0] + (ic[0] + (double)T.QP[qx]) * g.hc[0]);
+      y = R(g.lo[1] + (ic[1] + (double)T.QP[qy]) * g.hc[1]);
       wxy = T.W[qx] * T.W[qy];
     }
-    double h;
-    hill(g, x, DIM == 3 ? y : 0.0, h, dhx, dhy);
-    c = g.hc[DIM - 1] * (1.0 - h / g.H);
+    R h;
+    hill(g, x, y, h, dhx, dhy);
+    c = hz * (R(1) - h * iH);
   }
   // JxW at vertical quadrature index qz
-  __device__ double jxw(const Geo& g, const Tab& T, int qz) const {
-    return (DIM == 2 ? g.hc[0] : g.hc[0] * g.hc[1]) * c * wxy * T.W[qz];
+  __device__ R jxw(const Geo& g, const TabT<R>& T, int qz) const {
+    return (DIM == 2 ? hx : hx * hy) * c * wxy * T.W[qz];
   }
-  // gradient contribution (physical G, already containing no JxW) -> reference coords * JxW
-  __device__ void gref(const Geo& g, const Tab& T, const int* ic, int qz, const double* G, double* R) const {
-    const double zeta = (ic[DIM - 1] + T.QP[qz]) * g.hc[DIM - 1];
-    const double fz = 1.0 - zeta / g.H;
-    const double w = wxy * T.W[qz];
+  // gradient contribution (physical G, without JxW) -> reference coordinates * JxW
+  __device__ void gref(const Geo& g, const TabT<R>& T, const int* ic, int qz, const R* G, R* Rf) const {
+    const R zeta = zeta0 + T.QP[qz] * hz;
+    const R fz = R(1) - zeta * iH;
+    const R w = wxy * T.W[qz];
     if (DIM == 2) {
-      const double dx = dhx * fz * g.hc[0];
-      R[0] = G[0] * c * w;
-      R[1] = (-dx * G[0] + g.hc[0] * G[1]) * w;
+      const R dx = dhx * fz * hx;
+      Rf[0] = G[0] * c * w;
+      Rf[1] = (-dx * G[0] + hx * G[1]) * w;
     } else {
-      const double hx = g.hc[0], hy = g.hc[1];
-      const double dx = dhx * fz * hx, dy = dhy * fz * hy;
-      R[0] = G[0] * hy * c * w;
-      R[1] = G[1] * hx * c * w;
-      R[2] = (-dx * hy * G[0] - dy * hx * G[1] + hx * hy * G[DIM - 1]) * w;
+      const R dx = dhx * fz * hx, dy = dhy * fz * hy;
+      Rf[0] = G[0] * hy * c * w;
+      Rf[1] = G[1] * hx * c * w;
+      Rf[2] = (-dx * hy * G[0] - dy * hx * G[1] + hx * hy * G[DIM - 1]) * w;
     }
   }
 };
 
 // Geometry of face f = 2*dir + side of cell ic at face point fq (2D: fq = q; 3D: fq = qa + NQ qb,
 // qa/qb along the two tangential axes in increasing order). Returns outward normal and JxW.
-template <int DIM, int K>
-__device__ inline void face_geo(const Geo& g, const Tab& T, const int* ic, int f, int fq, double* n, double& jxw) {
+template <int DIM, int K, typename R = double>
+__device__ inline void face_geo(const Geo& g, const TabT<R>& T, const int* ic, int f, int fq, R* n, R& jxw) {
   using C = Cfg<DIM, K>;
   const int dir = f >> 1, side = f & 1;
-  const double sg = side ? 1.0 : -1.0;
+  const R sg = side ? R(1) : R(-1);
+  const R iH = R(1.0 / g.H);
   if (DIM == 2) {
     if (dir == 0) {
-      const double x = g.lo[0] + (ic[0] + side) * g.hc[0];
-      double h, hx, hy;
-      hill(g, x, 0.0, h, hx, hy);
-      n[0] = sg; n[1] = 0.0;
-      jxw = g.hc[1] * (1.0 - h / g.H) * T.W[fq];
+      const R x = R(g.lo[0] + (ic[0] + side) * g.hc[0]);
+      R h, hx, hy;
+      hill(g, x, R(0), h, hx, hy);
+      n[0] = sg; n[1] = R(0);
+      jxw = R(g.hc[1]) * (R(1) - h * iH) * T.W[fq];
     } else {
-      const double x = g.lo[0] + (ic[0] + T.QP[fq]) * g.hc[0];
-      double h, hx, hy;
-      hill(g, x, 0.0, h, hx, hy);
-      const double zeta = (ic[1] + side) * g.hc[1];
-      const double vx = -hx * (1.0 - zeta / g.H);
-      const double nv = sqrt(1.0 + vx * vx);
+      const R x = R(g.lo[0] + (ic[0] + (double)T.QP[fq]) * g.hc[0]);
+      R h, hx, hy;
+      hill(g, x, R(0), h, hx, hy);
+      const R zeta = R((ic[1] + side) * g.hc[1]);
+      const R vx = -hx * (R(1) - zeta * iH);
+      const R nv = sqrt(R(1) + vx * vx);
       n[0] = sg * vx / nv; n[1] = sg / nv;
-      jxw = g.hc[0] * nv * T.W[fq];
+      jxw = R(g.hc[0]) * nv * T.W[fq];
     }
   } else {
     const int qa = fq % C::NQ, qb = fq / C::NQ;
-    const double w = T.W[qa] * T.W[qb];
+    const R w = T.W[qa] * T.W[qb];
     double x, y, zeta;
     if (dir == 0) {
       x = g.lo[0] + (ic[0] + side) * g.hc[0];
-      y = g.lo[1] + (ic[1] + T.QP[qa]) * g.hc[1];
-      zeta = (ic[2] + T.QP[qb]) * g.hc[2];
+      y = g.lo[1] + (ic[1] + (double)T.QP[qa]) * g.hc[1];
+      zeta = (ic[2] + (double)T.QP[qb]) * g.hc[2];
     } else if (dir == 1) {
-      x = g.lo[0] + (ic[0] + T.QP[qa]) * g.hc[0];
+      x = g.lo[0] + (ic[0] + (double)T.QP[qa]) * g.hc[0];
       y = g.lo[1] + (ic[1] + side) * g.hc[1];
-      zeta = (ic[2] + T.QP[qb]) * g.hc[2];
+      zeta = (ic[2] + (double)T.QP[qb]) * g.hc[2];
     } else {
-      x = g.lo[0] + (ic[0] + T.QP[qa]) * g.hc[0];
-      y = g.lo[1] + (ic[1] + T.QP[qb]) * g.hc[1];
+      x = g.lo[0] + (ic[0] + (double)T.QP[qa]) * g.hc[0];
+      y = g.lo[1] + (ic[1] + (double)T.QP[qb]) * g.hc[1];
       zeta = (ic[2] + side) * g.hc[2];
     }
-    double h, hx, hy;
-    hill(g, x, y, h, hx, hy);
-    const double c = g.hc[2] * (1.0 - h / g.H);
-    if (dir == 0) { n[0] = sg; n[1] = 0.0; n[2] = 0.0; jxw = g.hc[1] * c * w; }
-    else if (dir == 1) { n[0] = 0.0; n[1] = sg; n[2] = 0.0; jxw = g.hc[0] * c * w; }
+    R h, hx, hy;
+    hill(g, R(x), R(y), h, hx, hy);
+    const R c = R(g.hc[2]) * (R(1) - h * iH);
+    if (dir == 0) { n[0] = sg; n[1] = R(0); n[2] = R(0); jxw = R(g.hc[1]) * c * w; }
+    else if (dir == 1) { n[0] = R(0); n[1] = sg; n[2] = R(0); jxw = R(g.hc[0]) * c * w; }
     else {
-      const double fz = 1.0 - zeta / g.H;
-      const double vx = -hx * fz, vy = -hy * fz;
-      const double nv = sqrt(1.0 + vx * vx + vy * vy);
+      const R fz = R(1) - R(zeta) * iH;
+      const R vx = -hx * fz, vy = -hy * fz;
+      const R nv = sqrt(R(1) + vx * vx + vy * vy);
       n[0] = sg * vx / nv; n[1] = sg * vy / nv; n[2] = sg / nv;
-      jxw = g.hc[0] * g.hc[1] * nv * w;
+      jxw = R(g.hc[0]) * R(g.hc[1]) * nv * w;
     }
   }
 }
@@ -221,46 +230,23 @@ __device__ inline int face_node(int dir, int side, int t) {
   return ta + N1 * (tb + N1 * s);
 }
 
-// Trace of a scalar field of cell `cell` on face (dir, side) at face point fq (direct sum
-// over the face nodes; the 1D basis is nodal with end points, so only face nodes contribute).
-template <int DIM, int K>
-__device__ inline double trace(const double* __restrict__ f, int64_t cell, int dir, int side, int fq, const Tab& T) {
-  using C = Cfg<DIM, K>;
-  const double* u = f + cell * C::NPC;
-  double s = 0.0;
-  if (DIM == 2) {
-#pragma unroll
-    for (int t = 0; t < C::N1; ++t) s += T.S[fq][t] * u[face_node<DIM, K>(dir, side, t)];
-  } else {
-    const int qa = fq % C::NQ, qb = fq / C::NQ;
-#pragma unroll
-    for (int tb = 0; tb < C::N1; ++tb) {
-      double r = 0.0;
-#pragma unroll
-      for (int ta = 0; ta < C::N1; ++ta) r += T.S[qa][ta] * u[face_node<DIM, K>(dir, side, ta + C::N1 * tb)];
-      s += T.S[qb][tb] * r;
-    }
-  }
-  return s;
-}
-
 // ---------------------------------------------------------------------------------------
 // Evaluate a scalar field (global dofs) at the quadrature points of the thread's line.
 // All threads of the block must call it (it contains barriers).
-template <int DIM, int K>
-__device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K>& s, double* t1, double* t2,
-                                 const Tab& T, double (&v)[2 * K + 1]) {
+template <int DIM, int K, typename R = double>
+__device__ inline void eval_line(const R* __restrict__ f, const Slot<DIM, K>& s, R* t1, R* t2,
+                                 const TabT<R>& T, R (&v)[2 * K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ;
-  const double* u = f + (s.act ? s.cell : 0) * C::NPC;
+  const R* u = f + (s.act ? s.cell : 0) * C::NPC;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
-      double d[N1];
+      R d[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * d[i];
         t1[s.l * NQ + q] = a;
@@ -268,12 +254,12 @@ __device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K
     }
     __syncthreads();
     if (s.act) {
-      double c[N1];
+      R c[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) c[i] = t1[i * NQ + s.l];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         v[q] = a;
@@ -282,12 +268,12 @@ __device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K
     __syncthreads();
   } else {
     if (s.act && s.l < N1 * N1) {
-      double d[N1];
+      R d[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * d[i];
         t1[s.l * NQ + q] = a;
@@ -296,12 +282,12 @@ __device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K
     __syncthreads();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
-      double c[N1];
+      R c[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) c[i] = t1[(iz * N1 + i) * NQ + qx];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         t2[(iz * NQ + q) * NQ + qx] = a;
@@ -309,12 +295,12 @@ __device__ inline void eval_line(const double* __restrict__ f, const Slot<DIM, K
     }
     __syncthreads();
     if (s.act) {
-      double c[N1];
+      R c[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) c[i] = t2[i * NQ * NQ + s.l];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         v[q] = a;
@@ -347,21 +333,21 @@ struct FaceLds {
 // Row owners copy their own face nodes and fetch the neighbour's matching face nodes (2 scalar
 // loads per face direction) into fn during the x sweep; face-point threads interpolate after the
 // first barrier, so no barrier is added. fn: FaceLds::FN doubles per cell.
-template <int DIM, int K>
-__device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
-                                       double* t1, double* t2, double* fn, const Tab& T, double (&v)[2 * K + 1],
-                                       double (&tr)[2 * DIM][2]) {
+template <int DIM, int K, typename R = double>
+__device__ inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
+                                       R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[2 * K + 1],
+                                       R (&tr)[2 * DIM][2]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN;
-  const double* u = f + (s.act ? s.cell : 0) * NPC;
-  auto FN = [&](int face, int side, int t) -> double& { return fn[(face * 2 + side) * NFN + t]; };
+  const R* u = f + (s.act ? s.cell : 0) * NPC;
+  auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
-    double d[N1];
+    R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      double a = 0.0;
+      R a = 0.0;
 #pragma unroll
       for (int i = 0; i < N1; ++i) a += T.S[q][i] * d[i];
       t1[s.l * NQ + q] = a;
@@ -398,7 +384,7 @@ __device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<
     for (int fc = 0; fc < 2 * DIM; ++fc)
 #pragma unroll
       for (int sd = 0; sd < 2; ++sd) {
-        double a = 0.0;
+        R a = 0.0;
         if (DIM == 2) {
 #pragma unroll
           for (int t = 0; t < N1; ++t) a += T.S[s.l][t] * FN(fc, sd, t);
@@ -406,7 +392,7 @@ __device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<
           const int qa = s.l % NQ, qb = s.l / NQ;
 #pragma unroll
           for (int tb = 0; tb < N1; ++tb) {
-            double r = 0.0;
+            R r = 0.0;
 #pragma unroll
             for (int ta = 0; ta < N1; ++ta) r += T.S[qa][ta] * FN(fc, sd, ta + N1 * tb);
             a += T.S[qb][tb] * r;
@@ -420,12 +406,12 @@ __device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<
   }
   if (DIM == 2) {
     if (s.act) {
-      double c[N1];
+      R c[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) c[i] = t1[i * NQ + s.l];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         v[q] = a;
@@ -435,12 +421,12 @@ __device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<
   } else {
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
-      double c[N1];
+      R c[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) c[i] = t1[(iz * N1 + i) * NQ + qx];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         t2[(iz * NQ + q) * NQ + qx] = a;
@@ -448,12 +434,12 @@ __device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<
     }
     __syncthreads();
     if (s.act) {
-      double c[N1];
+      R c[N1];
 #pragma unroll
       for (int i = 0; i < N1; ++i) c[i] = t2[i * NQ * NQ + s.l];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         v[q] = a;
@@ -464,16 +450,16 @@ __device__ inline void eval_line_faces(const double* __restrict__ f, const Slot<
 }
 
 // Same as eval_line but the dofs come from an LDS copy of the cell (u = lds + cb*NPC).
-template <int DIM, int K>
-__device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, double* t1, double* t2,
-                                     const Tab& T, double (&v)[2 * K + 1]) {
+template <int DIM, int K, typename R = double>
+__device__ inline void eval_line_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2,
+                                     const TabT<R>& T, R (&v)[2 * K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[s.l * N1 + i];
         t1[s.l * NQ + q] = a;
@@ -483,7 +469,7 @@ __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, dou
     if (s.act) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * t1[i * NQ + s.l];
         v[q] = a;
@@ -494,7 +480,7 @@ __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, dou
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[s.l * N1 + i];
         t1[s.l * NQ + q] = a;
@@ -505,7 +491,7 @@ __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, dou
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * t1[(iz * N1 + i) * NQ + qx];
         t2[(iz * NQ + q) * NQ + qx] = a;
@@ -515,7 +501,7 @@ __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, dou
     if (s.act) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * t2[i * NQ * NQ + s.l];
         v[q] = a;
@@ -527,9 +513,9 @@ __device__ inline void eval_line_lds(const double* u, const Slot<DIM, K>& s, dou
 
 // Batched evaluation of NF scalar fields whose cell dofs sit in LDS (field f at u + f*NPC).
 // t1 needs NF*EV1 doubles, t2 NF*EV2 doubles per cell. Contains barriers.
-template <int DIM, int K, int NF>
-__device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, double* t1, double* t2, const Tab& T,
-                                      double (&v)[NF][2 * K + 1]) {
+template <int DIM, int K, int NF, typename R = double>
+__device__ inline void eval_lines_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2, const TabT<R>& T,
+                                      R (&v)[NF][2 * K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, E1 = C::EV1, E2 = C::EV2;
   if (DIM == 2) {
@@ -538,7 +524,7 @@ __device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, do
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[f * NPC + s.l * N1 + i];
           t1[f * E1 + s.l * NQ + q] = a;
@@ -548,12 +534,12 @@ __device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, do
     if (s.act) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        double c[N1];
+        R c[N1];
 #pragma unroll
         for (int i = 0; i < N1; ++i) c[i] = t1[f * E1 + i * NQ + s.l];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
           v[f][q] = a;
@@ -567,7 +553,7 @@ __device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, do
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int i = 0; i < N1; ++i) a += T.S[q][i] * u[f * NPC + s.l * N1 + i];
           t1[f * E1 + s.l * NQ + q] = a;
@@ -578,12 +564,12 @@ __device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, do
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        double c[N1];
+        R c[N1];
 #pragma unroll
         for (int i = 0; i < N1; ++i) c[i] = t1[f * E1 + (iz * N1 + i) * NQ + qx];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
           t2[f * E2 + (iz * NQ + q) * NQ + qx] = a;
@@ -594,12 +580,12 @@ __device__ inline void eval_lines_lds(const double* u, const Slot<DIM, K>& s, do
     if (s.act) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        double c[N1];
+        R c[N1];
 #pragma unroll
         for (int i = 0; i < N1; ++i) c[i] = t2[f * E2 + i * NQ * NQ + s.l];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
           v[f][q] = a;
@@ -620,9 +606,9 @@ struct IntV {
   static constexpr int I2 = DIM == 2 ? 0 : N1 * N1 * NQ;
 };
 
-template <int DIM, int K, int NF>
-__device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, double* i2, const Tab& T,
-                                         const double (&V)[NF][2 * K + 1], double (&out)[NF][K + 1]) {
+template <int DIM, int K, int NF, typename R = double>
+__device__ inline void integrate_lines_v(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
+                                         const R (&V)[NF][2 * K + 1], R (&out)[NF][K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, A1 = IntV<DIM, K>::I1, A2 = IntV<DIM, K>::I2;
   if (DIM == 2) {
@@ -631,7 +617,7 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, doub
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int q = 0; q < NQ; ++q) a += T.S[q][i] * V[f][q];
           i1[f * A1 + i * NQ + s.l] = a;
@@ -643,7 +629,7 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, doub
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int q = 0; q < NQ; ++q) a += T.S[q][i] * i1[f * A1 + s.l * NQ + q];
           out[f][i] = a;
@@ -656,7 +642,7 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, doub
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int q = 0; q < NQ; ++q) a += T.S[q][i] * V[f][q];
           i1[f * A1 + i * NQ * NQ + s.l] = a;
@@ -669,7 +655,7 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, doub
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int q = 0; q < NQ; ++q) a += T.S[q][i] * i1[f * A1 + (iz * NQ + q) * NQ + qx];
           i2[f * A2 + (iz * N1 + i) * NQ + qx] = a;
@@ -681,7 +667,7 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, doub
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
+          R a = 0.0;
 #pragma unroll
           for (int q = 0; q < NQ; ++q) a += T.S[q][i] * i2[f * A2 + s.l * NQ + q];
           out[f][i] = a;
@@ -694,19 +680,19 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, double* i1, doub
 // Integrate line data (values V and reference-gradient contributions G[d], all already
 // multiplied by JxW) against the test functions. Row `l` (l < ROWS) of the result ends up
 // in out[N1]. HAS_V / HAS_G select the terms. Contains barriers.
-template <int DIM, int K, bool HAS_V, bool HAS_G>
-__device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double* i2, const Tab& T,
-                                      const double (&V)[2 * K + 1], const double (&G)[DIM][2 * K + 1],
-                                      double (&out)[K + 1]) {
+template <int DIM, int K, bool HAS_V, bool HAS_G, typename R = double>
+__device__ inline void integrate_line(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
+                                      const R (&V)[2 * K + 1], const R (&G)[DIM][2 * K + 1],
+                                      R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ;
   if (DIM == 2) {
-    double* ta = i1;
-    double* tb = i1 + N1 * NQ;
+    R* ta = i1;
+    R* tb = i1 + N1 * NQ;
     if (s.act) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0, b = 0.0;
+        R a = 0.0, b = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if (HAS_V) a += T.S[q][i] * V[q];
@@ -720,7 +706,7 @@ __device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double*
     if (s.act && s.l < N1) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           a += T.S[q][i] * ta[s.l * NQ + q];
@@ -731,15 +717,15 @@ __device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double*
     }
     __syncthreads();
   } else {
-    double* ta = i1;
-    double* tx = i1 + N1 * NQ * NQ;
-    double* ty = i1 + 2 * N1 * NQ * NQ;
-    double* ua = i2;
-    double* ux = i2 + N1 * N1 * NQ;
+    R* ta = i1;
+    R* tx = i1 + N1 * NQ * NQ;
+    R* ty = i1 + 2 * N1 * NQ * NQ;
+    R* ua = i2;
+    R* ux = i2 + N1 * N1 * NQ;
     if (s.act) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0, bx = 0.0, by = 0.0;
+        R a = 0.0, bx = 0.0, by = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if (HAS_V) a += T.S[q][i] * V[q];
@@ -754,7 +740,7 @@ __device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double*
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0, bx = 0.0;
+        R a = 0.0, bx = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           const int idx = (iz * NQ + q) * NQ + qx;
@@ -769,7 +755,7 @@ __device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double*
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0;
+        R a = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           a += T.S[q][i] * ua[s.l * NQ + q];
@@ -784,8 +770,8 @@ __device__ inline void integrate_line(const Slot<DIM, K>& s, double* i1, double*
 
 // Add face-flux integrals to node row l. Fq points to the LDS block [2*DIM][NFQ] of this cell
 // (values already multiplied by JxW_f and oriented for this cell). Only rows on faces change.
-template <int DIM, int K>
-__device__ inline void add_faces(const Slot<DIM, K>& s, const double* Fq, const Tab& T, double (&out)[K + 1],
+template <int DIM, int K, typename R = double>
+__device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<R>& T, R (&out)[K + 1],
                                  unsigned face_mask) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, NFQ = C::NFQ;
@@ -793,21 +779,21 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const double* Fq, const 
   if (DIM == 2) {
     const int iz = s.l;
     // x faces: nodes (0, iz) and (K, iz)
-    if (face_mask & 1u) { double a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[0 * NFQ + q]; out[0] += a; }
-    if (face_mask & 2u) { double a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[1 * NFQ + q]; out[K] += a; }
+    if (face_mask & 1u) { R a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[0 * NFQ + q]; out[0] += a; }
+    if (face_mask & 2u) { R a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[1 * NFQ + q]; out[K] += a; }
     const int fz = (iz == 0) ? 2 : ((iz == K) ? 3 : -1);
     if (fz >= 0 && (face_mask & (1u << fz))) {
 #pragma unroll
-      for (int i = 0; i < N1; ++i) { double a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][i] * Fq[fz * NFQ + q]; out[i] += a; }
+      for (int i = 0; i < N1; ++i) { R a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][i] * Fq[fz * NFQ + q]; out[i] += a; }
     }
   } else {
     const int iy = s.l % N1, iz = s.l / N1;
     for (int side = 0; side < 2; ++side) {
       if (!(face_mask & (1u << side))) continue;
-      const double* F = Fq + side * NFQ;
-      double a = 0.0;
+      const R* F = Fq + side * NFQ;
+      R a = 0.0;
       for (int qb = 0; qb < NQ; ++qb) {
-        double r = 0.0;
+        R r = 0.0;
         for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][iy] * F[qb * NQ + qa];
         a += T.S[qb][iz] * r;
       }
@@ -815,12 +801,12 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const double* Fq, const 
     }
     const int fy = (iy == 0) ? 2 : ((iy == K) ? 3 : -1);
     if (fy >= 0 && (face_mask & (1u << fy))) {
-      const double* F = Fq + fy * NFQ;
+      const R* F = Fq + fy * NFQ;
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0;
+        R a = 0.0;
         for (int qb = 0; qb < NQ; ++qb) {
-          double r = 0.0;
+          R r = 0.0;
           for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][i] * F[qb * NQ + qa];
           a += T.S[qb][iz] * r;
         }
@@ -829,12 +815,12 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const double* Fq, const 
     }
     const int fz = (iz == 0) ? 4 : ((iz == K) ? 5 : -1);
     if (fz >= 0 && (face_mask & (1u << fz))) {
-      const double* F = Fq + fz * NFQ;
+      const R* F = Fq + fz * NFQ;
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
-        double a = 0.0;
+        R a = 0.0;
         for (int qb = 0; qb < NQ; ++qb) {
-          double r = 0.0;
+          R r = 0.0;
           for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][i] * F[qb * NQ + qa];
           a += T.S[qb][iy] * r;
         }
@@ -845,11 +831,11 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const double* Fq, const 
 }
 
 // Write node row l of a cell.
-template <int DIM, int K>
-__device__ inline void store_row(double* __restrict__ dst, const Slot<DIM, K>& s, const double (&out)[K + 1]) {
+template <int DIM, int K, typename R = double>
+__device__ inline void store_row(R* __restrict__ dst, const Slot<DIM, K>& s, const R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
   if (s.act && s.l < C::ROWS) {
-    double* o = dst + s.cell * C::NPC + s.l * C::N1;
+    R* o = dst + s.cell * C::NPC + s.l * C::N1;
 #pragma unroll
     for (int i = 0; i < C::N1; ++i) o[i] = out[i];
   }

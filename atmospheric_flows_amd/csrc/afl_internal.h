@@ -36,6 +36,9 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst);
 void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst);
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x);
 void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* w);
+template <typename R>
+void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
+                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
 
 // vector kernels (afl_blas.hip); reductions leave their result in device memory
 void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);

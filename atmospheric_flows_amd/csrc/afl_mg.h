@@ -1,0 +1,51 @@
+// p-multigrid preconditioner for the Schur system (see afl_mg.hip).
+#pragma once
+
+#include <vector>
+
+#include "afl_internal.h"
+
+namespace afl {
+
+struct MgConfig {
+  int smoother_degree = 3;      // Chebyshev degree on every level but the coarsest
+  int coarse_degree = 16;       // Chebyshev degree on the coarsest level
+  double smoothing_range = 15;  // lambda_max / lambda_min of the smoothed interval
+  int max_levels = 1;
+  double cell_tol = 0.05;       // cell-local A solve tolerance on the levels (preconditioner accuracy)
+  int cell_maxit = 40;
+  int eig_iters = 12;           // power iterations for lambda_max
+};
+
+template <typename R>
+struct MgLevelT {
+  int dim = 2, k = 1;
+  int64_t ncells = 0, N = 0;
+  Geo geo{};
+  TabT<R> T{};
+  double MI[5][5] = {};
+  double E[5][5] = {};   // nodal interpolation fine level -> this level
+  double P[5][5] = {};   // prolongation this (coarse) level -> next finer level
+  double Pt[5][5] = {};  // restriction (transpose of P)
+  double lam_max = 1.0;
+  double cell_tol = 0.05;
+  int cell_maxit = 40;
+  hipStream_t stream = nullptr;
+  R *rho = nullptr, *pfix = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *r2 = nullptr, *z = nullptr,
+    *d = nullptr, *t = nullptr, *w = nullptr, *jbar = nullptr;
+};
+
+template <typename R>
+struct MgT {
+  std::vector<MgLevelT<R>> levels;
+  int smoother_degree = 3, coarse_degree = 8;
+  double smoothing_range = 15.0;
+  int setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k);
+  void release();
+  void set_coefficients(const double* rho_fine, const double* pfix_fine);
+  int estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, double* scal_d, double* scal_h);
+  void apply(const OpCtx& fine, const double* v, double* z);
+  ~MgT() { release(); }
+};
+
+}  // namespace afl

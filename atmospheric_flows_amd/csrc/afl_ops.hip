@@ -32,15 +32,15 @@ struct Lds {
   (void)t2; (void)i2; (void)fn;
 
 // Face geometry of the thread's face point on all 2*dim faces (outward normals, JxW).
-template <int DIM, int K>
+template <int DIM, int K, typename R = double>
 struct FacePts {
-  double n[2 * DIM][DIM];
-  double jxw[2 * DIM];
-  __device__ FacePts(const Geo& g, const Tab& T, const Slot<DIM, K>& s) {
+  R n[2 * DIM][DIM];
+  R jxw[2 * DIM];
+  __device__ FacePts(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s) {
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
-      double nn[3];
-      face_geo<DIM, K>(g, T, s.ic, f, s.l, nn, jxw[f]);
+      R nn[3];
+      face_geo<DIM, K, R>(g, T, s.ic, f, s.l, nn, jxw[f]);
 #pragma unroll
       for (int d = 0; d < DIM; ++d) n[f][d] = nn[d];
     }
@@ -424,19 +424,18 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, Tab T, const double*
 // ---------------------------------------------------------------------------------------
 // B: vmult_pressure (euler_operator.h:1510-1617). Volume -f p div v, faces +f {p} n . v
 // (own side), walls +f p n . v, with f = coeff dt / Ma^2. fn: face-node LDS of this cell.
-template <int DIM, int K>
-__device__ inline void apply_B_rows(const Geo& g, const Tab& T, const Slot<DIM, K>& s, const LineGeo<DIM, K>& lg,
-                                    const Nbrs<DIM>& nb, const FacePts<DIM, K>& fp, const double* __restrict__ p,
-                                    double f, double* t1, double* t2, double* i1, double* i2, double* fn, double* Fq,
-                                    double (&out)[DIM][K + 1]) {
+template <int DIM, int K, typename R = double>
+__device__ inline void apply_B_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
+                                    const Nbrs<DIM>& nb, const FacePts<DIM, K, R>& fp, const R* __restrict__ p,
+                                    R f, R* t1, R* t2, R* i1, R* i2, R* fn, R* Fq, R (&out)[DIM][K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ;
-  double pq[NQ], trp[2 * DIM][2];
-  eval_line_faces<DIM, K>(p, s, nb, t1, t2, fn, T, pq, trp);
+  R pq[NQ], trp[2 * DIM][2];
+  eval_line_faces<DIM, K, R>(p, s, nb, t1, t2, fn, T, pq, trp);
   if (s.act) {
 #pragma unroll
     for (int fc = 0; fc < 2 * DIM; ++fc) {
-      const double avg = 0.5 * (trp[fc][0] + trp[fc][1]);  // walls: trp[fc][1] == trp[fc][0] (p_D = p)
+      const R avg = R(0.5) * (trp[fc][0] + trp[fc][1]);  // walls: trp[fc][1] == trp[fc][0] (p_D = p)
 #pragma unroll
       for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + fc) * C::NFQ + s.l] = f * avg * fp.n[fc][a] * fp.jxw[fc];
     }
@@ -444,16 +443,16 @@ __device__ inline void apply_B_rows(const Geo& g, const Tab& T, const Slot<DIM, 
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    double V[NQ], G[DIM][NQ];
+    R V[NQ], G[DIM][NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      double Gp[3] = {0, 0, 0}, R[3];
+      R Gp[3] = {0, 0, 0}, Rg[3];
       Gp[a] = -f * pq[q];
-      lg.gref(g, T, s.ic, q, Gp, R);
-      for (int d = 0; d < DIM; ++d) G[d][q] = R[d];
+      lg.gref(g, T, s.ic, q, Gp, Rg);
+      for (int d = 0; d < DIM; ++d) G[d][q] = Rg[d];
     }
-    integrate_line<DIM, K, false, true>(s, i1, i2, T, V, G, out[a]);
-    add_faces<DIM, K>(s, Fq + a * 2 * DIM * C::NFQ, T, out[a], 0xffu);
+    integrate_line<DIM, K, false, true, R>(s, i1, i2, T, V, G, out[a]);
+    add_faces<DIM, K, R>(s, Fq + a * 2 * DIM * C::NFQ, T, out[a], 0xffu);
   }
 }
 
@@ -468,7 +467,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, Tab T, double f, int6
   const Nbrs<DIM> nb(g, s.ic);
   const FacePts<DIM, K> fp(g, T, s);
   double out[DIM][N1];
-  apply_B_rows<DIM, K>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, shF + cbs * Lds<DIM, K>::F * DIM, out);
+  apply_B_rows<DIM, K, double>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, shF + cbs * Lds<DIM, K>::F * DIM, out);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) store_row<DIM, K>(dst + a * N, s, out[a]);
 }
@@ -476,67 +475,74 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, Tab T, double f, int6
 // ---------------------------------------------------------------------------------------
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
-template <int DIM, int K, bool WITH_D>
-__global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, Tab T, double cdt, int64_t N,
-                                                     const double* __restrict__ pfix, const double* __restrict__ w,
-                                                     const double* __restrict__ pD, double scaleD,
-                                                     double* __restrict__ dst) {
-  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
-  __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
-  AFL_SLOT_PTRS
-  constexpr double gr = kGamma / (kGamma - 1.0), igm1 = 1.0 / (kGamma - 1.0);
-  const double sgn = WITH_D ? -1.0 : 1.0;
-  LineGeo<DIM, K> lg;
+template <int DIM, int K, bool WITH_D, typename R = double>
+__global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+                                                     const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
+                                                     R* __restrict__ dst) {
+  __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+  using C = Cfg<DIM, K>;
+  constexpr int NQ = C::NQ, N1 = C::N1;
+  Slot<DIM, K> s(g);
+  const int cbs = s.cb < C::CPB ? s.cb : 0;
+  R* t1 = shw + cbs * Lds<DIM, K>::WF;
+  R* t2 = t1 + C::EV1;
+  R* i1 = t1;
+  R* i2 = t1 + C::IN1;
+  R* fn = t1 + Lds<DIM, K>::W;
+  const R gr = R(kGamma / (kGamma - 1.0)), igm1 = R(1.0 / (kGamma - 1.0)), half = R(0.5);
+  const R sgn = WITH_D ? R(-1) : R(1);
+  LineGeo<DIM, K, R> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K> fp(g, T, s);
-  double pq[NQ], G[DIM][NQ], V[NQ];
-  double trp[2 * DIM][2], trw[DIM][2 * DIM][2];
-  eval_line_faces<DIM, K>(pfix, s, nb, t1, t2, fn, T, pq, trp);
+  const FacePts<DIM, K, R> fp(g, T, s);
+  R pq[NQ], G[DIM][NQ], V[NQ];
+  R trp[2 * DIM][2], trw[DIM][2 * DIM][2];
+  eval_line_faces<DIM, K, R>(pfix, s, nb, t1, t2, fn, T, pq, trp);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    double wq[NQ];
-    eval_line_faces<DIM, K>(w + d * N, s, nb, t1, t2, fn, T, wq, trw[d]);
+    R wq[NQ];
+    eval_line_faces<DIM, K, R>(w + d * N, s, nb, t1, t2, fn, T, wq, trw[d]);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) G[d][q] = sgn * (-cdt * gr * pq[q] * wq[q]);
   }
   if (WITH_D) {
-    eval_line<DIM, K>(pD, s, t1, t2, T, V);
+    eval_line<DIM, K, R>(pD, s, t1, t2, T, V);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    double Gp[3] = {0, 0, 0}, R[3];
+    R Gp[3] = {0, 0, 0}, Rg[3];
     for (int d = 0; d < DIM; ++d) Gp[d] = G[d][q];
-    lg.gref(g, T, s.ic, q, Gp, R);
-    for (int d = 0; d < DIM; ++d) G[d][q] = R[d];
+    lg.gref(g, T, s.ic, q, Gp, Rg);
+    for (int d = 0; d < DIM; ++d) G[d][q] = Rg[d];
   }
-  double* Fq = shF + cbs * Lds<DIM, K>::F;
+  R* Fq = shF + cbs * Lds<DIM, K>::F;
   if (s.act) {
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
-      double F = 0.0;
+      R F = R(0);
       if (nb.c[f] >= 0) {
-        const double pp = trp[f][0], pm = trp[f][1];
-        double avg = 0.0, wnp = 0.0, wnm = 0.0;
+        const R pp = trp[f][0], pm = trp[f][1];
+        R avg = R(0), wnp = R(0), wnm = R(0);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-          avg += 0.5 * gr * (pp * trw[d][f][0] + pm * trw[d][f][1]) * fp.n[f][d];
+          avg += half * gr * (pp * trw[d][f][0] + pm * trw[d][f][1]) * fp.n[f][d];
           wnp += trw[d][f][0] * fp.n[f][d];
           wnm += trw[d][f][1] * fp.n[f][d];
         }
-        const double lam = fmax(fabs(wnp), fabs(wnm));
-        F = sgn * cdt * (avg + 0.5 * lam * igm1 * (pp - pm)) * fp.jxw[f];
+        const R lam = fmax(fabs(wnp), fabs(wnm));
+        F = sgn * cdt * (avg + half * lam * igm1 * (pp - pm)) * fp.jxw[f];
       }
       Fq[f * C::NFQ + s.l] = F;
     }
   }
   __syncthreads();
-  double out[N1];
-  integrate_line<DIM, K, WITH_D, true>(s, i1, i2, T, V, G, out);
-  add_faces<DIM, K>(s, Fq, T, out, 0xffu);
-  store_row<DIM, K>(dst, s, out);
+  R out[N1];
+  integrate_line<DIM, K, WITH_D, true, R>(s, i1, i2, T, V, G, out);
+  add_faces<DIM, K, R>(s, Fq, T, out, 0xffu);
+  store_row<DIM, K, R>(dst, s, out);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -556,28 +562,28 @@ struct SolvLds {
   static constexpr int PER = WF + C::NPC;            // + the component's dofs
 };
 
-template <int DIM, int K, int NC>
-__device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const LineGeo<DIM, K>& lg,
-                                 const double (&co)[2 * K + 1], double* sl /* NC*PER per cell */,
-                                 double tol, int maxit, const double (&b)[NC][K + 1], double (&x)[NC][K + 1]) {
+template <int DIM, int K, int NC, typename R = double>
+__device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
+                                 const R (&co)[2 * K + 1], R* sl /* NC*PER per cell */,
+                                 R tol, int maxit, const R (&b)[NC][K + 1], R (&x)[NC][K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ, N1 = C::N1, ROWS = C::ROWS, NPC = C::NPC;
   constexpr int WF = SolvLds<DIM, K>::WF;
-  double* dl = sl;                 // NC*NPC dofs
-  double* wk = sl + NC * NPC;      // NC*WF work
-  double* t1 = wk;
-  double* t2 = wk + NC * C::EV1;
-  double* i1 = wk;
-  double* i2 = wk + NC * IntV<DIM, K>::I1;
+  R* dl = sl;                 // NC*NPC dofs
+  R* wk = sl + NC * NPC;      // NC*WF work
+  R* t1 = wk;
+  R* t2 = wk + NC * C::EV1;
+  R* i1 = wk;
+  R* i2 = wk + NC * IntV<DIM, K>::I1;
   const bool row_owner = s.act && s.l < ROWS;
-  __shared__ double red[3 * AFL_NT];
+  __shared__ R red[3 * AFL_NT];
   __shared__ int nit_blk;
   // cell mean and spectral bounds of P^{-1}A
   {
-    double lsum = 0.0, lmin = 1e300, lmax = 0.0;
+    R lsum = R(0), lmin = R(1e30), lmax = R(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const double r = co[q] / (lg.wxy * T.W[q]);
+      const R r = co[q] / (lg.wxy * T.W[q]);
       lsum += co[q];
       lmin = fmin(lmin, r);
       lmax = fmax(lmax, r);
@@ -588,10 +594,10 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
     if (threadIdx.x == 0) nit_blk = 1;
   }
   __syncthreads();
-  double mean = 1.0, cmin = 1.0, cmax = 1.0;
+  R mean = R(1), cmin = R(1), cmax = R(1);
   if (s.act) {
     const int base = s.cb * C::LT;
-    double a = 0.0, mn = 1e300, mx = 0.0;
+    R a = R(0), mn = R(1e30), mx = R(0);
     for (int t = 0; t < C::LT; ++t) {
       a += red[base + t];
       mn = fmin(mn, red[AFL_NT + base + t]);
@@ -599,30 +605,30 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
     }
     mean = a; cmin = mn; cmax = mx;
   }
-  const double alpha = cmin / mean, beta = cmax / mean;
-  const double theta = 0.5 * (alpha + beta);
-  const double delta = fmax(0.5 * (beta - alpha), 1e-14 * theta);
-  const double sigma1 = theta / delta;
+  const R alpha = cmin / mean, beta = cmax / mean;
+  const R theta = R(0.5) * (alpha + beta);
+  const R delta = fmax(R(0.5) * (beta - alpha), R(1e-6) * R(1e-8) * theta);
+  const R sigma1 = theta / delta;
   {
-    const double sk = sqrt(beta / alpha);
+    const double sk = sqrt((double)beta / (double)alpha);
     const double sg = (sk - 1.0) / (sk + 1.0);
     int n = 1;
-    if (sg > 1e-300) n = (int)ceil(log(0.5 * tol) / log(sg));
+    if (sg > 1e-300) n = (int)ceil(log(0.5 * (double)tol) / log(sg));
     n = n < 1 ? 1 : (n > maxit ? maxit : n);
     if (s.act && s.l == 0) atomicMax(&nit_blk, n);
   }
   __syncthreads();
   const int nit = nit_blk;
-  const double pinv = 1.0 / mean;
+  const R pinv = R(1) / mean;
 
   // z = P^{-1} r for all components (rows in registers)
-  auto precond = [&](const double (&r)[NC][N1], double (&z)[NC][N1]) {
+  auto precond = [&](const R (&r)[NC][N1], R (&z)[NC][N1]) {
     if (row_owner) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          double a = 0.0;
+          R a = R(0);
 #pragma unroll
           for (int j = 0; j < N1; ++j) a += T.MI[i][j] * r[c][j];
           dl[c * NPC + s.l * N1 + i] = a;
@@ -633,12 +639,12 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
       if (s.act && s.l < N1) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          double v[N1];
+          R v[N1];
 #pragma unroll
           for (int j = 0; j < N1; ++j) v[j] = dl[c * NPC + j * N1 + s.l];
 #pragma unroll
           for (int i = 0; i < N1; ++i) {
-            double a = 0.0;
+            R a = R(0);
 #pragma unroll
             for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
             t1[c * NPC + i * N1 + s.l] = a;
@@ -656,12 +662,12 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
         const int ix = s.l % N1, iz = s.l / N1;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          double v[N1];
+          R v[N1];
 #pragma unroll
           for (int j = 0; j < N1; ++j) v[j] = dl[c * NPC + (iz * N1 + j) * N1 + ix];
 #pragma unroll
           for (int i = 0; i < N1; ++i) {
-            double a = 0.0;
+            R a = R(0);
 #pragma unroll
             for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
             t1[c * NPC + (iz * N1 + i) * N1 + ix] = a;
@@ -673,12 +679,12 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
         const int ix = s.l % N1, iy = s.l / N1;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          double v[N1];
+          R v[N1];
 #pragma unroll
           for (int j = 0; j < N1; ++j) v[j] = t1[c * NPC + (j * N1 + iy) * N1 + ix];
 #pragma unroll
           for (int i = 0; i < N1; ++i) {
-            double a = 0.0;
+            R a = R(0);
 #pragma unroll
             for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
             dl[c * NPC + (i * N1 + iy) * N1 + ix] = a;
@@ -695,29 +701,29 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
     __syncthreads();
   };
   // y = A v for all components
-  auto apply = [&](const double (&v)[NC][N1], double (&y)[NC][N1]) {
+  auto apply = [&](const R (&v)[NC][N1], R (&y)[NC][N1]) {
     if (row_owner)
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int i = 0; i < N1; ++i) dl[c * NPC + s.l * N1 + i] = v[c][i];
     __syncthreads();
-    double vq[NC][NQ];
-    eval_lines_lds<DIM, K, NC>(dl, s, t1, t2, T, vq);
+    R vq[NC][NQ];
+    eval_lines_lds<DIM, K, NC, R>(dl, s, t1, t2, T, vq);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int q = 0; q < NQ; ++q) vq[c][q] *= co[q];
-    integrate_lines_v<DIM, K, NC>(s, i1, i2, T, vq, y);
+    integrate_lines_v<DIM, K, NC, R>(s, i1, i2, T, vq, y);
   };
 
-  double r[NC][N1], z[NC][N1], d[NC][N1];
+  R r[NC][N1], z[NC][N1], d[NC][N1];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int i = 0; i < N1; ++i) { x[c][i] = 0.0; r[c][i] = row_owner ? b[c][i] : 0.0; }
+    for (int i = 0; i < N1; ++i) { x[c][i] = R(0); r[c][i] = row_owner ? b[c][i] : R(0); }
   precond(r, z);
-  double rho = 1.0 / sigma1;
+  R rho = R(1) / sigma1;
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -728,18 +734,18 @@ __device__ inline void cell_cheb(const Tab& T, const Slot<DIM, K>& s, const Line
 #pragma unroll
       for (int i = 0; i < N1; ++i) x[c][i] += d[c][i];
     if (k == nit - 1) break;
-    double ad[NC][N1];
+    R ad[NC][N1];
     apply(d, ad);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int i = 0; i < N1; ++i) r[c][i] -= ad[c][i];
     precond(r, z);
-    const double rn = 1.0 / (2.0 * sigma1 - rho);
+    const R rn = R(1) / (R(2) * sigma1 - rho);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) d[c][i] = rn * rho * d[c][i] + (2.0 * rn / delta) * z[c][i];
+      for (int i = 0; i < N1; ++i) d[c][i] = rn * rho * d[c][i] + (R(2) * rn / delta) * z[c][i];
     rho = rn;
   }
 }
@@ -768,45 +774,44 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const doubl
 #pragma unroll
     for (int i = 0; i < N1; ++i)
       bb[c][i] = (s.act && s.l < C::ROWS) ? b[c * N + s.cell * C::NPC + s.l * N1 + i] : 0.0;
-  cell_cheb<DIM, K, NC>(T, s, lg, co, base, tol, maxit, bb, xx);
+  cell_cheb<DIM, K, NC, double>(T, s, lg, co, base, tol, maxit, bb, xx);
 #pragma unroll
   for (int c = 0; c < NC; ++c) store_row<DIM, K>(x + c * N, s, xx[c]);
 }
 
 // Schur first half: w = A^{-1} B p (fused B apply + cell-local A solve).
-template <int DIM, int K>
-__global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, Tab T, const double* __restrict__ rho, double f,
-                                                     int64_t N, const double* __restrict__ p, double* __restrict__ w,
-                                                     double tol, int maxit) {
+template <int DIM, int K, typename R = double>
+__global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, TabT<R> T, const R* __restrict__ rho, R f, int64_t N,
+                                                     const R* __restrict__ p, R* __restrict__ w, R tol, int maxit) {
   constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;  // B phase: work + face nodes + face fluxes
-  constexpr int PS = DIM * SolvLds<DIM, K>::PER;               // solver phase
+  constexpr int PS = DIM * SolvLds<DIM, K>::PER;              // solver phase
   constexpr int PC = PB > PS ? PB : PS;
-  __shared__ double shw[Cfg<DIM, K>::CPB * PC];
+  __shared__ R shw[Cfg<DIM, K>::CPB * PC];
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ, N1 = C::N1;
   Slot<DIM, K> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
-  double* base = shw + cbs * PC;
-  double* t1 = base;
-  double* t2 = base + C::EV1;
-  double* i1 = base;
-  double* i2 = base + C::IN1;
-  double* fn = base + Lds<DIM, K>::W;
-  double* Fq = base + Lds<DIM, K>::WF;
-  LineGeo<DIM, K> lg;
+  R* base = shw + cbs * PC;
+  R* t1 = base;
+  R* t2 = base + C::EV1;
+  R* i1 = base;
+  R* i2 = base + C::IN1;
+  R* fn = base + Lds<DIM, K>::W;
+  R* Fq = base + Lds<DIM, K>::WF;
+  LineGeo<DIM, K, R> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K> fp(g, T, s);
-  double bp[DIM][N1];
-  apply_B_rows<DIM, K>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, Fq, bp);
-  double co[NQ];
-  eval_line<DIM, K>(rho, s, t1, t2, T, co);
+  const FacePts<DIM, K, R> fp(g, T, s);
+  R bp[DIM][N1];
+  apply_B_rows<DIM, K, R>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, Fq, bp);
+  R co[NQ];
+  eval_line<DIM, K, R>(rho, s, t1, t2, T, co);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) co[q] *= lg.jxw(g, T, q);
-  double xx[DIM][N1];
-  cell_cheb<DIM, K, DIM>(T, s, lg, co, base, tol, maxit, bp, xx);
+  R xx[DIM][N1];
+  cell_cheb<DIM, K, DIM, R>(T, s, lg, co, base, tol, maxit, bp, xx);
 #pragma unroll
-  for (int c = 0; c < DIM; ++c) store_row<DIM, K>(w + c * N, s, xx[c]);
+  for (int c = 0; c < DIM; ++c) store_row<DIM, K, R>(w + c * N, s, xx[c]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -935,9 +940,9 @@ void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const d
   const double cdt = coeff * o.dt;
   AFL_DISPATCH(o.dim, o.k, {
     if (pD)
-      k_apply_C<D_, K_, true><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
+      k_apply_C<D_, K_, true, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
     else
-      k_apply_C<D_, K_, false><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, cdt, o.N, pfix, w, nullptr, 0.0, dst);
+      k_apply_C<D_, K_, false, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, cdt, o.N, pfix, w, nullptr, 0.0, dst);
   });
 }
 
@@ -956,8 +961,24 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
-    k_schur_w<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, f, o.N, p, w, tol2, o.cell_maxit);
+    k_schur_w<D_, K_, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, f, o.N, p, w, tol2, o.cell_maxit);
   });
 }
+
+// Level Schur apply y = D p - C A^{-1} B p in precision R (multigrid levels).
+template <typename R>
+void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
+                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
+  AFL_DISPATCH(dim, k, {
+    const dim3 gr = grid_for<D_, K_>(geo.ncells);
+    k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    k_apply_C<D_, K_, true, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
+  });
+}
+template void launch_schur_pair<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+                                       const float*, float, float, float, int, int64_t, const float*, float*, float*);
+template void launch_schur_pair<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
+                                        const double*, double, double, double, int, int64_t, const double*, double*,
+                                        double*);
 
 }  // namespace afl

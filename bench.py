@@ -130,21 +130,30 @@ def main():
     t_schur = timed(lambda: op.vmult(dst_s, p), args.reps)
     t_sw = timed(lambda: op.vmult_schur_velocity(dst_v, p), args.reps)
 
-    # IMEX steps
-    infos = []
-    for _ in range(args.warmup):
-        op.imex_step(rho, u, p)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        infos.append(op.imex_step(rho, u, p))
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_s = wall / args.steps
+    # IMEX steps: FP64 plain path (Jacobi GMRES) and FP32 p-multigrid FGMRES, same initial
+    # state, same true-residual tolerance 1e-10
+    from atmospheric_flows_amd import make_solver_params
+    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32
+
+    def run_path(precond):
+        o = EulerOperator(mesh, mach=1.0, froude=FR, dt=1.0)
+        o.set_solver_params(make_solver_params(gmres_eps=1e-10, gmres_restart=28, fp_tol=1e-10,
+                                               schur_precond=precond))
+        r0, u0, p0 = o.init_hill_state(st)
+        infos = []
+        for _ in range(args.warmup):
+            o.imex_step(r0, u0, p0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            infos.append(o.imex_step(r0, u0, p0))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps, infos, (r0, u0, p0)
+
+    step_fp64, infos64, s64 = run_path(AFL_SCHUR_JACOBI)
+    step_s, infos, s32 = run_path(AFL_SCHUR_MG_FP32)
+    dev = {name: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+           for name, a, b in (("rho", s32[0], s64[0]), ("u", s32[1], s64[1]), ("p", s32[2], s64[2]))}
     dofs = (2 + dim) * N
 
     # roofline of the dominant kernel of the step (w = A^{-1} B p, inside every Schur apply):
@@ -165,11 +174,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"2D Agnesi hill {args.nx}x{args.nz} cells, k={args.degree}, FP64 plain path "
-                               f"(Jacobi-GMRES Schur), dt=1 s, tol 1e-10",
-                   "n_per_field": N, "dof": dofs, "solver": "fp64-jacobi-gmres"},
+        "config": {"workload": f"2D Agnesi hill {args.nx}x{args.nz} cells, k={args.degree}, FP64 state, "
+                               f"FP32 p-multigrid FGMRES Schur solve, dt=1 s, true residual 1e-10",
+                   "n_per_field": N, "dof": dofs, "solver": "fgmres+fp32-pmg"},
         "fp_iters_per_step": [list(i.fp_iters) for i in infos],
         "gmres_iters_per_step": [list(i.gmres_iters) for i in infos],
+        "fp64_plain_path": {"ms_per_step": step_fp64 * 1e3, "value": dofs / step_fp64,
+                            "gmres_iters_per_step": [list(i.gmres_iters) for i in infos64],
+                            "fp_iters_per_step": [list(i.fp_iters) for i in infos64]},
+        "speedup_fp32mg_vs_fp64": step_fp64 / step_s,
+        "rel_l2_fp32mg_vs_fp64_after_steps": dev,
         "isolated_applies_us": {"E1_stage3": t_e1 * 1e6, "E2_stage3": t_e2 * 1e6, "E3_stage3": t_e3 * 1e6,
                                 "schur_vmult": t_schur * 1e6, "schur_velocity_kernel": t_sw * 1e6},
         "isolated_applies_GBs": {"E1_stage3": 10 * field_bytes / t_e1 / 1e9,

@@ -47,7 +47,20 @@ typedef struct afl_solver_params {
   double fp_tol;           /* fixed-point tolerance on the relative inf-norm change of p [1e-10] */
   int32_t fp_max_iter;     /* [100] */
   int32_t pad_;
+  /* Schur preconditioner (zero-initialised fields select the reference's plain path):
+   * AFL_SCHUR_JACOBI: GMRES + point Jacobi on D, all FP64;
+   * AFL_SCHUR_MG_FP32 / AFL_SCHUR_MG_FP64: flexible GMRES (FP64) + p-multigrid V-cycle with
+   * Chebyshev smoothers run in FP32 / FP64. Non-positive MG fields take the defaults below. */
+  int32_t schur_precond;
+  int32_t mg_smoother_degree;  /* Chebyshev degree of pre/post smoothing [3] */
+  int32_t mg_coarse_degree;    /* Chebyshev degree on the coarsest level [16] */
+  int32_t mg_levels;           /* maximum number of degree levels k, k/2, ..., 1 [1]; with one level the
+                                * V-cycle is the Chebyshev polynomial preconditioner on the fine level */
+  double mg_smoothing_range;   /* lambda_max / lambda_min of the smoothed interval [15] */
+  double mg_cell_tol;          /* cell-local A-solve tolerance inside the levels [0.05] */
 } afl_solver_params;
+
+enum afl_schur_precond { AFL_SCHUR_JACOBI = 0, AFL_SCHUR_MG_FP32 = 1, AFL_SCHUR_MG_FP64 = 2 };
 
 /* Per-step diagnostics (the reference prints these through pcout). */
 typedef struct afl_step_info {
