@@ -546,13 +546,16 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int
 }
 
 // ---------------------------------------------------------------------------------------
-// Cell-local solver for the block-diagonal mass-type matrices A = sum_q co(q) phi_i phi_j
-// (co = scale rho JxW): Chebyshev iteration (Saad, Iterative Methods, Alg. 12.1) preconditioned
-// with P = mean(co) * (M1 x M1 (x M1)), M1 the reference 1D mass matrix. The spectrum of
-// P^{-1} A lies in [min_q co/(mean w_q), max_q co/(mean w_q)] (w_q reference weights), so the
-// interval and the iteration count for the requested A-norm tolerance (2 sigma^n <= tol) are
-// known per cell up front: no inner products, no convergence flags. All NC components share
-// every barrier. b rows in, x rows out.
+// Cell-local solver for the block-diagonal mass-type matrices A = S^T diag(co) S (co = scale rho
+// JxW at the quadrature points, S the tensor interpolation matrix): Chebyshev iteration (Saad,
+// Iterative Methods, Alg. 12.1) preconditioned with the separable matrix P = S^T diag(c~) S,
+// c~ = a (x) b [(x) c] / total^(dim-1) built from the marginal sums of co over the cell. P is a
+// Kronecker product of dim weighted 1D mass matrices, so P^{-1} is the product of their 5x5
+// inverses. The Rayleigh quotient of P^{-1} A lies in [min co/c~, max co/c~]: the interval and
+// the iteration count for the requested A-norm tolerance (2 sigma^n <= tol) are known per cell
+// up front, with no inner products and no convergence flags. On the terrain-following mesh the
+// Jacobian factorises exactly, so mass and D solves finish in one step and density-weighted
+// solves in a few. All NC components share every barrier. b rows in, x rows out.
 template <int DIM, int K>
 struct SolvLds {
   using C = Cfg<DIM, K>;
@@ -560,14 +563,37 @@ struct SolvLds {
   static constexpr int INF = IntV<DIM, K>::I1 + IntV<DIM, K>::I2;
   static constexpr int WF = EVF > INF ? EVF : INF;  // per component work area
   static constexpr int PER = WF + C::NPC;            // + the component's dofs
+  // set-up scratch: co per line, line sums (3D), marginals, per-line ratio bounds
+  static constexpr int SETUP = C::LT * C::NQ + (DIM == 3 ? C::LT : 0) + DIM * C::NQ + 2 * C::LT;
+  static constexpr int SEP = DIM * C::N1 * C::N1;    // the dim inverse 1D matrices (persist)
+  static constexpr int sep_off(int nc) { return nc * PER > SETUP ? nc * PER : SETUP; }
+  static constexpr int size(int nc) { return sep_off(nc) + SEP; }
 };
+
+// In-place inverse of a small SPD matrix (Gauss-Jordan without pivoting, fully unrolled).
+template <int N, typename R>
+__device__ inline void inv_spd(R (&A)[N][N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const R p = R(1) / A[k][k];
+#pragma unroll
+    for (int j = 0; j < N; ++j) A[k][j] = j == k ? p : A[k][j] * p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i == k) continue;
+      const R f = A[i][k];
+#pragma unroll
+      for (int j = 0; j < N; ++j) A[i][j] = j == k ? -f * p : A[i][j] - f * A[k][j];
+    }
+  }
+}
 
 template <int DIM, int K, int NC, typename R = double>
 __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
-                                 const R (&co)[2 * K + 1], R* sl /* NC*PER per cell */,
+                                 const R (&co)[2 * K + 1], R* sl /* SolvLds::size(NC) per cell */,
                                  R tol, int maxit, const R (&b)[NC][K + 1], R (&x)[NC][K + 1]) {
   using C = Cfg<DIM, K>;
-  constexpr int NQ = C::NQ, N1 = C::N1, ROWS = C::ROWS, NPC = C::NPC;
+  constexpr int NQ = C::NQ, N1 = C::N1, ROWS = C::ROWS, NPC = C::NPC, LT = C::LT;
   constexpr int WF = SolvLds<DIM, K>::WF;
   R* dl = sl;                 // NC*NPC dofs
   R* wk = sl + NC * NPC;      // NC*WF work
@@ -575,39 +601,91 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   R* t2 = wk + NC * C::EV1;
   R* i1 = wk;
   R* i2 = wk + NC * IntV<DIM, K>::I1;
+  R* Mi = sl + SolvLds<DIM, K>::sep_off(NC);  // [DIM][N1][N1]: x, (y,) z inverse 1D matrices
   const bool row_owner = s.act && s.l < ROWS;
-  __shared__ R red[3 * AFL_NT];
   __shared__ int nit_blk;
-  // cell mean and spectral bounds of P^{-1}A
+  // ---- set-up: marginals of co, separable weights, ratio bounds, 1D inverses ----
+  R* sco = sl;                                   // [LT][NQ]
+  R* sL = sco + LT * NQ;                         // [LT] line sums (3D)
+  R* smg = sL + (DIM == 3 ? LT : 0);             // [DIM][NQ] marginals a, (b,) c
+  R* srm = smg + DIM * NQ;                       // [2][LT] per-line min / max of co / c~
+  if (threadIdx.x == 0) nit_blk = 1;
+  R lsum = R(0);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) lsum += co[q];
+  if (s.act) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sco[s.l * NQ + q] = co[q];
+    if (DIM == 3) sL[s.l] = lsum;
+  }
+  __syncthreads();
+  if (s.act && s.l < NQ) {
+    // vertical marginal: sum over the lines of co at qz = l
+    R c = R(0);
+    for (int t = 0; t < LT; ++t) c += sco[t * NQ + s.l];
+    smg[(DIM - 1) * NQ + s.l] = c;
+    if (DIM == 2) {
+      smg[s.l] = lsum;  // x marginal: the thread's own line sum
+    } else {
+      R ax = R(0), by = R(0);
+#pragma unroll
+      for (int t = 0; t < NQ; ++t) { ax += sL[t * NQ + s.l]; by += sL[s.l * NQ + t]; }
+      smg[s.l] = ax;       // sum over qy of the lines (qx = l, qy)
+      smg[NQ + s.l] = by;  // sum over qx of the lines (qx, qy = l)
+    }
+  }
+  __syncthreads();
+  R total = R(0);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) total += smg[q];
+  const R itot = R(1) / total;
   {
-    R lsum = R(0), lmin = R(1e30), lmax = R(0);
+    // bounds of co / c~ on the thread's own line, c~ = prod_d marginal_d / total^(dim-1)
+    const int qx = s.l % NQ, qy = DIM == 3 ? s.l / NQ : 0;
+    const R hor = DIM == 2 ? smg[qx] : smg[qx] * smg[NQ + qy] * itot;
+    R lmin = R(1e30), lmax = R(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const R r = co[q] / (lg.wxy * T.W[q]);
-      lsum += co[q];
+      const R r = co[q] / (hor * smg[(DIM - 1) * NQ + q] * itot);
       lmin = fmin(lmin, r);
       lmax = fmax(lmax, r);
     }
-    red[threadIdx.x] = lsum;
-    red[AFL_NT + threadIdx.x] = lmin;
-    red[2 * AFL_NT + threadIdx.x] = lmax;
-    if (threadIdx.x == 0) nit_blk = 1;
+    if (s.act) { srm[s.l] = lmin; srm[LT + s.l] = lmax; }
+  }
+  // 1D weighted mass matrices and their inverses: thread l < DIM of each cell builds axis l
+  if (s.act && s.l < DIM) {
+    R Am[N1][N1];
+    const R sc = s.l == 0 ? R(1) : itot;
+    const R* wq = smg + s.l * NQ;
+#pragma unroll
+    for (int i = 0; i < N1; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        R a = R(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) a += wq[q] * T.S[q][i] * T.S[q][j];
+        Am[i][j] = sc * a;
+        Am[j][i] = sc * a;
+      }
+    inv_spd<N1, R>(Am);
+#pragma unroll
+    for (int i = 0; i < N1; ++i)
+#pragma unroll
+      for (int j = 0; j < N1; ++j) Mi[(s.l * N1 + i) * N1 + j] = Am[i][j];
   }
   __syncthreads();
-  R mean = R(1), cmin = R(1), cmax = R(1);
+  R cmin = R(1), cmax = R(1);
   if (s.act) {
-    const int base = s.cb * C::LT;
-    R a = R(0), mn = R(1e30), mx = R(0);
-    for (int t = 0; t < C::LT; ++t) {
-      a += red[base + t];
-      mn = fmin(mn, red[AFL_NT + base + t]);
-      mx = fmax(mx, red[2 * AFL_NT + base + t]);
+    R mn = R(1e30), mx = R(0);
+    for (int t = 0; t < LT; ++t) {
+      mn = fmin(mn, srm[t]);
+      mx = fmax(mx, srm[LT + t]);
     }
-    mean = a; cmin = mn; cmax = mx;
+    cmin = mn; cmax = mx;
   }
-  const R alpha = cmin / mean, beta = cmax / mean;
+  const R alpha = cmin, beta = cmax;
   const R theta = R(0.5) * (alpha + beta);
-  const R delta = fmax(R(0.5) * (beta - alpha), R(1e-6) * R(1e-8) * theta);
+  const R delta = fmax(R(0.5) * (beta - alpha), R(1e-14) * theta);
   const R sigma1 = theta / delta;
   {
     const double sk = sqrt((double)beta / (double)alpha);
@@ -619,34 +697,45 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   }
   __syncthreads();
   const int nit = nit_blk;
-  const R pinv = R(1) / mean;
+  const R* MX = Mi;
+  const R* MY = Mi + N1 * N1;
+  const R* MZ = Mi + (DIM - 1) * N1 * N1;
 
   // z = P^{-1} r for all components (rows in registers)
   auto precond = [&](const R (&r)[NC][N1], R (&z)[NC][N1]) {
     if (row_owner) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
+      for (int i = 0; i < N1; ++i) {
+        R m[N1];
 #pragma unroll
-        for (int i = 0; i < N1; ++i) {
+        for (int j = 0; j < N1; ++j) m[j] = MX[i * N1 + j];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
           R a = R(0);
 #pragma unroll
-          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * r[c][j];
+          for (int j = 0; j < N1; ++j) a += m[j] * r[c][j];
           dl[c * NPC + s.l * N1 + i] = a;
         }
+      }
     }
     __syncthreads();
     if (DIM == 2) {
       if (s.act && s.l < N1) {
+        R v[NC][N1];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          R v[N1];
+        for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int j = 0; j < N1; ++j) v[j] = dl[c * NPC + j * N1 + s.l];
+          for (int j = 0; j < N1; ++j) v[c][j] = dl[c * NPC + j * N1 + s.l];
 #pragma unroll
-          for (int i = 0; i < N1; ++i) {
+        for (int i = 0; i < N1; ++i) {
+          R m[N1];
+#pragma unroll
+          for (int j = 0; j < N1; ++j) m[j] = MZ[i * N1 + j];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+            for (int j = 0; j < N1; ++j) a += m[j] * v[c][j];
             t1[c * NPC + i * N1 + s.l] = a;
           }
         }
@@ -656,7 +745,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int i = 0; i < N1; ++i) z[c][i] = pinv * t1[c * NPC + s.l * N1 + i];
+          for (int i = 0; i < N1; ++i) z[c][i] = t1[c * NPC + s.l * N1 + i];
     } else {
       if (s.act && s.l < N1 * N1) {
         const int ix = s.l % N1, iz = s.l / N1;
@@ -669,7 +758,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
           for (int i = 0; i < N1; ++i) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+            for (int j = 0; j < N1; ++j) a += MY[i * N1 + j] * v[j];
             t1[c * NPC + (iz * N1 + i) * N1 + ix] = a;
           }
         }
@@ -686,7 +775,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
           for (int i = 0; i < N1; ++i) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+            for (int j = 0; j < N1; ++j) a += MZ[i * N1 + j] * v[j];
             dl[c * NPC + (i * N1 + iy) * N1 + ix] = a;
           }
         }
@@ -696,7 +785,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int i = 0; i < N1; ++i) z[c][i] = pinv * dl[c * NPC + s.l * N1 + i];
+          for (int i = 0; i < N1; ++i) z[c][i] = dl[c * NPC + s.l * N1 + i];
     }
     __syncthreads();
   };
@@ -751,10 +840,10 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 }
 
 template <int DIM, int K, int NC>
-__global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const double* __restrict__ rho, double scale,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_mass_solve(Geo g, Tab T, const double* __restrict__ rho, double scale,
                                                         int64_t N, const double* __restrict__ b,
                                                         double* __restrict__ x, double tol, int maxit) {
-  constexpr int PC = Lds<DIM, K>::W > NC * SolvLds<DIM, K>::PER ? Lds<DIM, K>::W : NC * SolvLds<DIM, K>::PER;
+  constexpr int PC = Lds<DIM, K>::W > SolvLds<DIM, K>::size(NC) ? Lds<DIM, K>::W : SolvLds<DIM, K>::size(NC);
   __shared__ double shw[Cfg<DIM, K>::CPB * PC];
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ, N1 = C::N1;
@@ -781,10 +870,10 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const doubl
 
 // Schur first half: w = A^{-1} B p (fused B apply + cell-local A solve).
 template <int DIM, int K, typename R = double>
-__global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, TabT<R> T, const R* __restrict__ rho, R f, int64_t N,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_schur_w(Geo g, TabT<R> T, const R* __restrict__ rho, R f, int64_t N,
                                                      const R* __restrict__ p, R* __restrict__ w, R tol, int maxit) {
   constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;  // B phase: work + face nodes + face fluxes
-  constexpr int PS = DIM * SolvLds<DIM, K>::PER;              // solver phase
+  constexpr int PS = SolvLds<DIM, K>::size(DIM);             // solver phase
   constexpr int PC = PB > PS ? PB : PS;
   __shared__ R shw[Cfg<DIM, K>::CPB * PC];
   using C = Cfg<DIM, K>;
