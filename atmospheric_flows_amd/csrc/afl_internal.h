@@ -39,6 +39,10 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
 template <typename R>
 void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
                        R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
+template <typename R>
+void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
+                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar, const R* rin, R* rout,
+                       R c1, R c2, R* d, R* x);
 
 // vector kernels (afl_blas.hip); reductions leave their result in device memory
 void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);

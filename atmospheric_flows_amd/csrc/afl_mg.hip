@@ -327,6 +327,17 @@ static void cheb_step(const MgLevelT<R>& L, bool first, const R* rin, const R* t
   });
 }
 
+// S p followed by one fused Chebyshev step: r' = rin - S p -> rout, d = c1 d + c2 P^{-1} r', x += d
+template <typename R>
+static void level_schur_cheb(const MgLevelT<R>& L, const OpCtx& fine, const R* p, const R* rin, R* rout, double c1,
+                             double c2, R* d, R* x) {
+  const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
+  const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
+  const R cdt = (R)(coeff * fine.dt);
+  launch_schur_cheb<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.pfix, f, cdt, (R)L.cell_tol, L.cell_maxit, L.N, p,
+                       L.w, L.jbar, rin, rout, (R)c1, (R)c2, d, x);
+}
+
 template <typename R>
 static void smooth(const MgLevelT<R>& L, const OpCtx& fine, const R* b, R* x, bool zero_init, int degree,
                    double lo, double hi) {
@@ -337,16 +348,14 @@ static void smooth(const MgLevelT<R>& L, const OpCtx& fine, const R* b, R* x, bo
   if (zero_init) {
     cheb_step(L, true, b, (const R*)nullptr, (R*)nullptr, 0.0, 1.0 / theta, L.d, x);
   } else {
-    level_schur(L, fine, x, L.t);
-    cheb_step(L, false, b, L.t, rnext, 0.0, 1.0 / theta, L.d, x);
+    level_schur_cheb(L, fine, x, b, rnext, 0.0, 1.0 / theta, L.d, x);
     rcur = rnext;
     std::swap(rnext, ralt);
   }
   double rho = 1.0 / sigma1;
   for (int k = 1; k < degree; ++k) {
-    level_schur(L, fine, L.d, L.t);
     const double rn = 1.0 / (2.0 * sigma1 - rho);
-    cheb_step(L, false, rcur, L.t, rnext, rn * rho, 2.0 * rn / delta, L.d, x);
+    level_schur_cheb(L, fine, L.d, rcur, rnext, rn * rho, 2.0 * rn / delta, L.d, x);
     rcur = rnext;
     std::swap(rnext, ralt);
     rho = rn;

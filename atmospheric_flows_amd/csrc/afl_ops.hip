@@ -475,17 +475,15 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, Tab T, double f, int6
 // ---------------------------------------------------------------------------------------
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
-template <int DIM, int K, bool WITH_D, typename R = double>
-__global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
-                                                     const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
-                                                     R* __restrict__ dst) {
-  __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
-  __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+// Result rows of C (optionally fused with D p) for the thread's node row: out = (WITH_D ?
+// scaleD M pD - C w : C w). sh: the cell's work area (Lds::WF), Fq: its face flux block (Lds::F).
+template <int DIM, int K, bool WITH_D, typename R>
+__device__ inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, R* sh, R* Fq, R cdt,
+                                    const R* __restrict__ pfix, const R* __restrict__ w, int64_t N,
+                                    const R* __restrict__ pD, R scaleD, R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
-  constexpr int NQ = C::NQ, N1 = C::N1;
-  Slot<DIM, K> s(g);
-  const int cbs = s.cb < C::CPB ? s.cb : 0;
-  R* t1 = shw + cbs * Lds<DIM, K>::WF;
+  constexpr int NQ = C::NQ;
+  R* t1 = sh;
   R* t2 = t1 + C::EV1;
   R* i1 = t1;
   R* i2 = t1 + C::IN1;
@@ -518,7 +516,6 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int
     lg.gref(g, T, s.ic, q, Gp, Rg);
     for (int d = 0; d < DIM; ++d) G[d][q] = Rg[d];
   }
-  R* Fq = shF + cbs * Lds<DIM, K>::F;
   if (s.act) {
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
@@ -539,10 +536,121 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int
     }
   }
   __syncthreads();
-  R out[N1];
   integrate_line<DIM, K, WITH_D, true, R>(s, i1, i2, T, V, G, out);
   add_faces<DIM, K, R>(s, Fq, T, out, 0xffu);
+}
+
+// C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
+// Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
+template <int DIM, int K, bool WITH_D, typename R = double>
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_apply_C(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+                                                     const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
+                                                     R* __restrict__ dst) {
+  __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+  Slot<DIM, K> s(g);
+  const int cbs = s.cb < Cfg<DIM, K>::CPB ? s.cb : 0;
+  R out[K + 1];
+  apply_C_rows<DIM, K, WITH_D, R>(g, T, s, shw + cbs * Lds<DIM, K>::WF, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w,
+                                  N, pD, scaleD, out);
   store_row<DIM, K, R>(dst, s, out);
+}
+
+// Schur second half fused with one Chebyshev smoothing step of the multigrid levels:
+//   t = S pD (= D pD - C w);  r' = rin - t -> rout;  z = (gamma-1)/Jbar (MI x MI [x MI]) r';
+//   d = c1 d + c2 z;  x += d.
+// The block preconditioner is cell-local (LDS transposes of the cell's rows); pD, d and x are
+// read only inside the thread's own cell, so d and x (and pD == d or x) update in place.
+template <int DIM, int K, typename R>
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_apply_C_cheb(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+                                                          const R* __restrict__ w, const R* pD, R scaleD,
+                                                          const R* __restrict__ jbar, const R* __restrict__ rin,
+                                                          R* __restrict__ rout, R c1, R c2, R* d, R* x) {
+  __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
+  Slot<DIM, K> s(g);
+  const int cbs = s.cb < C::CPB ? s.cb : 0;
+  R* sh = shw + cbs * Lds<DIM, K>::WF;
+  R t[N1];
+  apply_C_rows<DIM, K, true, R>(g, T, s, sh, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w, N, pD, scaleD, t);
+  const bool ro = s.act && s.l < ROWS;
+  const int64_t base = (s.act ? s.cell : 0) * NPC + s.l * N1;
+  R* tmp = sh;
+  R* tmp2 = sh + NPC;
+  if (ro) {
+    R r[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      r[i] = rin[base + i] - t[i];
+      rout[base + i] = r[i];
+    }
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      R a = R(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) a += T.MI[i][j] * r[j];
+      tmp[s.l * N1 + i] = a;
+    }
+  }
+  __syncthreads();
+  const R* zr;
+  if (DIM == 2) {
+    if (s.act && s.l < N1) {
+      R v[N1];
+#pragma unroll
+      for (int j = 0; j < N1; ++j) v[j] = tmp[j * N1 + s.l];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        R a = R(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+        tmp2[i * N1 + s.l] = a;
+      }
+    }
+    __syncthreads();
+    zr = tmp2;
+  } else {
+    if (s.act && s.l < N1 * N1) {
+      const int ix = s.l % N1, iz = s.l / N1;
+      R v[N1];
+#pragma unroll
+      for (int j = 0; j < N1; ++j) v[j] = tmp[(iz * N1 + j) * N1 + ix];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        R a = R(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+        tmp2[(iz * N1 + i) * N1 + ix] = a;
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < N1 * N1) {
+      const int ix = s.l % N1, iy = s.l / N1;
+      R v[N1];
+#pragma unroll
+      for (int j = 0; j < N1; ++j) v[j] = tmp2[(j * N1 + iy) * N1 + ix];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        R a = R(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+        tmp[(i * N1 + iy) * N1 + ix] = a;
+      }
+    }
+    __syncthreads();
+    zr = tmp;
+  }
+  if (ro) {
+    const R zs = R(kGamma - 1.0) / jbar[s.cell];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      const R dn = (c1 != R(0) ? c1 * d[base + i] : R(0)) + c2 * zs * zr[s.l * N1 + i];
+      d[base + i] = dn;
+      x[base + i] += dn;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1064,6 +1172,24 @@ void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStre
     k_apply_C<D_, K_, true, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
   });
 }
+// w = A^{-1} B p followed by the fused C/D apply and Chebyshev step (see k_apply_C_cheb)
+template <typename R>
+void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
+                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar, const R* rin, R* rout,
+                       R c1, R c2, R* d, R* x) {
+  AFL_DISPATCH(dim, k, {
+    const dim3 gr = grid_for<D_, K_>(geo.ncells);
+    k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    k_apply_C_cheb<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), jbar,
+                                                           rin, rout, c1, c2, d, x);
+  });
+}
+template void launch_schur_cheb<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+                                       const float*, float, float, float, int, int64_t, const float*, float*,
+                                       const float*, const float*, float*, float, float, float*, float*);
+template void launch_schur_cheb<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
+                                        const double*, double, double, double, int, int64_t, const double*, double*,
+                                        const double*, const double*, double*, double, double, double*, double*);
 template void launch_schur_pair<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
                                        const float*, float, float, float, int, int64_t, const float*, float*, float*);
 template void launch_schur_pair<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
