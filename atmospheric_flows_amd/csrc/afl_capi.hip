@@ -178,6 +178,7 @@ struct afl_operator {
   MgT<double> mg64;
   int mg_built = 0;          // precision of the built hierarchy (0: none)
   bool mg_eig_stale = true;  // rho_for_fixed changed since the last eigenvalue estimate
+  bool mg_coef_stale = true; // rho_for_fixed / pres_fixed changed since the levels copied them
   Tab tabs_by_k[5]{};        // 1D tables per degree (same node family)
   DevBuf gZ, mgw;
   afl_sponge sponge{15000.0, 0.0, {0.0, 0.0, 0.0}, 0, 0};
@@ -355,6 +356,7 @@ int ensure_mg(afl_operator* op) {
     if (rc) return fail(AFL_ERR_DEVICE, "multigrid set-up: hipMalloc failed");
     op->mg_built = want;
     op->mg_eig_stale = true;
+    op->mg_coef_stale = true;
   }
   {  // smoothing degrees and range take effect without a rebuild
     const MgConfig cfg = mg_config(op->sp);
@@ -368,8 +370,11 @@ int ensure_mg(afl_operator* op) {
   }
   if ((rc = op->mgw.alloc(op->N() + red_work_size(1) + 16))) return rc;
   // stage coefficients (rho_for_fixed, pres_fixed) on every level
-  if (want == AFL_SCHUR_MG_FP32) op->mg32.set_coefficients(op->rho_fix.p, op->p_fix.p);
-  else op->mg64.set_coefficients(op->rho_fix.p, op->p_fix.p);
+  if (op->mg_coef_stale) {
+    if (want == AFL_SCHUR_MG_FP32) op->mg32.set_coefficients(op->rho_fix.p, op->p_fix.p);
+    else op->mg64.set_coefficients(op->rho_fix.p, op->p_fix.p);
+    op->mg_coef_stale = false;
+  }
   if (op->mg_eig_stale) {
     const int it = mg_config(op->sp).eig_iters;
     rc = want == AFL_SCHUR_MG_FP32
@@ -591,12 +596,14 @@ int afl_set_ns_stage(afl_operator* op, unsigned s) {
 int afl_set_rho_for_fixed(afl_operator* op, const double* v) {
   if (!op || !v) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   vec_copy(op->o.stream, op->o.N, v, op->rho_fix.p);
+  op->mg_coef_stale = true;
   op->mg_eig_stale = true;
   return check_device("afl_set_rho_for_fixed");
 }
 int afl_set_pres_fixed(afl_operator* op, const double* v) {
   if (!op || !v) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   vec_copy(op->o.stream, op->o.N, v, op->p_fix.p);
+  op->mg_coef_stale = true;
   return check_device("afl_set_pres_fixed");
 }
 int afl_set_u_fixed(afl_operator* op, const double* v) {
@@ -696,6 +703,28 @@ int afl_vmult_schur_velocity(afl_operator* op, double* w, const double* p) {
   return check_device("vmult_schur_velocity");
 }
 
+int afl_mg_vmult(afl_operator* op, double* dst, const double* src) {
+  if (!op || !dst || !src) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int pc = op->sp.schur_precond;
+  if (pc != AFL_SCHUR_MG_FP32 && pc != AFL_SCHUR_MG_FP64)
+    return fail(AFL_ERR_STATE, "mg_vmult needs solver_params.schur_precond = AFL_SCHUR_MG_FP32 or AFL_SCHUR_MG_FP64");
+  int rc;
+  if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
+  if (pc == AFL_SCHUR_MG_FP32) op->mg32.apply(op->o, src, dst);
+  else op->mg64.apply(op->o, src, dst);
+  return check_device("mg_vmult");
+}
+
+int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {
+  if (!op || !w || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (op->sp.schur_precond != AFL_SCHUR_MG_FP32)
+    return fail(AFL_ERR_STATE, "vmult_schur_velocity_f32 needs solver_params.schur_precond = AFL_SCHUR_MG_FP32");
+  int rc;
+  if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
+  op->mg32.schur_velocity(op->o, p, w);
+  return check_device("vmult_schur_velocity_f32");
+}
+
 int afl_compute_diagonal(afl_operator* op) {
   if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
   const unsigned ns = op->ns;
@@ -769,6 +798,7 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
     for (; it < op->sp.fp_max_iter; ++it) {
       // pressure_fixed_point, Atmospheric_Flow.cc:545-673
       vec_copy(st, N, op->pfix_old.p, op->p_fix.p);
+      op->mg_coef_stale = true;
       vec_copy(st, NU, op->ufix.p, op->u_fix.p);
       if (stage == 1) {
         const double* sp_[5] = {rho, u, p, rho_new, op->ufix.p};
@@ -817,6 +847,7 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
   { const double* s[2] = {rho, u}; launch_rhs_rho(o, 1, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho2.p);
   vec_copy(st, N, op->rho2.p, op->rho_fix.p);
+  op->mg_coef_stale = true;
   op->mg_eig_stale = true;
   if ((rc = fixed_point(1, op->rho2.p, p, u, op->p2.p, op->u2.p))) return rc;
   // stage 2
@@ -824,6 +855,7 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
   { const double* s[4] = {rho, u, op->rho2.p, op->u2.p}; launch_rhs_rho(o, 2, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho3.p);
   vec_copy(st, N, op->rho3.p, op->rho_fix.p);
+  op->mg_coef_stale = true;
   op->mg_eig_stale = true;
   if ((rc = fixed_point(2, op->rho3.p, op->p2.p, op->u2.p, op->p3.p, op->u3.p))) return rc;
   // stage 3
@@ -831,6 +863,7 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
   { const double* s[6] = {rho, u, op->rho2.p, op->u2.p, op->rho3.p, op->u3.p}; launch_rhs_rho(o, 3, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rhoc.p);
   vec_copy(st, N, op->rhoc.p, op->rho_fix.p);
+  op->mg_coef_stale = true;
   {
     const double* s[9] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p};
     launch_rhs_u(o, 3, s, op->rhs_u.p);

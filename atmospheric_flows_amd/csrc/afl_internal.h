@@ -40,6 +40,9 @@ template <typename R>
 void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
                        R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
 template <typename R>
+void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, R f, R tol,
+                      int maxit, int64_t N, const R* p, R* w);
+template <typename R>
 void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
                        R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar, const R* rin, R* rout,
                        R c1, R c2, R* d, R* x);

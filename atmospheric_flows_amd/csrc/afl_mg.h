@@ -32,19 +32,23 @@ struct MgLevelT {
   int cell_maxit = 40;
   hipStream_t stream = nullptr;
   R *rho = nullptr, *pfix = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *r2 = nullptr, *z = nullptr,
-    *d = nullptr, *t = nullptr, *w = nullptr, *jbar = nullptr;
+    *d = nullptr, *t = nullptr, *w = nullptr, *jbar = nullptr, *ev = nullptr;
+  bool ev_valid = false;  // ev holds the power-iteration vector of an earlier estimate
 };
 
 template <typename R>
 struct MgT {
   std::vector<MgLevelT<R>> levels;
   int smoother_degree = 3, coarse_degree = 8;
+  int warm_eig_iters = 4;  // power iterations when restarting from the previous eigenvector
   double smoothing_range = 15.0;
   int setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k);
   void release();
   void set_coefficients(const double* rho_fine, const double* pfix_fine);
   int estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, double* scal_d, double* scal_h);
   void apply(const OpCtx& fine, const double* v, double* z);
+  // fine-level w = A^{-1} B p in precision R on the level's coefficient copies
+  void schur_velocity(const OpCtx& fine, const R* p, R* w);
   ~MgT() { release(); }
 };
 

@@ -440,7 +440,8 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
         for (int j = 0; j < 5; ++j) L.Pt[i][j] = L.P[j][i];
     }
     const int64_t nv = L.N * L.dim;
-    R** bufs[] = {&L.rho, &L.pfix, &L.x, &L.b, &L.r, &L.r2, &L.z, &L.d, &L.t};
+    R** bufs[] = {&L.rho, &L.pfix, &L.x, &L.b, &L.r, &L.r2, &L.z, &L.d, &L.t, &L.ev};
+    L.ev_valid = false;
     for (R** b : bufs)
       if (hipMalloc((void**)b, L.N * sizeof(R)) != hipSuccess) return 3;
     if (hipMalloc((void**)&L.w, nv * sizeof(R)) != hipSuccess) return 3;
@@ -455,7 +456,7 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
 template <typename R>
 void MgT<R>::release() {
   for (auto& L : levels) {
-    R* bufs[] = {L.rho, L.pfix, L.x, L.b, L.r, L.r2, L.z, L.d, L.t, L.w, L.jbar};
+    R* bufs[] = {L.rho, L.pfix, L.x, L.b, L.r, L.r2, L.z, L.d, L.t, L.w, L.jbar, L.ev};
     for (R* b : bufs)
       if (b) hipFree(b);
   }
@@ -487,28 +488,36 @@ void MgT<R>::set_coefficients(const double* rho_fine, const double* pfix_fine) {
   }
 }
 
-// lambda_max of P_D^{-1} S on every level by power iteration (norms read on the host)
+// lambda_max of P_D^{-1} S on every level by power iteration (norms read on the host). The
+// iteration vector persists per level, so a re-estimate after a change of the stage
+// coefficients starts from the previous eigenvector and needs only a few steps.
 template <typename R>
 int MgT<R>::estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, double* scal_d, double* scal_h) {
   for (auto& L : levels) {
-    k_hash_fill<R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, 12345, L.x);
+    int its = iters;
+    if (!L.ev_valid) {
+      k_hash_fill<R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, 12345, L.ev);
+    } else {
+      its = warm_eig_iters < iters ? warm_eig_iters : iters;
+    }
     double lam = 1.0;
-    for (int it = 0; it < iters; ++it) {
-      level_schur(L, fine, L.x, L.t);
+    for (int it = 0; it < its; ++it) {
+      level_schur(L, fine, L.ev, L.t);
       level_pinv(L, L.t, L.z);
       // |z| / |x| (norms in double through the shared reduction kernels)
       k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.z, work_d);
       vec_norm2(L.stream, L.N, work_d, work_d + L.N, scal_d);
-      k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.x, work_d);
+      k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.ev, work_d);
       vec_norm2(L.stream, L.N, work_d, work_d + L.N, scal_d + 1);
       if (hipMemcpyAsync(scal_h, scal_d, 2 * sizeof(double), hipMemcpyDeviceToHost, L.stream) != hipSuccess) return 3;
       if (hipStreamSynchronize(L.stream) != hipSuccess) return 3;
       lam = scal_h[1] > 0 ? scal_h[0] / scal_h[1] : 1.0;
       // x = z / |z|
       k_vec_scale<R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, (R)(1.0 / (scal_h[0] > 0 ? scal_h[0] : 1.0)),
-                                                                      L.z, L.x);
+                                                                      L.z, L.ev);
     }
     L.lam_max = lam > 1.0 ? lam : 1.0;
+    L.ev_valid = true;
   }
   return 0;
 }
@@ -520,6 +529,14 @@ void MgT<R>::apply(const OpCtx& fine, const double* v, double* z) {
   k_convert<double, R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, v, L.b);
   vcycle(*this, fine, 0);
   k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.x, z);
+}
+
+template <typename R>
+void MgT<R>::schur_velocity(const OpCtx& fine, const R* p, R* w) {
+  const MgLevelT<R>& L = levels[0];
+  const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
+  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, (R)(coeff * fine.dt / (fine.Ma * fine.Ma)),
+                      (R)L.cell_tol, L.cell_maxit, L.N, p, w);
 }
 
 static void interp_matrix(const Tab& to, int kto, const Tab& from, int kfrom, double (&E)[5][5]) {

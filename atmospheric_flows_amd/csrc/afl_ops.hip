@@ -543,7 +543,7 @@ __device__ inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<D
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
 template <int DIM, int K, bool WITH_D, typename R = double>
-__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_apply_C(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+__global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
                                                      const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
                                                      R* __restrict__ dst) {
   __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 // The block preconditioner is cell-local (LDS transposes of the cell's rows); pD, d and x are
 // read only inside the thread's own cell, so d and x (and pD == d or x) update in place.
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_apply_C_cheb(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+__global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
                                                           const R* __restrict__ w, const R* pD, R scaleD,
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
@@ -696,7 +696,7 @@ __device__ inline void inv_spd(R (&A)[N][N]) {
   }
 }
 
-template <int DIM, int K, int NC, typename R = double>
+template <int DIM, int K, int NC, typename R = double, bool SEP = true>
 __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
                                  const R (&co)[2 * K + 1], R* sl /* SolvLds::size(NC) per cell */,
                                  R tol, int maxit, const R (&b)[NC][K + 1], R (&x)[NC][K + 1]) {
@@ -748,20 +748,21 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   for (int q = 0; q < NQ; ++q) total += smg[q];
   const R itot = R(1) / total;
   {
-    // bounds of co / c~ on the thread's own line, c~ = prod_d marginal_d / total^(dim-1)
+    // bounds of co / c~ on the thread's own line: c~ = prod_d marginal_d / total^(dim-1) (SEP),
+    // or c~ = total * (reference weight) for the cheaper P = total (M1 x M1 [x M1])
     const int qx = s.l % NQ, qy = DIM == 3 ? s.l / NQ : 0;
-    const R hor = DIM == 2 ? smg[qx] : smg[qx] * smg[NQ + qy] * itot;
+    const R hor = SEP ? (DIM == 2 ? smg[qx] : smg[qx] * smg[NQ + qy] * itot) : total * lg.wxy;
     R lmin = R(1e30), lmax = R(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const R r = co[q] / (hor * smg[(DIM - 1) * NQ + q] * itot);
+      const R r = co[q] / (SEP ? hor * smg[(DIM - 1) * NQ + q] * itot : hor * T.W[q]);
       lmin = fmin(lmin, r);
       lmax = fmax(lmax, r);
     }
     if (s.act) { srm[s.l] = lmin; srm[LT + s.l] = lmax; }
   }
   // 1D weighted mass matrices and their inverses: thread l < DIM of each cell builds axis l
-  if (s.act && s.l < DIM) {
+  if (SEP && s.act && s.l < DIM) {
     R Am[N1][N1];
     const R sc = s.l == 0 ? R(1) : itot;
     const R* wq = smg + s.l * NQ;
@@ -805,9 +806,11 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   }
   __syncthreads();
   const int nit = nit_blk;
-  const R* MX = Mi;
-  const R* MY = Mi + N1 * N1;
-  const R* MZ = Mi + (DIM - 1) * N1 * N1;
+  const R* MX = SEP ? Mi : &T.MI[0][0];
+  const R* MY = SEP ? Mi + N1 * N1 : &T.MI[0][0];
+  const R* MZ = SEP ? Mi + (DIM - 1) * N1 * N1 : &T.MI[0][0];
+  const R zsc = SEP ? R(1) : itot;
+  constexpr int MS = SEP ? N1 : 5;  // row stride: per-cell N1 x N1 matrices or the 5x5 table
 
   // z = P^{-1} r for all components (rows in registers)
   auto precond = [&](const R (&r)[NC][N1], R (&z)[NC][N1]) {
@@ -816,7 +819,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
       for (int i = 0; i < N1; ++i) {
         R m[N1];
 #pragma unroll
-        for (int j = 0; j < N1; ++j) m[j] = MX[i * N1 + j];
+        for (int j = 0; j < N1; ++j) m[j] = MX[i * MS + j];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           R a = R(0);
@@ -838,7 +841,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
         for (int i = 0; i < N1; ++i) {
           R m[N1];
 #pragma unroll
-          for (int j = 0; j < N1; ++j) m[j] = MZ[i * N1 + j];
+          for (int j = 0; j < N1; ++j) m[j] = MZ[i * MS + j];
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             R a = R(0);
@@ -853,7 +856,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int i = 0; i < N1; ++i) z[c][i] = t1[c * NPC + s.l * N1 + i];
+          for (int i = 0; i < N1; ++i) z[c][i] = zsc * t1[c * NPC + s.l * N1 + i];
     } else {
       if (s.act && s.l < N1 * N1) {
         const int ix = s.l % N1, iz = s.l / N1;
@@ -866,7 +869,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
           for (int i = 0; i < N1; ++i) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += MY[i * N1 + j] * v[j];
+            for (int j = 0; j < N1; ++j) a += MY[i * MS + j] * v[j];
             t1[c * NPC + (iz * N1 + i) * N1 + ix] = a;
           }
         }
@@ -883,7 +886,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
           for (int i = 0; i < N1; ++i) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += MZ[i * N1 + j] * v[j];
+            for (int j = 0; j < N1; ++j) a += MZ[i * MS + j] * v[j];
             dl[c * NPC + (i * N1 + iy) * N1 + ix] = a;
           }
         }
@@ -893,7 +896,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int i = 0; i < N1; ++i) z[c][i] = dl[c * NPC + s.l * N1 + i];
+          for (int i = 0; i < N1; ++i) z[c][i] = zsc * dl[c * NPC + s.l * N1 + i];
     }
     __syncthreads();
   };
@@ -948,7 +951,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 }
 
 template <int DIM, int K, int NC>
-__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_mass_solve(Geo g, Tab T, const double* __restrict__ rho, double scale,
+__global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const double* __restrict__ rho, double scale,
                                                         int64_t N, const double* __restrict__ b,
                                                         double* __restrict__ x, double tol, int maxit) {
   constexpr int PC = Lds<DIM, K>::W > SolvLds<DIM, K>::size(NC) ? Lds<DIM, K>::W : SolvLds<DIM, K>::size(NC);
@@ -978,7 +981,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 
 // Schur first half: w = A^{-1} B p (fused B apply + cell-local A solve).
 template <int DIM, int K, typename R = double>
-__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 3 : 1))) void k_schur_w(Geo g, TabT<R> T, const R* __restrict__ rho, R f, int64_t N,
+__global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, TabT<R> T, const R* __restrict__ rho, R f, int64_t N,
                                                      const R* __restrict__ p, R* __restrict__ w, R tol, int maxit) {
   constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;  // B phase: work + face nodes + face fluxes
   constexpr int PS = SolvLds<DIM, K>::size(DIM);             // solver phase
@@ -1006,7 +1009,9 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 #pragma unroll
   for (int q = 0; q < NQ; ++q) co[q] *= lg.jxw(g, T, q);
   R xx[DIM][N1];
-  cell_cheb<DIM, K, DIM, R>(T, s, lg, co, base, tol, maxit, bp, xx);
+  // FP64 (exact solves): separable per-cell preconditioner; FP32 (multigrid level operator, one
+  // Chebyshev step at preconditioner accuracy): the cheaper reference-mass preconditioner
+  cell_cheb<DIM, K, DIM, R, sizeof(R) == 8>(T, s, lg, co, base, tol, maxit, bp, xx);
 #pragma unroll
   for (int c = 0; c < DIM; ++c) store_row<DIM, K, R>(w + c * N, s, xx[c]);
 }
@@ -1172,6 +1177,19 @@ void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStre
     k_apply_C<D_, K_, true, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
   });
 }
+// w = A^{-1} B p alone in precision R (the first half of the level operator)
+template <typename R>
+void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, R f, R tol,
+                      int maxit, int64_t N, const R* p, R* w) {
+  AFL_DISPATCH(dim, k, {
+    k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+  });
+}
+template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*, float, float,
+                                      int, int64_t, const float*, float*);
+template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*, double,
+                                       double, int, int64_t, const double*, double*);
+
 // w = A^{-1} B p followed by the fused C/D apply and Chebyshev step (see k_apply_C_cheb)
 template <typename R>
 void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,

@@ -50,7 +50,8 @@ def load_library():
     L.afl_set_sponge.argtypes = [vp, C.POINTER(Sponge)]
     for n in ("afl_vmult_rhs_rho_update", "afl_vmult_rhs_velocity_fixed", "afl_vmult_rhs_pressure"):
         getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.c_int]
-    for n in ("afl_vmult", "afl_vmult_pressure", "afl_vmult_enthalpy", "afl_solve_mass", "afl_vmult_schur_velocity"):
+    for n in ("afl_vmult", "afl_vmult_pressure", "afl_vmult_enthalpy", "afl_solve_mass", "afl_vmult_schur_velocity",
+              "afl_mg_vmult", "afl_vmult_schur_velocity_f32"):
         getattr(L, n).argtypes = [vp, dp, dp]
     L.afl_compute_diagonal.argtypes = [vp]
     L.afl_get_matrix_diagonal_inverse.argtypes = [vp, dp]
@@ -64,11 +65,11 @@ def load_library():
     return L
 
 
-def _ptr(t):
+def _ptr(t, size=8):
     if t is None:
         return None
-    if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8 and t.is_contiguous()):
-        raise AflError("expected a contiguous float64 device tensor")
+    if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == size and t.is_contiguous()):
+        raise AflError(f"expected a contiguous float{8 * size} device tensor")
     return C.c_void_p(t.data_ptr())
 
 
@@ -146,6 +147,16 @@ class EulerOperator:
     def vmult_enthalpy(self, dst, src): self._chk(self.L.afl_vmult_enthalpy(self.h, _ptr(dst), _ptr(src))); return dst
     def vmult_schur_velocity(self, w, p):
         self._chk(self.L.afl_vmult_schur_velocity(self.h, _ptr(w), _ptr(p)))
+        return w
+
+    def mg_vmult(self, dst, src):
+        """Schur preconditioner apply (p-multigrid V-cycle of the selected schur_precond)."""
+        self._chk(self.L.afl_mg_vmult(self.h, _ptr(dst), _ptr(src)))
+        return dst
+
+    def vmult_schur_velocity_f32(self, w, p):
+        """FP32 fine-level w = A^-1 B p of the FP32 V-cycle (float32 tensors)."""
+        self._chk(self.L.afl_vmult_schur_velocity_f32(self.h, _ptr(w, 4), _ptr(p, 4)))
         return w
 
     def compute_diagonal(self): self._chk(self.L.afl_compute_diagonal(self.h))

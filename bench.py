@@ -82,6 +82,20 @@ def run_cpu_subprocess(args, threads):
     return out
 
 
+def profiled_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload (calibrated against an L2-cold copy of known size); None if not profiled."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{args.nx}x{args.nz}_k{args.degree}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel + ":"):
+            for tok in line.split():
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=")[1])
+    return None
+
+
 def main():
     args = parse()
     if args.cpu_only:
@@ -113,6 +127,14 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
 
+    # calibration for the rocprofv3 FETCH_SIZE / WRITE_SIZE passes: an L2-cold scale-by-one of
+    # 4N doubles (torch elementwise kernel; reads and writes exactly 32 N bytes)
+    cal_src = torch.rand(4 * N, device="cuda", dtype=torch.float64)
+    flush = torch.empty(16 * 1024 * 1024, device="cuda", dtype=torch.float64).fill_(1.0)
+    cal_dst = torch.mul(cal_src, 1.0)
+    torch.cuda.synchronize()
+    del cal_src, cal_dst, flush
+
     # isolated applies on the initial state (stage-3 explicit applies, Schur pieces)
     r2, u2, p2 = rho.clone(), u.clone(), p.clone()
     dst_s, dst_v = op.new_scalar(), op.new_vector()
@@ -129,11 +151,19 @@ def main():
     op.set_NS_stage(2)
     t_schur = timed(lambda: op.vmult(dst_s, p), args.reps)
     t_sw = timed(lambda: op.vmult_schur_velocity(dst_v, p), args.reps)
+    # the FP32 level operator of the preconditioner: w = A^-1 B p in single precision (the kernel
+    # that dominates the FP32-preconditioned step) and one full V-cycle
+    from atmospheric_flows_amd import make_solver_params
+    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32
+    op.set_solver_params(make_solver_params(gmres_eps=1e-10, gmres_restart=28, fp_tol=1e-10,
+                                            schur_precond=AFL_SCHUR_MG_FP32))
+    p32 = p.float()
+    w32 = torch.empty(dim * N, device="cuda", dtype=torch.float32)
+    t_sw32 = timed(lambda: op.vmult_schur_velocity_f32(w32, p32), args.reps)
+    t_mg = timed(lambda: op.mg_vmult(dst_s, p), max(1, args.reps // 10))
 
     # IMEX steps: FP64 plain path (Jacobi GMRES) and FP32 p-multigrid FGMRES, same initial
     # state, same true-residual tolerance 1e-10
-    from atmospheric_flows_amd import make_solver_params
-    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32
 
     def run_path(precond):
         o = EulerOperator(mesh, mach=1.0, froude=FR, dt=1.0)
@@ -156,10 +186,10 @@ def main():
            for name, a, b in (("rho", s32[0], s64[0]), ("u", s32[1], s64[1]), ("p", s32[2], s64[2]))}
     dofs = (2 + dim) * N
 
-    # roofline of the dominant kernel of the step (w = A^{-1} B p, inside every Schur apply):
-    # algorithmic bytes = (p, rho_fix read; dim velocity components written) * N * 8
-    bytes_sw = (2 + dim) * N * 8
-    ach = bytes_sw / t_sw / 1e9
+    # roofline of the dominant kernel of the FP32-preconditioned step, the FP32 w = A^{-1} B p:
+    # algorithmic bytes = (p, rho_fix read; dim velocity components written) * N * 4
+    bytes_sw32 = (2 + dim) * N * 4
+    ach = bytes_sw32 / t_sw32 / 1e9
     field_bytes = N * 8
     res = {
         "metric": "IMEX step wall-time, FP32-MG vs FP64 path at equal residual; DoF/s, % HBM peak",
@@ -185,13 +215,16 @@ def main():
         "speedup_fp32mg_vs_fp64": step_fp64 / step_s,
         "rel_l2_fp32mg_vs_fp64_after_steps": dev,
         "isolated_applies_us": {"E1_stage3": t_e1 * 1e6, "E2_stage3": t_e2 * 1e6, "E3_stage3": t_e3 * 1e6,
-                                "schur_vmult": t_schur * 1e6, "schur_velocity_kernel": t_sw * 1e6},
+                                "schur_vmult": t_schur * 1e6, "schur_velocity_kernel": t_sw * 1e6,
+                                "schur_velocity_kernel_fp32": t_sw32 * 1e6, "fp32_vcycle": t_mg * 1e6},
         "isolated_applies_GBs": {"E1_stage3": 10 * field_bytes / t_e1 / 1e9,
                                  "E2_stage3": 14 * field_bytes / t_e2 / 1e9,
                                  "E3_stage3": 16 * field_bytes / t_e3 / 1e9,
-                                 "schur_vmult": 9 * field_bytes / t_schur / 1e9},
-        "roofline": {"kernel": "k_schur_w (w = A^-1 B p)", "bound": "hbm", "achieved": ach,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None},
+                                 "schur_vmult": 9 * field_bytes / t_schur / 1e9,
+                                 "schur_velocity_kernel": (2 + dim) * field_bytes / t_sw / 1e9},
+        "roofline": {"kernel": "k_schur_w<float> (FP32 w = A^-1 B p)", "bound": "hbm", "achieved": ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("k_schur_w<2, 4, float>", args)},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

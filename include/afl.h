@@ -84,6 +84,15 @@ int afl_vmult_enthalpy(afl_operator* op, double* dst, const double* src); /* C, 
 /* First half of the Schur apply, w = A^{-1} B p (euler_operator.h:2380-2397: vmult_pressure
  * followed by the inner CG on A), one fused kernel with an exact cell-local A solve. */
 int afl_vmult_schur_velocity(afl_operator* op, double* w, const double* p);
+/* Preconditioner apply z = P^{-1} v of the Schur solve: the p-multigrid V-cycle with Chebyshev
+ * smoothers (the reference wires deal.II PreconditionMG + PreconditionChebyshev into its
+ * mass-type solves, Atmospheric_Flow.cc:455-527; here the V-cycle smooths the Schur operator of
+ * the current stage coefficients). Requires solver_params.schur_precond = AFL_SCHUR_MG_FP32 or
+ * AFL_SCHUR_MG_FP64; FP64 vectors in and out, level arithmetic in the selected precision. */
+int afl_mg_vmult(afl_operator* op, double* dst, const double* src);
+/* The FP32 fine-level half of that V-cycle's operator, w = A^{-1} B p in single precision on the
+ * FP32 copies of rho_for_fixed (requires schur_precond = AFL_SCHUR_MG_FP32). */
+int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p);
 /* compute_diagonal / get_matrix_diagonal_inverse (euler_operator.h:2589-2626) */
 int afl_compute_diagonal(afl_operator* op);
 int afl_get_matrix_diagonal_inverse(afl_operator* op, double* dst);

@@ -79,3 +79,59 @@ def test_mg_fp32_imex_steps_match_oracle(oracle_mod, dim, k, nc):
     assert rel(dr.cpu().numpy(), rho) < 1e-10
     assert rel(du.cpu().numpy(), u) < 1e-8
     assert rel(dp.cpu().numpy(), p) < 1e-10
+
+
+def _stage_operator(dim, k, nc, precond, **kw):
+    import torch
+    xr = (-20e3, 20e3)
+    mesh = make_mesh(dim=dim, degree=k, n_cells=nc, x_range=xr, y_range=xr, z_top=25e3)
+    gpu = EulerOperator(mesh, mach=1.0, froude=FR, dt=2.0)
+    gpu.set_solver_params(make_solver_params(schur_precond=precond, **kw))
+    rho, u, p = gpu.init_hill_state(make_hill_state(theta_sigma=0.5, seed=5))
+    gpu.set_HYPERBOLIC_stage(1)
+    gpu.set_rho_for_fixed(rho)
+    gpu.set_pres_fixed(p)
+    gpu.set_NS_stage(2)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    v = torch.randn(p.shape, generator=g, device="cuda", dtype=torch.float64)
+    return gpu, p, v
+
+
+@pytest.mark.parametrize("dim,k,nc", [(2, 4, (24, 1, 10)), (3, 2, (4, 4, 3))])
+def test_mg_vmult_is_a_linear_contraction(dim, k, nc):
+    """The FP64 V-cycle with tight cell solves is a fixed linear operator approximating S^-1:
+    linear to rounding, and I - S P^-1 contracts; the FP32 V-cycle agrees with it to FP32 accuracy."""
+    say(f"[gpu mg] mg_vmult dim={dim} k={k} cells={nc}")
+    import torch
+    gpu, p, v = _stage_operator(dim, k, nc, AFL_SCHUR_MG_FP64, mg_cell_tol=1e-14)
+    w = torch.roll(v, 17) * 0.3
+    z1, z2, z12 = gpu.new_scalar(), gpu.new_scalar(), gpu.new_scalar()
+    gpu.mg_vmult(z1, v)
+    gpu.mg_vmult(z2, w)
+    gpu.mg_vmult(z12, 2.0 * v + w)
+    lin = float(torch.linalg.norm(z12 - (2.0 * z1 + z2)) / torch.linalg.norm(z12))
+    r = v - gpu.vmult(gpu.new_scalar(), z1)
+    contraction = float(torch.linalg.norm(r) / torch.linalg.norm(v))
+    say(f"  linearity {lin:.2e}, ||v - S P^-1 v|| / ||v|| = {contraction:.3f}")
+    assert lin < 1e-12
+    assert contraction < 0.5
+    g32, _, _ = _stage_operator(dim, k, nc, AFL_SCHUR_MG_FP32, mg_cell_tol=1e-6)
+    z32 = g32.mg_vmult(g32.new_scalar(), v)
+    d32 = float(torch.linalg.norm(z32 - z1) / torch.linalg.norm(z1))
+    say(f"  fp32 vs fp64 V-cycle {d32:.2e}")
+    assert d32 < 1e-4
+
+
+@pytest.mark.parametrize("dim,k,nc", [(2, 4, (24, 1, 10)), (2, 3, (16, 1, 8)), (3, 2, (4, 4, 3))])
+def test_schur_velocity_f32_matches_fp64(dim, k, nc):
+    say(f"[gpu mg] schur_velocity_f32 dim={dim} k={k} cells={nc}")
+    import torch
+    # the V-cycle sees residual-sized vectors, not the hydrostatic pressure itself (whose FP32
+    # gradient loses digits to cancellation), so the comparison uses a random vector
+    gpu, p, v = _stage_operator(dim, k, nc, AFL_SCHUR_MG_FP32, mg_cell_tol=1e-6)
+    w64 = gpu.vmult_schur_velocity(gpu.new_vector(), v)
+    w32 = torch.empty(w64.shape, device="cuda", dtype=torch.float32)
+    gpu.vmult_schur_velocity_f32(w32, v.float())
+    err = float(torch.linalg.norm(w32.double() - w64) / torch.linalg.norm(w64))
+    say(f"  rel L2 fp32 vs fp64 {err:.2e}")
+    assert err < 1e-5
