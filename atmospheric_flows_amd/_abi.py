@@ -36,6 +36,17 @@ class StepInfo(C.Structure):
     ]
 
 
+class Diagnostics(C.Structure):
+    _fields_ = [
+        ("min_density", C.c_double), ("max_density", C.c_double), ("max_celerity", C.c_double),
+        ("max_velocity", C.c_double), ("min_cell_diameter", C.c_double), ("cfl_c", C.c_double),
+        ("cfl_u", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class HillState(C.Structure):
     _fields_ = [
         ("theta0", C.c_double), ("brunt", C.c_double), ("p0", C.c_double), ("u0", C.c_double),

@@ -2,6 +2,7 @@
 // Host side of the drop-in: EULEROperator state, NS_stage dispatch, the device-resident
 // Schur GMRES and the IMEX-ARK2 step driver. All vectors live on the device; the host only
 // sees reduction scalars (one small copy per Krylov iteration / fixed-point iteration).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -180,7 +181,8 @@ struct afl_operator {
   bool mg_eig_stale = true;  // rho_for_fixed changed since the last eigenvalue estimate
   bool mg_coef_stale = true; // rho_for_fixed / pres_fixed changed since the levels copied them
   Tab tabs_by_k[5]{};        // 1D tables per degree (same node family)
-  DevBuf gZ, mgw;
+  DevBuf gZ, mgw, diagw;
+  double min_diam = 0.0;  // GridTools::minimal_cell_diameter of the mapped mesh
   afl_sponge sponge{15000.0, 0.0, {0.0, 0.0, 0.0}, 0, 0};
   DevBuf rho_fix, p_fix, u_fix, inv_diag, dinv_D, red, dsc, wS, gV, gW, ydev;
   int diag_comps = 0;
@@ -189,7 +191,7 @@ struct afl_operator {
   DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
   double* host_sc = nullptr;  // pinned scalars
   ~afl_operator() {
-    DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw,
+    DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
                      &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix};
     for (DevBuf* b : all) b->release();
@@ -200,6 +202,37 @@ struct afl_operator {
 };
 
 namespace {
+
+// minimal_cell_diameter on the terrain-following map: min over cells of the longest diagonal
+// between opposite vertices (deal.II's CellAccessor::diameter for quadrilaterals / hexahedra)
+double min_cell_diameter(const afl_mesh_desc* md) {
+  const int d3 = md->dim == 3;
+  const int nx = md->n_cells[0], ny = d3 ? md->n_cells[1] : 1, nz = md->n_cells[2];
+  const double hx = (md->x_max - md->x_min) / nx, hy = d3 ? (md->y_max - md->y_min) / ny : 0.0,
+               hz = md->z_top / nz;
+  const double ia2 = 1.0 / (md->hill_width * md->hill_width);
+  auto zmap = [&](double x, double y, double zeta) {
+    const double h = md->hill_height / (1.0 + (x * x + y * y) * ia2);
+    return zeta + h * (1.0 - zeta / md->z_top);
+  };
+  double best = 1e300;
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const double x0 = md->x_min + ix * hx, y0 = d3 ? md->y_min + iy * hy : 0.0, z0 = iz * hz;
+        double diam = 0.0;
+        // opposite vertex pairs: (a, b, 0) <-> (1-a, 1-b, 1)
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < (d3 ? 2 : 1); ++b) {
+            const double xa = x0 + a * hx, ya = y0 + b * hy, xb = x0 + (1 - a) * hx, yb = y0 + (d3 ? (1 - b) * hy : 0.0);
+            const double za = zmap(xa, ya, z0), zb = zmap(xb, yb, z0 + hz);
+            const double dx = xb - xa, dy = yb - ya, dz = zb - za;
+            diam = std::max(diam, std::sqrt(dx * dx + dy * dy + dz * dz));
+          }
+        best = std::min(best, diam);
+      }
+  return best;
+}
 
 constexpr int kMaxRestart = 60;
 // device scalar slots: [0,64) first CGS pass, [64,128) second pass, 128 norm, 129.. misc
@@ -531,6 +564,7 @@ int afl_create(const afl_mesh_desc* md, double mach, double froude, double dt, a
     delete op;
     return rc;
   }
+  op->min_diam = min_cell_diameter(md);
   vec_fill(o.stream, o.N, 1.0, op->rho_fix.p);
   vec_fill(o.stream, o.N, 0.0, op->p_fix.p);
   vec_fill(o.stream, o.N * o.dim, 0.0, op->u_fix.p);
@@ -723,6 +757,39 @@ int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {
   if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
   op->mg32.schur_velocity(op->o, p, w);
   return check_device("vmult_schur_velocity_f32");
+}
+
+int afl_compute_diagnostics(afl_operator* op, const double* rho, const double* u, const double* p,
+                            afl_diagnostics* out) {
+  if (!op || !rho || !out) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  int rc;
+  if ((rc = ensure_common(op)) || (rc = op->diagw.alloc(diag_work_size(op->o)))) return rc;
+  launch_nodal_stats(op->o, rho, u, p, op->diagw.p, op->dsc.p + kScA + 4);
+  if ((rc = fetch(op, kScA + 4, 4))) return rc;
+  const double* h = op->host_sc + kScA + 4;
+  const OpCtx& o = op->o;
+  out->min_density = -h[0];
+  out->max_density = h[1];
+  out->max_celerity = p ? h[2] : 0.0;
+  out->max_velocity = u ? h[3] : 0.0;
+  out->min_cell_diameter = op->min_diam;
+  const double f = o.dt * o.k * std::sqrt((double)o.dim) / op->min_diam;
+  out->cfl_c = f * out->max_celerity / o.Ma;
+  out->cfl_u = f * out->max_velocity;
+  return check_device("compute_diagnostics");
+}
+
+int afl_l2_difference(afl_operator* op, int ncomp, const double* a, const double* b, double* err_l2,
+                      double* ref_l2) {
+  if (!op || !a || !err_l2 || !ref_l2) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (ncomp != 1 && ncomp != op->o.dim) return fail(AFL_ERR_INVALID_ARGUMENT, "ncomp must be 1 or dim");
+  int rc;
+  if ((rc = ensure_common(op)) || (rc = op->diagw.alloc(diag_work_size(op->o)))) return rc;
+  launch_l2_difference(op->o, ncomp, a, b, op->diagw.p, op->dsc.p + kScA + 4);
+  if ((rc = fetch(op, kScA + 4, 2))) return rc;
+  *err_l2 = std::sqrt(std::max(0.0, op->host_sc[kScA + 4]));
+  *ref_l2 = std::sqrt(std::max(0.0, op->host_sc[kScA + 5]));
+  return check_device("l2_difference");
 }
 
 int afl_compute_diagonal(afl_operator* op) {

@@ -47,6 +47,13 @@ void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStre
                        R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar, const R* rin, R* rout,
                        R c1, R c2, R* d, R* x);
 
+// diagnostics (afl_diag.hip): out4 = {-min rho, max |rho|, max celerity, max speed};
+// out2 = {sum JxW (a-b)^2, sum JxW b^2}; work holds diag_work_size(o) doubles
+int64_t diag_work_size(const OpCtx& o);
+void launch_nodal_stats(const OpCtx& o, const double* rho, const double* u, const double* p, double* work,
+                        double* out4);
+void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const double* b, double* work, double* out2);
+
 // vector kernels (afl_blas.hip); reductions leave their result in device memory
 void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);
 void vec_axpby(hipStream_t st, int64_t n, double a, const double* x, double b, double* y);  // y = a x + b y

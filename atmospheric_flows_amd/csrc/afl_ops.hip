@@ -672,7 +672,7 @@ struct SolvLds {
   static constexpr int WF = EVF > INF ? EVF : INF;  // per component work area
   static constexpr int PER = WF + C::NPC;            // + the component's dofs
   // set-up scratch: co per line, line sums (3D), marginals, per-line ratio bounds
-  static constexpr int SETUP = C::LT * C::NQ + (DIM == 3 ? C::LT : 0) + DIM * C::NQ + 2 * C::LT;
+  static constexpr int SETUP = C::LT * C::NQ + (DIM == 3 ? C::LT : 0) + (DIM + 1) * C::NQ + 2 * C::LT;
   static constexpr int SEP = DIM * C::N1 * C::N1;    // the dim inverse 1D matrices (persist)
   static constexpr int sep_off(int nc) { return nc * PER > SETUP ? nc * PER : SETUP; }
   static constexpr int size(int nc) { return sep_off(nc) + SEP; }
@@ -716,7 +716,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   R* sco = sl;                                   // [LT][NQ]
   R* sL = sco + LT * NQ;                         // [LT] line sums (3D)
   R* smg = sL + (DIM == 3 ? LT : 0);             // [DIM][NQ] marginals a, (b,) c
-  R* srm = smg + DIM * NQ;                       // [2][LT] per-line min / max of co / c~
+  R* srm = smg + (DIM + 1) * NQ;                 // [2][LT] per-line min / max of co / c~
   if (threadIdx.x == 0) nit_blk = 1;
   R lsum = R(0);
 #pragma unroll
@@ -728,10 +728,11 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   }
   __syncthreads();
   if (s.act && s.l < NQ) {
-    // vertical marginal: sum over the lines of co at qz = l
+    // vertical marginal: sum over the lines of co at qz = l (and its reciprocal)
     R c = R(0);
     for (int t = 0; t < LT; ++t) c += sco[t * NQ + s.l];
     smg[(DIM - 1) * NQ + s.l] = c;
+    smg[DIM * NQ + s.l] = R(1) / c;
     if (DIM == 2) {
       smg[s.l] = lsum;  // x marginal: the thread's own line sum
     } else {
@@ -751,11 +752,13 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
     // bounds of co / c~ on the thread's own line: c~ = prod_d marginal_d / total^(dim-1) (SEP),
     // or c~ = total * (reference weight) for the cheaper P = total (M1 x M1 [x M1])
     const int qx = s.l % NQ, qy = DIM == 3 ? s.l / NQ : 0;
-    const R hor = SEP ? (DIM == 2 ? smg[qx] : smg[qx] * smg[NQ + qy] * itot) : total * lg.wxy;
+    const R ihor = SEP ? total / (DIM == 2 ? smg[qx] : smg[qx] * smg[NQ + qy] * itot) : itot / lg.wxy;
     R lmin = R(1e30), lmax = R(0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const R r = co[q] / (SEP ? hor * smg[(DIM - 1) * NQ + q] * itot : hor * T.W[q]);
+      R r;
+      if constexpr (SEP) r = co[q] * ihor * smg[DIM * NQ + q];
+      else r = co[q] * ihor / T.W[q];
       lmin = fmin(lmin, r);
       lmax = fmax(lmax, r);
     }
@@ -797,20 +800,25 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
   const R delta = fmax(R(0.5) * (beta - alpha), R(1e-14) * theta);
   const R sigma1 = theta / delta;
   {
-    const double sk = sqrt((double)beta / (double)alpha);
-    const double sg = (sk - 1.0) / (sk + 1.0);
+    // sigma = (sqrt(kappa) - 1) / (sqrt(kappa) + 1) = (kappa - 1) / (sqrt(kappa) + 1)^2, with
+    // kappa - 1 formed in R so that kappa close to one keeps its digits
+    const float km1 = (float)((beta - alpha) / alpha);
+    const float sk = sqrtf(1.0f + km1);
+    const float sg = km1 / ((sk + 1.0f) * (sk + 1.0f));
     int n = 1;
-    if (sg > 1e-300) n = (int)ceil(log(0.5 * (double)tol) / log(sg));
+    if (sg > 1e-30f) n = (int)ceilf(logf(0.5f * (float)tol) / logf(sg));
     n = n < 1 ? 1 : (n > maxit ? maxit : n);
     if (s.act && s.l == 0) atomicMax(&nit_blk, n);
   }
   __syncthreads();
   const int nit = nit_blk;
-  const R* MX = SEP ? Mi : &T.MI[0][0];
-  const R* MY = SEP ? Mi + N1 * N1 : &T.MI[0][0];
-  const R* MZ = SEP ? Mi + (DIM - 1) * N1 * N1 : &T.MI[0][0];
   const R zsc = SEP ? R(1) : itot;
-  constexpr int MS = SEP ? N1 : 5;  // row stride: per-cell N1 x N1 matrices or the 5x5 table
+  // entry (i, j) of the 1D inverse along axis ax: per-cell matrices in LDS (SEP) or the
+  // reference table (uniform, no address taken of the kernel argument)
+  auto minv = [&](int ax, int i, int j) -> R {
+    if constexpr (SEP) return Mi[(ax * N1 + i) * N1 + j];
+    else return T.MI[i][j];
+  };
 
   // z = P^{-1} r for all components (rows in registers)
   auto precond = [&](const R (&r)[NC][N1], R (&z)[NC][N1]) {
@@ -819,7 +827,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
       for (int i = 0; i < N1; ++i) {
         R m[N1];
 #pragma unroll
-        for (int j = 0; j < N1; ++j) m[j] = MX[i * MS + j];
+        for (int j = 0; j < N1; ++j) m[j] = minv(0, i, j);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           R a = R(0);
@@ -841,7 +849,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
         for (int i = 0; i < N1; ++i) {
           R m[N1];
 #pragma unroll
-          for (int j = 0; j < N1; ++j) m[j] = MZ[i * MS + j];
+          for (int j = 0; j < N1; ++j) m[j] = minv(DIM - 1, i, j);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             R a = R(0);
@@ -869,7 +877,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
           for (int i = 0; i < N1; ++i) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += MY[i * MS + j] * v[j];
+            for (int j = 0; j < N1; ++j) a += minv(1, i, j) * v[j];
             t1[c * NPC + (iz * N1 + i) * N1 + ix] = a;
           }
         }
@@ -886,7 +894,7 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
           for (int i = 0; i < N1; ++i) {
             R a = R(0);
 #pragma unroll
-            for (int j = 0; j < N1; ++j) a += MZ[i * MS + j] * v[j];
+            for (int j = 0; j < N1; ++j) a += minv(DIM - 1, i, j) * v[j];
             dl[c * NPC + (i * N1 + iy) * N1 + ix] = a;
           }
         }

@@ -14,7 +14,7 @@ Vectors are torch.float64 CUDA tensors in the layout documented in include/afl.h
 import ctypes as C
 import os
 
-from ._abi import MeshDesc, SolverParams, StepInfo, HillState, Sponge
+from ._abi import MeshDesc, SolverParams, StepInfo, HillState, Sponge, Diagnostics
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafl_hip.so")
@@ -59,6 +59,8 @@ def load_library():
     L.afl_imex_step.argtypes = [vp, dp, dp, dp, C.POINTER(StepInfo)]
     L.afl_init_hill_state.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
     L.afl_background.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
+    L.afl_compute_diagnostics.argtypes = [vp, dp, dp, dp, C.POINTER(Diagnostics)]
+    L.afl_l2_difference.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     for n in ("afl_pack_dofs", "afl_unpack_dofs"):
         getattr(L, n).argtypes = [vp, C.c_int, dp, dp, C.c_int64, C.c_int64, C.c_int64, dp]
     _lib = L
@@ -158,6 +160,19 @@ class EulerOperator:
         """FP32 fine-level w = A^-1 B p of the FP32 V-cycle (float32 tensors)."""
         self._chk(self.L.afl_vmult_schur_velocity_f32(self.h, _ptr(w, 4), _ptr(p, 4)))
         return w
+
+    # ---- diagnostics (Atmospheric_Flow.cc:925-1087, 1225-1235) ---------------------------------
+    def diagnostics(self, rho, u=None, p=None):
+        """Density bounds, maximum celerity and speed, minimal cell diameter and CFL numbers."""
+        d = Diagnostics()
+        self._chk(self.L.afl_compute_diagnostics(self.h, _ptr(rho), _ptr(u), _ptr(p), C.byref(d)))
+        return d.as_dict()
+
+    def l2_difference(self, ncomp, a, b=None):
+        """(||a - b||_L2, ||b||_L2) of ncomp-component fields."""
+        e, r = C.c_double(), C.c_double()
+        self._chk(self.L.afl_l2_difference(self.h, ncomp, _ptr(a), _ptr(b), C.byref(e), C.byref(r)))
+        return e.value, r.value
 
     def compute_diagonal(self): self._chk(self.L.afl_compute_diagonal(self.h))
 

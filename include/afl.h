@@ -93,6 +93,21 @@ int afl_mg_vmult(afl_operator* op, double* dst, const double* src);
 /* The FP32 fine-level half of that V-cycle's operator, w = A^{-1} B p in single precision on the
  * FP32 copies of rho_for_fixed (requires schur_precond = AFL_SCHUR_MG_FP32). */
 int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p);
+
+/* ---- diagnostics ------------------------------------------------------------------------ */
+/* Density bounds, maximum celerity and speed of a state and the two Courant numbers
+ * (Atmospheric_Flow.cc:992-1087 get_max_velocity / get_min_density / get_max_density /
+ * compute_max_celerity; CFL_c and CFL_u as printed at :1225-1235). The reference evaluates
+ * these with QGaussLobatto(k+1) on Gauss-Lobatto support points, i.e. at the nodes, which is
+ * what this computes. u and p may be NULL (their entries are then 0). Synchronises the stream. */
+int afl_compute_diagnostics(afl_operator* op, const double* rho, const double* u, const double* p,
+                            afl_diagnostics* out);
+/* L2 norms ||a - b|| and ||b|| of ncomp-component fields (1 or dim), the quantities of
+ * VectorTools::integrate_difference + compute_global_error in analyze_results (:925-985),
+ * with b a discrete reference field (b == NULL: b = 0). Gauss quadrature with 2k+1 points on
+ * the mapped cells. Synchronises the stream. */
+int afl_l2_difference(afl_operator* op, int ncomp, const double* a, const double* b, double* err_l2,
+                      double* ref_l2);
 /* compute_diagonal / get_matrix_diagonal_inverse (euler_operator.h:2589-2626) */
 int afl_compute_diagonal(afl_operator* op);
 int afl_get_matrix_diagonal_inverse(afl_operator* op, double* dst);

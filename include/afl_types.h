@@ -90,6 +90,19 @@ typedef struct afl_sponge {
   int32_t pad_;
 } afl_sponge;
 
+/* Diagnostics of a state (EulerSolver::get_min_density / get_max_density /
+ * compute_max_celerity / get_max_velocity and the Courant numbers printed in run,
+ * Atmospheric_Flow.cc:992-1087, 1225-1235). */
+typedef struct afl_diagnostics {
+  double min_density;        /* min nodal rho */
+  double max_density;        /* max |rho| (linfty_norm) */
+  double max_celerity;       /* max sqrt(gamma p / rho); 0 if p was not given */
+  double max_velocity;       /* max sqrt(u_0^2 + u_1^2) (the reference's formula, also in 3D); 0 if u not given */
+  double min_cell_diameter;  /* minimal_cell_diameter: min over cells of the longest vertex diagonal */
+  double cfl_c;              /* dt max_celerity k sqrt(dim) / (Ma min_cell_diameter) */
+  double cfl_u;              /* dt max_velocity k sqrt(dim) / min_cell_diameter */
+} afl_diagnostics;
+
 #ifdef __cplusplus
 }
 #endif

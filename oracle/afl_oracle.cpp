@@ -1245,6 +1245,72 @@ int orc_init_hill_state(orc_operator* op, const afl_hill_state* st, double* rho,
   return 0;
 }
 
+// Diagnostics, Atmospheric_Flow.cc:992-1087 (get_max_velocity, get_min_density, get_max_density,
+// compute_max_celerity) and the Courant numbers of run (:1225-1235). The reference evaluates on
+// QGaussLobatto(k+1), which on Gauss-Lobatto support points are the nodal values. The minimal
+// cell diameter is the longest vertex diagonal minimised over cells (vertices through Mesh::map).
+// out = {min rho, max |rho|, max celerity, max speed(u0, u1), min diameter, CFL_c, CFL_u}.
+int orc_diagnostics(orc_operator* op, const double* rho, const double* u, const double* p, double* out) {
+  const int64_t N = op->N;
+  double mn = 1e300, mx = 0.0, cel = 0.0, vel = 0.0;
+  for (int64_t i = 0; i < N; ++i) {
+    mn = std::min(mn, rho[i]);
+    mx = std::max(mx, std::fabs(rho[i]));
+    if (p) cel = std::max(cel, std::sqrt(kGamma * p[i] / rho[i]));
+    if (u) vel = std::max(vel, std::sqrt(u[i] * u[i] + u[N + i] * u[N + i]));
+  }
+  const Mesh& m = op->mesh;
+  double dmin = 1e300;
+  for (int c = 0; c < m.ncells; ++c) {
+    int ic[3];
+    m.cell_index(c, ic);
+    double diam = 0.0;
+    const int nv = op->dim == 2 ? 2 : 4;
+    for (int v = 0; v < nv; ++v) {
+      // vertex with reference corner r and its opposite 1 - r
+      double r0[3] = {double(v & 1), double((v >> 1) & 1), 0.0}, r1[3];
+      if (op->dim == 2) r0[1] = 0.0;
+      for (int a = 0; a < 3; ++a) r1[a] = 1.0 - r0[a];
+      if (op->dim == 2) { r1[2] = 0.0; r1[1] = 1.0; }
+      double X0[3], X1[3], J[3][3];
+      m.map(ic, r0, X0, J);
+      m.map(ic, r1, X1, J);
+      double d2 = 0.0;
+      for (int a = 0; a < op->dim; ++a) d2 += (X1[a] - X0[a]) * (X1[a] - X0[a]);
+      diam = std::max(diam, std::sqrt(d2));
+    }
+    dmin = std::min(dmin, diam);
+  }
+  const double f = op->dt * op->B.k * std::sqrt((double)op->dim) / dmin;
+  out[0] = mn; out[1] = mx; out[2] = p ? cel : 0.0; out[3] = u ? vel : 0.0;
+  out[4] = dmin; out[5] = f * out[2] / op->Ma; out[6] = f * out[3];
+  return 0;
+}
+
+// L2 norms ||a - b|| and ||b|| (b may be null) of ncomp-component fields with the operators'
+// Gauss quadrature (the quantities of analyze_results, Atmospheric_Flow.cc:925-985).
+int orc_l2_difference(orc_operator* op, int ncomp, const double* a, const double* b, double* err, double* ref) {
+  const int nqc = op->nqc;
+  double se = 0.0, sn = 0.0;
+  Scratch s;
+  std::vector<double> va((size_t)nqc), vb((size_t)nqc, 0.0);
+  for (int c = 0; c < op->mesh.ncells; ++c) {
+    op->cell_geometry(c, s);
+    for (int d = 0; d < ncomp; ++d) {
+      op->eval_cell(a + (int64_t)d * op->N, c, va.data(), s);
+      if (b) op->eval_cell(b + (int64_t)d * op->N, c, vb.data(), s);
+      for (int q = 0; q < nqc; ++q) {
+        const double df = va[q] - vb[q];
+        se += s.JxW[q] * df * df;
+        sn += s.JxW[q] * vb[q] * vb[q];
+      }
+    }
+  }
+  *err = std::sqrt(se);
+  *ref = std::sqrt(sn);
+  return 0;
+}
+
 int orc_basis_tables(const orc_operator* op, double* nodes, double* qp, double* qw, double* S, double* D) {
   std::copy(op->B.nodes.begin(), op->B.nodes.end(), nodes);
   std::copy(op->B.qp.begin(), op->B.qp.end(), qp);

@@ -56,6 +56,8 @@ int orc_init_hill_state(orc_operator* op, const afl_hill_state* st, double* rho,
 int orc_background(orc_operator* op, const afl_hill_state* st, double* rho_bar, double* p_bar, double* theta_bar);
 
 /* Tables for pinning tests. */
+int orc_diagnostics(orc_operator* op, const double* rho, const double* u, const double* p, double* out7);
+int orc_l2_difference(orc_operator* op, int ncomp, const double* a, const double* b, double* err, double* ref);
 int orc_basis_tables(const orc_operator* op, double* nodes, double* qp, double* qw, double* S, double* D);
 int orc_node_coords(const orc_operator* op, double* X); /* dim blocks of N */
 

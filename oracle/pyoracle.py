@@ -56,6 +56,8 @@ def lib():
         L.orc_background.argtypes = [C.c_void_p, C.POINTER(HillState), _P, _P, _P]
         L.orc_basis_tables.argtypes = [C.c_void_p, _P, _P, _P, _P, _P]
         L.orc_node_coords.argtypes = [C.c_void_p, _P]
+        L.orc_diagnostics.argtypes = [C.c_void_p, _P, _P, _P, _P]
+        L.orc_l2_difference.argtypes = [C.c_void_p, C.c_int, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -158,6 +160,20 @@ class OracleOperator:
         S, D = np.zeros(nq * n1), np.zeros(nq * n1)
         self._chk(self.L.orc_basis_tables(self.h, _ptr(nodes), _ptr(qp), _ptr(qw), _ptr(S), _ptr(D)))
         return nodes, qp, qw, S.reshape(nq, n1), D.reshape(nq, n1)
+
+    def diagnostics(self, rho, u=None, p=None):
+        out = np.zeros(7)
+        self._chk(self.L.orc_diagnostics(self.h, _ptr(np.ascontiguousarray(rho)),
+                                         None if u is None else _ptr(np.ascontiguousarray(u)),
+                                         None if p is None else _ptr(np.ascontiguousarray(p)), _ptr(out)))
+        keys = ("min_density", "max_density", "max_celerity", "max_velocity", "min_cell_diameter", "cfl_c", "cfl_u")
+        return dict(zip(keys, out.tolist()))
+
+    def l2_difference(self, ncomp, a, b=None):
+        e, r = C.c_double(), C.c_double()
+        self._chk(self.L.orc_l2_difference(self.h, ncomp, _ptr(np.ascontiguousarray(a)),
+                                           None if b is None else _ptr(np.ascontiguousarray(b)), C.byref(e), C.byref(r)))
+        return e.value, r.value
 
     def node_coords(self):
         X = np.zeros(self.dim * self.N)

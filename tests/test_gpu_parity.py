@@ -209,3 +209,27 @@ def test_init_state_matches_oracle(oracle_mod):
     assert rel(rg.cpu().numpy(), rc) <= 1e-14
     assert rel(ug.cpu().numpy(), uc) <= 1e-14
     assert rel(pg.cpu().numpy(), pc) <= 1e-14
+
+
+@pytest.mark.parametrize("dim,k,nc", [(2, 2, (16, 1, 6)), (2, 4, (12, 1, 5)), (3, 2, (4, 3, 3)), (3, 3, (3, 2, 2))])
+def test_diagnostics_match_oracle(oracle_mod, dim, k, nc):
+    say(f"[gpu parity] diagnostics dim={dim} k={k} cells={nc}")
+    import torch
+    xr = (-6e3, 6e3)
+    mesh = make_mesh(dim=dim, degree=k, n_cells=nc, x_range=xr, y_range=xr, z_top=25e3, hill_height=400.0,
+                     hill_width=1000.0)
+    cpu = oracle_mod.OracleOperator(mesh, mach=1.0, froude=FR, dt=2.0)
+    gpu = EulerOperator(mesh, mach=1.0, froude=FR, dt=2.0)
+    rho, u, p = cpu.init_hill_state(make_hill_state(theta_sigma=0.3, seed=8))
+    u = u + np.random.default_rng(2).standard_normal(u.size)
+    dc = cpu.diagnostics(rho, u, p)
+    dg = gpu.diagnostics(*(torch.from_numpy(a.copy()).cuda() for a in (rho, u, p)))
+    for key, v in dc.items():
+        assert abs(dg[key] - v) <= 1e-13 * abs(v), key
+    ref = rho * (1 + 1e-3 * np.random.default_rng(5).standard_normal(rho.size))
+    ec, rc = cpu.l2_difference(1, rho, ref)
+    eg, rg = gpu.l2_difference(1, torch.from_numpy(rho.copy()).cuda(), torch.from_numpy(ref.copy()).cuda())
+    assert abs(eg - ec) <= 1e-12 * ec and abs(rg - rc) <= 1e-12 * rc
+    ec, rc = cpu.l2_difference(dim, u)
+    eg, rg = gpu.l2_difference(dim, torch.from_numpy(u.copy()).cuda())
+    assert abs(eg - ec) <= 1e-12 * ec and rg == 0.0 and rc == 0.0
