@@ -404,8 +404,9 @@ int ensure_mg(afl_operator* op) {
   if ((rc = op->mgw.alloc(op->N() + red_work_size(1) + 16))) return rc;
   // stage coefficients (rho_for_fixed, pres_fixed) on every level
   if (op->mg_coef_stale) {
-    if (want == AFL_SCHUR_MG_FP32) op->mg32.set_coefficients(op->rho_fix.p, op->p_fix.p);
-    else op->mg64.set_coefficients(op->rho_fix.p, op->p_fix.p);
+    // rho_for_fixed changed whenever the eigenvalue estimate is stale
+    if (want == AFL_SCHUR_MG_FP32) op->mg32.set_coefficients(op->rho_fix.p, op->p_fix.p, op->mg_eig_stale);
+    else op->mg64.set_coefficients(op->rho_fix.p, op->p_fix.p, op->mg_eig_stale);
     op->mg_coef_stale = false;
   }
   if (op->mg_eig_stale) {
@@ -931,6 +932,7 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rhoc.p);
   vec_copy(st, N, op->rhoc.p, op->rho_fix.p);
   op->mg_coef_stale = true;
+  op->mg_eig_stale = true;
   {
     const double* s[9] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p};
     launch_rhs_u(o, 3, s, op->rhs_u.p);

@@ -36,16 +36,22 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst);
 void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst);
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x);
 void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* w);
+// Multigrid level operator in precision R. sinv != nullptr selects the mean-density block
+// solve (w = P^{-1} B p, sinv per cell from launch_cell_rho_mass_inv); otherwise the cell-local
+// A solve runs to tol with rho.
 template <typename R>
-void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
-                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
+void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
 template <typename R>
-void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, R f, R tol,
-                      int maxit, int64_t N, const R* p, R* w);
+void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+                      R f, R tol, int maxit, int64_t N, const R* p, R* w);
 template <typename R>
-void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
-                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar, const R* rin, R* rout,
-                       R c1, R c2, R* d, R* x);
+void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
+                       const R* rin, R* rout, R c1, R c2, R* d, R* x);
+template <typename R>
+void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho,
+                              R* sinv);
 
 // diagnostics (afl_diag.hip): out4 = {-min rho, max |rho|, max celerity, max speed};
 // out2 = {sum JxW (a-b)^2, sum JxW b^2}; work holds diag_work_size(o) doubles

@@ -32,7 +32,10 @@ struct MgLevelT {
   int cell_maxit = 40;
   hipStream_t stream = nullptr;
   R *rho = nullptr, *pfix = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *r2 = nullptr, *z = nullptr,
-    *d = nullptr, *t = nullptr, *w = nullptr, *jbar = nullptr, *ev = nullptr;
+    *d = nullptr, *t = nullptr, *w = nullptr, *jbar = nullptr, *ev = nullptr, *sinv = nullptr;
+  // preconditioner-accuracy cell solves (tolerance >= 0.05) use the mean-density block P = (sum
+  // rho JxW) (M1 x M1 [x M1]) with sinv = 1 / sum rho JxW per cell instead of iterating on A
+  bool lean() const { return cell_tol >= 0.05; }
   bool ev_valid = false;  // ev holds the power-iteration vector of an earlier estimate
 };
 
@@ -44,7 +47,8 @@ struct MgT {
   double smoothing_range = 15.0;
   int setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k);
   void release();
-  void set_coefficients(const double* rho_fine, const double* pfix_fine);
+  // nodal interpolation of pres_fixed (and of rho_for_fixed when rho_changed) to every level
+  void set_coefficients(const double* rho_fine, const double* pfix_fine, bool rho_changed);
   int estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, double* scal_d, double* scal_h);
   void apply(const OpCtx& fine, const double* v, double* z);
   // fine-level w = A^{-1} B p in precision R on the level's coefficient copies

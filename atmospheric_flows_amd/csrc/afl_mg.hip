@@ -295,7 +295,8 @@ static void level_schur(const MgLevelT<R>& L, const OpCtx& fine, const R* p, R* 
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
   const R tol = (R)L.cell_tol;
-  launch_schur_pair<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.pfix, f, cdt, tol, L.cell_maxit, L.N, p, L.w, y);
+  launch_schur_pair<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt, tol,
+                       L.cell_maxit, L.N, p, L.w, y);
 }
 
 template <typename R>
@@ -334,7 +335,8 @@ static void level_schur_cheb(const MgLevelT<R>& L, const OpCtx& fine, const R* p
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
-  launch_schur_cheb<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.pfix, f, cdt, (R)L.cell_tol, L.cell_maxit, L.N, p,
+  launch_schur_cheb<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt,
+                       (R)L.cell_tol, L.cell_maxit, L.N, p,
                        L.w, L.jbar, rin, rout, (R)c1, (R)c2, d, x);
 }
 
@@ -441,6 +443,7 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
     }
     const int64_t nv = L.N * L.dim;
     R** bufs[] = {&L.rho, &L.pfix, &L.x, &L.b, &L.r, &L.r2, &L.z, &L.d, &L.t, &L.ev};
+    if (hipMalloc((void**)&L.sinv, L.ncells * sizeof(R)) != hipSuccess) return 3;
     L.ev_valid = false;
     for (R** b : bufs)
       if (hipMalloc((void**)b, L.N * sizeof(R)) != hipSuccess) return 3;
@@ -456,7 +459,7 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
 template <typename R>
 void MgT<R>::release() {
   for (auto& L : levels) {
-    R* bufs[] = {L.rho, L.pfix, L.x, L.b, L.r, L.r2, L.z, L.d, L.t, L.w, L.jbar, L.ev};
+    R* bufs[] = {L.rho, L.pfix, L.x, L.b, L.r, L.r2, L.z, L.d, L.t, L.w, L.jbar, L.ev, L.sinv};
     for (R* b : bufs)
       if (b) hipFree(b);
   }
@@ -465,7 +468,7 @@ void MgT<R>::release() {
 
 // coefficients of the current stage: nodal interpolation of rho_for_fixed and pres_fixed
 template <typename R>
-void MgT<R>::set_coefficients(const double* rho_fine, const double* pfix_fine) {
+void MgT<R>::set_coefficients(const double* rho_fine, const double* pfix_fine, bool rho_changed) {
   for (auto& L : levels) {
     Mat5<double> M;
     for (int i = 0; i < 5; ++i)
@@ -475,16 +478,18 @@ void MgT<R>::set_coefficients(const double* rho_fine, const double* pfix_fine) {
     const int kf = levels[0].k;
     const int64_t Nf = levels[0].N;
     if (L.k == kf) {
-      k_convert<double, R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, rho_fine, L.rho);
+      if (rho_changed) k_convert<double, R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, rho_fine, L.rho);
       k_convert<double, R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, pfix_fine, L.pfix);
-      continue;
-    }
-    AFL_DISPATCH_PAIR(L.dim, L.k, kf, {
-      k_tensor_map<D_, KO_, KI_, false, double, R><<<dim3(g), dim3(AFL_NT), 0, L.stream>>>(L.ncells, 1, Nf, L.N, M,
-                                                                                            rho_fine, L.rho);
-      k_tensor_map<D_, KO_, KI_, false, double, R><<<dim3(g), dim3(AFL_NT), 0, L.stream>>>(L.ncells, 1, Nf, L.N, M,
+    } else {
+      AFL_DISPATCH_PAIR(L.dim, L.k, kf, {
+        if (rho_changed)
+          k_tensor_map<D_, KO_, KI_, false, double, R><<<dim3(g), dim3(AFL_NT), 0, L.stream>>>(L.ncells, 1, Nf, L.N, M,
+                                                                                              rho_fine, L.rho);
+        k_tensor_map<D_, KO_, KI_, false, double, R><<<dim3(g), dim3(AFL_NT), 0, L.stream>>>(L.ncells, 1, Nf, L.N, M,
                                                                                             pfix_fine, L.pfix);
-    });
+      });
+    }
+    if (rho_changed && L.lean()) launch_cell_rho_mass_inv<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.sinv);
   }
 }
 
@@ -535,7 +540,8 @@ template <typename R>
 void MgT<R>::schur_velocity(const OpCtx& fine, const R* p, R* w) {
   const MgLevelT<R>& L = levels[0];
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
-  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, (R)(coeff * fine.dt / (fine.Ma * fine.Ma)),
+  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.lean() ? L.sinv : nullptr,
+                      (R)(coeff * fine.dt / (fine.Ma * fine.Ma)),
                       (R)L.cell_tol, L.cell_maxit, L.N, p, w);
 }
 

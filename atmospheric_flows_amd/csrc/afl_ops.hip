@@ -1024,6 +1024,155 @@ __global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, TabT<R> T, const R* _
   for (int c = 0; c < DIM; ++c) store_row<DIM, K, R>(w + c * N, s, xx[c]);
 }
 
+// rows (held by the row owners) <- (MI x MI [x MI]) rows for NC components: the inverse of the
+// reference-cell mass matrix, one LDS transpose per further axis. tmp: 2*NC*NPC values per cell.
+template <int DIM, int K, int NC, typename R>
+__device__ inline void cell_mi_apply(const Slot<DIM, K>& s, const TabT<R>& T, R* tmp, R (&rows)[NC][K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
+  R* t2 = tmp + NC * NPC;
+  const bool ro = s.act && s.l < ROWS;
+  if (ro) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        R a = R(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) a += T.MI[i][j] * rows[c][j];
+        tmp[c * NPC + s.l * N1 + i] = a;
+      }
+  }
+  __syncthreads();
+  const R* res;
+  if (DIM == 2) {
+    if (s.act && s.l < N1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R v[N1];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = tmp[c * NPC + j * N1 + s.l];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          R a = R(0);
+#pragma unroll
+          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+          t2[c * NPC + i * N1 + s.l] = a;
+        }
+      }
+    }
+    __syncthreads();
+    res = t2;
+  } else {
+    if (s.act && s.l < N1 * N1) {
+      const int ix = s.l % N1, iz = s.l / N1;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R v[N1];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = tmp[c * NPC + (iz * N1 + j) * N1 + ix];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          R a = R(0);
+#pragma unroll
+          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+          t2[c * NPC + (iz * N1 + i) * N1 + ix] = a;
+        }
+      }
+    }
+    __syncthreads();
+    if (s.act && s.l < N1 * N1) {
+      const int ix = s.l % N1, iy = s.l / N1;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R v[N1];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v[j] = t2[c * NPC + (j * N1 + iy) * N1 + ix];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          R a = R(0);
+#pragma unroll
+          for (int j = 0; j < N1; ++j) a += T.MI[i][j] * v[j];
+          tmp[c * NPC + (i * N1 + iy) * N1 + ix] = a;
+        }
+      }
+    }
+    __syncthreads();
+    res = tmp;
+  }
+  if (ro)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) rows[c][i] = res[c * NPC + s.l * N1 + i];
+}
+
+// Multigrid level operator, first half, at preconditioner accuracy: w = P^{-1} B p with the
+// mean-density block P = (sum_q rho JxW) (M1 x M1 [x M1]) in place of A (sinv[c] = 1 / sum_q rho
+// JxW of cell c, refreshed with the stage coefficients). P^{-1} A has its spectrum within the
+// cell's relative variation of rho J about the mean, a few percent.
+template <int DIM, int K, typename R>
+__global__ __launch_bounds__(AFL_NT) void k_schur_w_lvl(Geo g, TabT<R> T, const R* __restrict__ sinv, R f, int64_t N,
+                                                         const R* __restrict__ p, R* __restrict__ w) {
+  using C = Cfg<DIM, K>;
+  constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;
+  constexpr int PM = 2 * DIM * C::NPC;
+  constexpr int PC = PB > PM ? PB : PM;
+  __shared__ R shw[C::CPB * PC];
+  constexpr int N1 = C::N1;
+  Slot<DIM, K> s(g);
+  const int cbs = s.cb < C::CPB ? s.cb : 0;
+  R* base = shw + cbs * PC;
+  LineGeo<DIM, K, R> lg;
+  lg.init(g, T, s.ic, s.l);
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K, R> fp(g, T, s);
+  R bp[DIM][N1];
+  apply_B_rows<DIM, K, R>(g, T, s, lg, nb, fp, p, f, base, base + C::EV1, base, base + C::IN1,
+                          base + Lds<DIM, K>::W, base + Lds<DIM, K>::WF, bp);
+  // the mass-inverse transposes use [0, PM), disjoint from the face fluxes add_faces may still read
+  cell_mi_apply<DIM, K, DIM, R>(s, T, base, bp);
+  const R sc = s.act ? sinv[s.cell] : R(0);
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) bp[c][i] *= sc;
+    store_row<DIM, K, R>(w + c * N, s, bp[c]);
+  }
+}
+
+// sinv[c] = 1 / sum_q rho(q) JxW(q) of cell c (one thread per cell)
+template <int DIM, int K, typename R>
+__global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, TabT<R> T, const R* __restrict__ rho,
+                                                               R* __restrict__ sinv) {
+  using C = Cfg<DIM, K>;
+  constexpr int NQ = C::NQ, N1 = C::N1, NPC = C::NPC;
+  for (int64_t c = blockIdx.x * (int64_t)AFL_NT + threadIdx.x; c < g.ncells; c += (int64_t)gridDim.x * AFL_NT) {
+    int ic[3];
+    ic[0] = (int)(c % g.nc[0]);
+    if (DIM == 2) { ic[1] = (int)(c / g.nc[0]); ic[2] = 0; }
+    else { ic[1] = (int)((c / g.nc[0]) % g.nc[1]); ic[2] = (int)(c / ((int64_t)g.nc[0] * g.nc[1])); }
+    const R* r = rho + c * NPC;
+    double acc = 0.0;
+    for (int l = 0; l < C::LT; ++l) {
+      LineGeo<DIM, K, R> lg;
+      lg.init(g, T, ic, l);
+      const int qx = l % NQ, qy = DIM == 3 ? l / NQ : 0;
+      for (int qz = 0; qz < NQ; ++qz) {
+        double v = 0.0;
+        for (int i = 0; i < NPC; ++i) {
+          const int ix = i % N1, iy = DIM == 3 ? (i / N1) % N1 : 0, iz = i / (DIM == 3 ? N1 * N1 : N1);
+          double phi = (double)T.S[qx][ix] * (double)T.S[qz][iz];
+          if (DIM == 3) phi *= (double)T.S[qy][iy];
+          v += phi * (double)r[i];
+        }
+        acc += v * (double)lg.jxw(g, T, qz);
+      }
+    }
+    sinv[c] = (R)(1.0 / acc);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Launchers (dispatch on dim and degree).
 template <int DIM, int K>
@@ -1177,49 +1326,65 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
 
 // Level Schur apply y = D p - C A^{-1} B p in precision R (multigrid levels).
 template <typename R>
-void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
-                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
+void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
   AFL_DISPATCH(dim, k, {
     const dim3 gr = grid_for<D_, K_>(geo.ncells);
-    k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+    else k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
     k_apply_C<D_, K_, true, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
   });
 }
 // w = A^{-1} B p alone in precision R (the first half of the level operator)
 template <typename R>
-void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, R f, R tol,
-                      int maxit, int64_t N, const R* p, R* w) {
+void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+                      R f, R tol, int maxit, int64_t N, const R* p, R* w) {
   AFL_DISPATCH(dim, k, {
-    k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    if (sinv)
+      k_schur_w_lvl<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+    else
+      k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
   });
 }
-template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*, float, float,
-                                      int, int64_t, const float*, float*);
-template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*, double,
-                                       double, int, int64_t, const double*, double*);
+template <typename R>
+void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho,
+                              R* sinv) {
+  const int nb = (int)((geo.ncells + AFL_NT - 1) / AFL_NT);
+  AFL_DISPATCH(dim, k, { k_cell_rho_mass_inv<D_, K_, R><<<dim3(nb), dim3(AFL_NT), 0, st>>>(geo, T, rho, sinv); });
+}
+template void launch_cell_rho_mass_inv<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+                                              float*);
+template void launch_cell_rho_mass_inv<double>(int, int, const Geo&, const TabT<double>&, hipStream_t,
+                                               const double*, double*);
+template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+                                      const float*, float, float, int, int64_t, const float*, float*);
+template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
+                                       const double*, double, double, int, int64_t, const double*, double*);
 
 // w = A^{-1} B p followed by the fused C/D apply and Chebyshev step (see k_apply_C_cheb)
 template <typename R>
-void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* pfix,
-                       R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar, const R* rin, R* rout,
-                       R c1, R c2, R* d, R* x) {
+void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
+                       const R* rin, R* rout, R c1, R c2, R* d, R* x) {
   AFL_DISPATCH(dim, k, {
     const dim3 gr = grid_for<D_, K_>(geo.ncells);
-    k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+    else k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
     k_apply_C_cheb<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), jbar,
                                                            rin, rout, c1, c2, d, x);
   });
 }
 template void launch_schur_cheb<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
-                                       const float*, float, float, float, int, int64_t, const float*, float*,
+                                       const float*, const float*, float, float, float, int, int64_t, const float*, float*,
                                        const float*, const float*, float*, float, float, float*, float*);
 template void launch_schur_cheb<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
-                                        const double*, double, double, double, int, int64_t, const double*, double*,
+                                        const double*, const double*, double, double, double, int, int64_t, const double*, double*,
                                         const double*, const double*, double*, double, double, double*, double*);
 template void launch_schur_pair<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
-                                       const float*, float, float, float, int, int64_t, const float*, float*, float*);
+                                       const float*, const float*, float, float, float, int, int64_t, const float*, float*,
+                                       float*);
 template void launch_schur_pair<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
-                                        const double*, double, double, double, int, int64_t, const double*, double*,
-                                        double*);
+                                        const double*, const double*, double, double, double, int, int64_t,
+                                        const double*, double*, double*);
 
 }  // namespace afl
