@@ -190,12 +190,14 @@ struct afl_operator {
   // IMEX work vectors
   DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
   double* host_sc = nullptr;  // pinned scalars
+  Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
   ~afl_operator() {
     DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
                      &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix};
     for (DevBuf* b : all) b->release();
     if (host_sc) hipHostFree(host_sc);
+    if (dtab) hipFree(dtab);
   }
   int64_t N() const { return o.N; }
   int dim() const { return o.dim; }
@@ -545,6 +547,12 @@ int afl_create(const afl_mesh_desc* md, double mach, double froude, double dt, a
   o.k = md->degree;
   build_tab(o.k, md->node_type, o.T);
   for (int kk = 1; kk <= 4; ++kk) build_tab(kk, md->node_type, op->tabs_by_k[kk]);
+  if (hipMalloc((void**)&op->dtab, sizeof(Tab)) != hipSuccess ||
+      hipMemcpy(op->dtab, &o.T, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess) {
+    delete op;
+    return fail(AFL_ERR_DEVICE, "afl_create: table upload failed");
+  }
+  o.dT = op->dtab;
   Geo& g = o.geo;
   g.nc[0] = nx; g.nc[1] = md->dim == 3 ? ny : nz; g.nc[2] = md->dim == 3 ? nz : 1;
   g.ncells = (int)ncells;

@@ -43,6 +43,33 @@ struct Geo {
   double H, h0, inv_a2;
 };
 
+// Runtime-indexed reads of the basis tables. The tables travel in the kernel argument (uniform
+// values); a select chain keeps them there, where an indexed memory access would make the
+// compiler copy the whole table into per-lane scratch.
+template <int CNT, int N, typename R>
+__device__ __attribute__((always_inline)) inline R pick(const R (&a)[N], int i) {
+  R v = a[0];
+#pragma unroll
+  for (int j = 1; j < CNT; ++j) v = (i == j) ? a[j] : v;
+  return v;
+}
+// S[r][col] for a runtime row r < ROWS and a compile-time column
+template <int ROWS, typename R>
+__device__ __attribute__((always_inline)) inline R pick_row(const R (&S)[9][5], int r, int col) {
+  R v = S[0][col];
+#pragma unroll
+  for (int j = 1; j < ROWS; ++j) v = (r == j) ? S[j][col] : v;
+  return v;
+}
+// S[row][c] for a compile-time row and a runtime column c < COLS
+template <int COLS, typename R>
+__device__ __attribute__((always_inline)) inline R pick_col(const R (&S)[9][5], int row, int c) {
+  R v = S[row][0];
+#pragma unroll
+  for (int j = 1; j < COLS; ++j) v = (c == j) ? S[row][j] : v;
+  return v;
+}
+
 template <int DIM, int K>
 struct Cfg {
   static constexpr int N1 = K + 1;
@@ -82,11 +109,11 @@ struct Slot {
 };
 
 template <int DIM>
-__device__ inline int vaxis() { return DIM - 1; }
+__device__ __attribute__((always_inline)) inline int vaxis() { return DIM - 1; }
 
 // Hill h(x,y) = h0 / (1 + r^2/a^2) and its horizontal derivatives (arithmetic in R).
 template <typename R>
-__device__ inline void hill(const Geo& g, R x, R y, R& h, R& hx, R& hy) {
+__device__ __attribute__((always_inline)) inline void hill(const Geo& g, R x, R y, R& h, R& hx, R& hy) {
   const R s = R(1) + (x * x + y * y) * R(g.inv_a2);
   const R is = R(1) / s;
   h = R(g.h0) * is;
@@ -111,13 +138,13 @@ struct LineGeo {
     zeta0 = R(ic[DIM - 1] * g.hc[DIM - 1]);
     R x, y = R(0);
     if (DIM == 2) {
-      x = R(g.lo[0] + (ic[0] + (double)T.QP[l]) * g.hc[0]);
-      wxy = T.W[l];
+      x = R(g.lo[0] + (ic[0] + (double)pick<C::NQ>(T.QP, l)) * g.hc[0]);
+      wxy = pick<C::NQ>(T.W, l);
     } else {
       const int qx = l % C::NQ, qy = l / C::NQ;
-      x = R(g.lo[0] + (ic[0] + (double)T.QP[qx]) * g.hc[0]);
-      y = R(g.lo[1] + (ic[1] + (double)T.QP[qy]) * g.hc[1]);
-      wxy = T.W[qx] * T.W[qy];
+      x = R(g.lo[0] + (ic[0] + (double)pick<C::NQ>(T.QP, qx)) * g.hc[0]);
+      y = R(g.lo[1] + (ic[1] + (double)pick<C::NQ>(T.QP, qy)) * g.hc[1]);
+      wxy = pick<C::NQ>(T.W, qx) * pick<C::NQ>(T.W, qy);
     }
     R h;
     hill(g, x, y, h, dhx, dhy);
@@ -148,7 +175,7 @@ struct LineGeo {
 // Geometry of face f = 2*dir + side of cell ic at face point fq (2D: fq = q; 3D: fq = qa + NQ qb,
 // qa/qb along the two tangential axes in increasing order). Returns outward normal and JxW.
 template <int DIM, int K, typename R = double>
-__device__ inline void face_geo(const Geo& g, const TabT<R>& T, const int* ic, int f, int fq, R* n, R& jxw) {
+__device__ __attribute__((always_inline)) inline void face_geo(const Geo& g, const TabT<R>& T, const int* ic, int f, int fq, R* n, R& jxw) {
   using C = Cfg<DIM, K>;
   const int dir = f >> 1, side = f & 1;
   const R sg = side ? R(1) : R(-1);
@@ -159,32 +186,33 @@ __device__ inline void face_geo(const Geo& g, const TabT<R>& T, const int* ic, i
       R h, hx, hy;
       hill(g, x, R(0), h, hx, hy);
       n[0] = sg; n[1] = R(0);
-      jxw = R(g.hc[1]) * (R(1) - h * iH) * T.W[fq];
+      jxw = R(g.hc[1]) * (R(1) - h * iH) * pick<C::NQ>(T.W, fq);
     } else {
-      const R x = R(g.lo[0] + (ic[0] + (double)T.QP[fq]) * g.hc[0]);
+      const R x = R(g.lo[0] + (ic[0] + (double)pick<C::NQ>(T.QP, fq)) * g.hc[0]);
       R h, hx, hy;
       hill(g, x, R(0), h, hx, hy);
       const R zeta = R((ic[1] + side) * g.hc[1]);
       const R vx = -hx * (R(1) - zeta * iH);
       const R nv = sqrt(R(1) + vx * vx);
       n[0] = sg * vx / nv; n[1] = sg / nv;
-      jxw = R(g.hc[0]) * nv * T.W[fq];
+      jxw = R(g.hc[0]) * nv * pick<C::NQ>(T.W, fq);
     }
   } else {
     const int qa = fq % C::NQ, qb = fq / C::NQ;
-    const R w = T.W[qa] * T.W[qb];
+    const R w = pick<C::NQ>(T.W, qa) * pick<C::NQ>(T.W, qb);
+    const double xa = (double)pick<C::NQ>(T.QP, qa), xb = (double)pick<C::NQ>(T.QP, qb);
     double x, y, zeta;
     if (dir == 0) {
       x = g.lo[0] + (ic[0] + side) * g.hc[0];
-      y = g.lo[1] + (ic[1] + (double)T.QP[qa]) * g.hc[1];
-      zeta = (ic[2] + (double)T.QP[qb]) * g.hc[2];
+      y = g.lo[1] + (ic[1] + xa) * g.hc[1];
+      zeta = (ic[2] + xb) * g.hc[2];
     } else if (dir == 1) {
-      x = g.lo[0] + (ic[0] + (double)T.QP[qa]) * g.hc[0];
+      x = g.lo[0] + (ic[0] + xa) * g.hc[0];
       y = g.lo[1] + (ic[1] + side) * g.hc[1];
-      zeta = (ic[2] + (double)T.QP[qb]) * g.hc[2];
+      zeta = (ic[2] + xb) * g.hc[2];
     } else {
-      x = g.lo[0] + (ic[0] + (double)T.QP[qa]) * g.hc[0];
-      y = g.lo[1] + (ic[1] + (double)T.QP[qb]) * g.hc[1];
+      x = g.lo[0] + (ic[0] + xa) * g.hc[0];
+      y = g.lo[1] + (ic[1] + xb) * g.hc[1];
       zeta = (ic[2] + side) * g.hc[2];
     }
     R h, hx, hy;
@@ -204,7 +232,7 @@ __device__ inline void face_geo(const Geo& g, const TabT<R>& T, const int* ic, i
 
 // Neighbour across face f (-1 for a wall).
 template <int DIM>
-__device__ inline int64_t neighbour(const Geo& g, const int* ic, int f) {
+__device__ __attribute__((always_inline)) inline int64_t neighbour(const Geo& g, const int* ic, int f) {
   const int dir = f >> 1, side = f & 1;
   const int n = g.nc[dir];
   int j[3] = {ic[0], ic[1], ic[2]};
@@ -220,7 +248,7 @@ __device__ inline int64_t neighbour(const Geo& g, const int* ic, int f) {
 
 // Node index of face node t (2D: t along the face; 3D: t = ta + N1 tb) on face (dir, side).
 template <int DIM, int K>
-__device__ inline int face_node(int dir, int side, int t) {
+__device__ __attribute__((always_inline)) inline int face_node(int dir, int side, int t) {
   constexpr int N1 = K + 1;
   const int s = side ? K : 0;
   if (DIM == 2) return dir == 0 ? t * N1 + s : s * N1 + t;
@@ -234,7 +262,7 @@ __device__ inline int face_node(int dir, int side, int t) {
 // Evaluate a scalar field (global dofs) at the quadrature points of the thread's line.
 // All threads of the block must call it (it contains barriers).
 template <int DIM, int K, typename R = double>
-__device__ inline void eval_line(const R* __restrict__ f, const Slot<DIM, K>& s, R* t1, R* t2,
+__device__ __attribute__((always_inline)) inline void eval_line(const R* __restrict__ f, const Slot<DIM, K>& s, R* t1, R* t2,
                                  const TabT<R>& T, R (&v)[2 * K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ;
@@ -334,7 +362,7 @@ struct FaceLds {
 // loads per face direction) into fn during the x sweep; face-point threads interpolate after the
 // first barrier, so no barrier is added. fn: FaceLds::FN doubles per cell.
 template <int DIM, int K, typename R = double>
-__device__ inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
+__device__ __attribute__((always_inline)) inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
                                        R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[2 * K + 1],
                                        R (&tr)[2 * DIM][2]) {
   using C = Cfg<DIM, K>;
@@ -387,15 +415,15 @@ __device__ inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, 
         R a = 0.0;
         if (DIM == 2) {
 #pragma unroll
-          for (int t = 0; t < N1; ++t) a += T.S[s.l][t] * FN(fc, sd, t);
+          for (int t = 0; t < N1; ++t) a += pick_row<NQ>(T.S, s.l, t) * FN(fc, sd, t);
         } else {
           const int qa = s.l % NQ, qb = s.l / NQ;
 #pragma unroll
           for (int tb = 0; tb < N1; ++tb) {
             R r = 0.0;
 #pragma unroll
-            for (int ta = 0; ta < N1; ++ta) r += T.S[qa][ta] * FN(fc, sd, ta + N1 * tb);
-            a += T.S[qb][tb] * r;
+            for (int ta = 0; ta < N1; ++ta) r += pick_row<NQ>(T.S, qa, ta) * FN(fc, sd, ta + N1 * tb);
+            a += pick_row<NQ>(T.S, qb, tb) * r;
           }
         }
         tr[fc][sd] = a;
@@ -449,9 +477,148 @@ __device__ inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, 
   }
 }
 
+// Batched eval_line_faces: NF scalar fields evaluated together (one barrier per sweep for all of
+// them), the first NFF of them also at the face points. Per cell: t1 NF*EV1, t2 NF*EV2 and
+// fn NFF*FaceLds::FN values. Row owners issue all NF row loads before the sweeps.
+template <int DIM, int K, int NF, int NFF, typename R = double>
+__device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* const (&f)[NF], const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
+                                        R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[NF][2 * K + 1],
+                                        R (&tr)[NFF > 0 ? NFF : 1][2 * DIM][2]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN, FNS = FaceLds<DIM, K>::FN;
+  constexpr int E1 = C::EV1, E2 = C::EV2;
+  const int64_t cell = s.act ? s.cell : 0;
+  auto FN = [&](int a, int face, int side, int t) -> R& { return fn[a * FNS + (face * 2 + side) * NFN + t]; };
+  if (s.act && s.l < C::ROWS) {
+    R d[NF][N1];
+#pragma unroll
+    for (int a = 0; a < NF; ++a)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) d[a][i] = f[a][cell * NPC + s.l * N1 + i];
+#pragma unroll
+    for (int a = 0; a < NF; ++a)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        R acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) acc += T.S[q][i] * d[a][i];
+        t1[a * E1 + s.l * NQ + q] = acc;
+      }
+#pragma unroll
+    for (int a = 0; a < NFF; ++a) {
+      const R* u = f[a];
+      FN(a, 0, 0, s.l) = d[a][0];
+      FN(a, 1, 0, s.l) = d[a][K];
+      FN(a, 0, 1, s.l) = nb.c[0] >= 0 ? u[nb.c[0] * NPC + s.l * N1 + K] : d[a][0];
+      FN(a, 1, 1, s.l) = nb.c[1] >= 0 ? u[nb.c[1] * NPC + s.l * N1] : d[a][K];
+      if (DIM == 2) {
+        if (s.l == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i) = d[a][i];
+        if (s.l == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i) = d[a][i];
+        if (nb.c[2] >= 0) FN(a, 2, 1, s.l) = u[nb.c[2] * NPC + K * N1 + s.l];
+        if (nb.c[3] >= 0) FN(a, 3, 1, s.l) = u[nb.c[3] * NPC + s.l];
+      } else {
+        const int iy = s.l % N1, iz = s.l / N1;
+        if (iy == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i + N1 * iz) = d[a][i];
+        if (iy == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i + N1 * iz) = d[a][i];
+        if (nb.c[2] >= 0) FN(a, 2, 1, iy + N1 * iz) = u[nb.c[2] * NPC + (iz * N1 + K) * N1 + iy];
+        if (nb.c[3] >= 0) FN(a, 3, 1, iy + N1 * iz) = u[nb.c[3] * NPC + (iz * N1) * N1 + iy];
+        if (iz == 0) for (int i = 0; i < N1; ++i) FN(a, 4, 0, i + N1 * iy) = d[a][i];
+        if (iz == K) for (int i = 0; i < N1; ++i) FN(a, 5, 0, i + N1 * iy) = d[a][i];
+        if (nb.c[4] >= 0) FN(a, 4, 1, iy + N1 * iz) = u[nb.c[4] * NPC + (K * N1 + iz) * N1 + iy];
+        if (nb.c[5] >= 0) FN(a, 5, 1, iy + N1 * iz) = u[nb.c[5] * NPC + iz * N1 + iy];
+      }
+    }
+  }
+  __syncthreads();
+  if (s.act) {
+    // the thread's rows of the interpolation table (face point l), read once
+    const int qa = DIM == 2 ? s.l : s.l % NQ, qb = DIM == 2 ? 0 : s.l / NQ;
+    R sa[N1], sb[N1];
+#pragma unroll
+    for (int t = 0; t < N1; ++t) { sa[t] = pick_row<NQ>(T.S, qa, t); sb[t] = pick_row<NQ>(T.S, qb, t); }
+#pragma unroll
+    for (int a = 0; a < NFF; ++a) {
+#pragma unroll
+      for (int fc = 0; fc < 2 * DIM; ++fc)
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+          R acc = 0.0;
+          if (DIM == 2) {
+#pragma unroll
+            for (int t = 0; t < N1; ++t) acc += sa[t] * FN(a, fc, sd, t);
+          } else {
+#pragma unroll
+            for (int tb = 0; tb < N1; ++tb) {
+              R r = 0.0;
+#pragma unroll
+              for (int ta = 0; ta < N1; ++ta) r += sa[ta] * FN(a, fc, sd, ta + N1 * tb);
+              acc += sb[tb] * r;
+            }
+          }
+          tr[a][fc][sd] = acc;
+        }
+#pragma unroll
+      for (int fc = 0; fc < 2 * DIM; ++fc)
+        if (nb.c[fc] < 0) tr[a][fc][1] = tr[a][fc][0];
+    }
+  }
+  if (DIM == 2) {
+    if (s.act) {
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        R c[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) c[i] = t1[a * E1 + i * NQ + s.l];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          R acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) acc += T.S[q][i] * c[i];
+          v[a][q] = acc;
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        R c[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) c[i] = t1[a * E1 + (iz * N1 + i) * NQ + qx];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          R acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) acc += T.S[q][i] * c[i];
+          t2[a * E2 + (iz * NQ + q) * NQ + qx] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    if (s.act) {
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        R c[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) c[i] = t2[a * E2 + i * NQ * NQ + s.l];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          R acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) acc += T.S[q][i] * c[i];
+          v[a][q] = acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Same as eval_line but the dofs come from an LDS copy of the cell (u = lds + cb*NPC).
 template <int DIM, int K, typename R = double>
-__device__ inline void eval_line_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2,
+__device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2,
                                      const TabT<R>& T, R (&v)[2 * K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ;
@@ -514,7 +681,7 @@ __device__ inline void eval_line_lds(const R* u, const Slot<DIM, K>& s, R* t1, R
 // Batched evaluation of NF scalar fields whose cell dofs sit in LDS (field f at u + f*NPC).
 // t1 needs NF*EV1 doubles, t2 NF*EV2 doubles per cell. Contains barriers.
 template <int DIM, int K, int NF, typename R = double>
-__device__ inline void eval_lines_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2, const TabT<R>& T,
+__device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2, const TabT<R>& T,
                                       R (&v)[NF][2 * K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, E1 = C::EV1, E2 = C::EV2;
@@ -607,7 +774,7 @@ struct IntV {
 };
 
 template <int DIM, int K, int NF, typename R = double>
-__device__ inline void integrate_lines_v(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
+__device__ __attribute__((always_inline)) inline void integrate_lines_v(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
                                          const R (&V)[NF][2 * K + 1], R (&out)[NF][K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, A1 = IntV<DIM, K>::I1, A2 = IntV<DIM, K>::I2;
@@ -681,7 +848,7 @@ __device__ inline void integrate_lines_v(const Slot<DIM, K>& s, R* i1, R* i2, co
 // multiplied by JxW) against the test functions. Row `l` (l < ROWS) of the result ends up
 // in out[N1]. HAS_V / HAS_G select the terms. Contains barriers.
 template <int DIM, int K, bool HAS_V, bool HAS_G, typename R = double>
-__device__ inline void integrate_line(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
+__device__ __attribute__((always_inline)) inline void integrate_line(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
                                       const R (&V)[2 * K + 1], const R (&G)[DIM][2 * K + 1],
                                       R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
@@ -771,7 +938,7 @@ __device__ inline void integrate_line(const Slot<DIM, K>& s, R* i1, R* i2, const
 // Add face-flux integrals to node row l. Fq points to the LDS block [2*DIM][NFQ] of this cell
 // (values already multiplied by JxW_f and oriented for this cell). Only rows on faces change.
 template <int DIM, int K, typename R = double>
-__device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<R>& T, R (&out)[K + 1],
+__device__ __attribute__((always_inline)) inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<R>& T, R (&out)[K + 1],
                                  unsigned face_mask) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NQ = C::NQ, NFQ = C::NFQ;
@@ -779,8 +946,15 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<
   if (DIM == 2) {
     const int iz = s.l;
     // x faces: nodes (0, iz) and (K, iz)
-    if (face_mask & 1u) { R a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[0 * NFQ + q]; out[0] += a; }
-    if (face_mask & 2u) { R a = 0.0; for (int q = 0; q < NQ; ++q) a += T.S[q][iz] * Fq[1 * NFQ + q]; out[K] += a; }
+    R sz[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sz[q] = pick_col<N1>(T.S, q, iz);
+    if (face_mask & 1u) { R a = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) a += sz[q] * Fq[0 * NFQ + q]; out[0] += a; }
+    if (face_mask & 2u) { R a = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) a += sz[q] * Fq[1 * NFQ + q]; out[K] += a; }
     const int fz = (iz == 0) ? 2 : ((iz == K) ? 3 : -1);
     if (fz >= 0 && (face_mask & (1u << fz))) {
 #pragma unroll
@@ -788,14 +962,20 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<
     }
   } else {
     const int iy = s.l % N1, iz = s.l / N1;
+    R sy[NQ], sz[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) { sy[q] = pick_col<N1>(T.S, q, iy); sz[q] = pick_col<N1>(T.S, q, iz); }
+#pragma unroll
     for (int side = 0; side < 2; ++side) {
       if (!(face_mask & (1u << side))) continue;
       const R* F = Fq + side * NFQ;
       R a = 0.0;
+#pragma unroll
       for (int qb = 0; qb < NQ; ++qb) {
         R r = 0.0;
-        for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][iy] * F[qb * NQ + qa];
-        a += T.S[qb][iz] * r;
+#pragma unroll
+        for (int qa = 0; qa < NQ; ++qa) r += sy[qa] * F[qb * NQ + qa];
+        a += sz[qb] * r;
       }
       out[side ? K : 0] += a;
     }
@@ -808,7 +988,7 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<
         for (int qb = 0; qb < NQ; ++qb) {
           R r = 0.0;
           for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][i] * F[qb * NQ + qa];
-          a += T.S[qb][iz] * r;
+          a += sz[qb] * r;
         }
         out[i] += a;
       }
@@ -822,7 +1002,7 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<
         for (int qb = 0; qb < NQ; ++qb) {
           R r = 0.0;
           for (int qa = 0; qa < NQ; ++qa) r += T.S[qa][i] * F[qb * NQ + qa];
-          a += T.S[qb][iy] * r;
+          a += sy[qb] * r;
         }
         out[i] += a;
       }
@@ -832,7 +1012,7 @@ __device__ inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<
 
 // Write node row l of a cell.
 template <int DIM, int K, typename R = double>
-__device__ inline void store_row(R* __restrict__ dst, const Slot<DIM, K>& s, const R (&out)[K + 1]) {
+__device__ __attribute__((always_inline)) inline void store_row(R* __restrict__ dst, const Slot<DIM, K>& s, const R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
   if (s.act && s.l < C::ROWS) {
     R* o = dst + s.cell * C::NPC + s.l * C::N1;

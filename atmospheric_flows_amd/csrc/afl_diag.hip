@@ -72,9 +72,10 @@ __global__ __launch_bounds__(AFL_NT) void k_diag_finish(const double* __restrict
 // Per-block partial sums of sum_q JxW (a - b)^2 and sum_q JxW b^2 over ncomp components
 // (b == nullptr: b = 0). part[b] and part[nblocks + b].
 template <int DIM, int K>
-__global__ __launch_bounds__(AFL_NT) void k_l2_partial(Geo g, Tab T, int ncomp, int64_t N, const double* __restrict__ a,
+__global__ __launch_bounds__(AFL_NT) void k_l2_partial(Geo g, const Tab* __restrict__ Tp, int ncomp, int64_t N, const double* __restrict__ a,
                                                        const double* __restrict__ b, int64_t nblocks,
                                                        double* __restrict__ part) {
+  const auto& T = *Tp;
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ;
   __shared__ double shw[C::CPB * (C::EV1 + C::EV2)];
@@ -129,7 +130,7 @@ void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const doub
   case D_ * 10 + K_: {                                                                                 \
     using C = Cfg<D_, K_>;                                                                             \
     const int64_t nb = (o.geo.ncells + C::CPB - 1) / C::CPB;                                           \
-    k_l2_partial<D_, K_><<<dim3((unsigned)nb), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, ncomp, o.N, a, b, nb, \
+    k_l2_partial<D_, K_><<<dim3((unsigned)nb), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, ncomp, o.N, a, b, nb, \
                                                                             work);                     \
     k_diag_finish<false><<<dim3(2), dim3(AFL_NT), 0, o.stream>>>(work, nb, nb, out2);                  \
   } break;

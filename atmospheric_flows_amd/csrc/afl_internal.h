@@ -16,7 +16,8 @@ struct TableauC {
 struct OpCtx {
   int dim = 2, k = 1;
   Geo geo{};
-  Tab T{};
+  Tab T{};          // host copy of the basis tables
+  const Tab* dT = nullptr;  // device copy (kernels read the tables through this pointer)
   int64_t N = 0;          // nodes per scalar field
   double Ma = 1, Fr = 1, dt = 1;
   unsigned hyp = 1;       // HYPERBOLIC_stage
@@ -40,17 +41,17 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
 // solve (w = P^{-1} B p, sinv per cell from launch_cell_rho_mass_inv); otherwise the cell-local
 // A solve runs to tol with rho.
 template <typename R>
-void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
 template <typename R>
-void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                       R f, R tol, int maxit, int64_t N, const R* p, R* w);
 template <typename R>
-void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
                        const R* rin, R* rout, R c1, R c2, R* d, R* x);
 template <typename R>
-void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho,
+void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
                               R* sinv);
 
 // diagnostics (afl_diag.hip): out4 = {-min rho, max |rho|, max celerity, max speed};

@@ -36,7 +36,9 @@ struct MgLevelT {
   // preconditioner-accuracy cell solves (tolerance >= 0.05) use the mean-density block P = (sum
   // rho JxW) (M1 x M1 [x M1]) with sinv = 1 / sum rho JxW per cell instead of iterating on A
   bool lean() const { return cell_tol >= 0.05; }
-  bool ev_valid = false;  // ev holds the power-iteration vector of an earlier estimate
+  bool ev_valid = false;
+  TabT<R>* dT = nullptr;  // device copy of T (kernels read the tables through it)
+  Tab* dTd = nullptr;     // device FP64 tables of this degree  // ev holds the power-iteration vector of an earlier estimate
 };
 
 template <typename R>

@@ -171,7 +171,8 @@ __global__ __launch_bounds__(AFL_NT) void k_cheb_step(int64_t ncells, const R* _
 
 // cell-mean Jacobian: sum_q JxW_q (reference weights sum to one)
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) void k_jbar(Geo g, Tab T, R* __restrict__ jbar) {
+__global__ __launch_bounds__(AFL_NT) void k_jbar(Geo g, const Tab* __restrict__ Tp, R* __restrict__ jbar) {
+  const auto& T = *Tp;
   constexpr int NQ = 2 * K + 1;
   for (int64_t c = blockIdx.x * (int64_t)AFL_NT + threadIdx.x; c < g.ncells; c += (int64_t)gridDim.x * AFL_NT) {
     int ic[3];
@@ -295,7 +296,7 @@ static void level_schur(const MgLevelT<R>& L, const OpCtx& fine, const R* p, R* 
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
   const R tol = (R)L.cell_tol;
-  launch_schur_pair<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt, tol,
+  launch_schur_pair<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt, tol,
                        L.cell_maxit, L.N, p, L.w, y);
 }
 
@@ -335,7 +336,7 @@ static void level_schur_cheb(const MgLevelT<R>& L, const OpCtx& fine, const R* p
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
-  launch_schur_cheb<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt,
+  launch_schur_cheb<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt,
                        (R)L.cell_tol, L.cell_maxit, L.N, p,
                        L.w, L.jbar, rin, rout, (R)c1, (R)c2, d, x);
 }
@@ -441,6 +442,12 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
       for (int i = 0; i < 5; ++i)
         for (int j = 0; j < 5; ++j) L.Pt[i][j] = L.P[j][i];
     }
+    if (hipMalloc((void**)&L.dT, sizeof(TabT<R>)) != hipSuccess ||
+        hipMemcpy(L.dT, &L.T, sizeof(TabT<R>), hipMemcpyHostToDevice) != hipSuccess)
+      return 3;
+    if (hipMalloc((void**)&L.dTd, sizeof(Tab)) != hipSuccess ||
+        hipMemcpy(L.dTd, &tb, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess)
+      return 3;
     const int64_t nv = L.N * L.dim;
     R** bufs[] = {&L.rho, &L.pfix, &L.x, &L.b, &L.r, &L.r2, &L.z, &L.d, &L.t, &L.ev};
     if (hipMalloc((void**)&L.sinv, L.ncells * sizeof(R)) != hipSuccess) return 3;
@@ -450,7 +457,7 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
     if (hipMalloc((void**)&L.w, nv * sizeof(R)) != hipSuccess) return 3;
     if (hipMalloc((void**)&L.jbar, L.ncells * sizeof(R)) != hipSuccess) return 3;
     AFL_DISPATCH_K(L.dim, L.k, {
-      k_jbar<D_, K_, R><<<dim3(grid_n(L.ncells)), dim3(AFL_NT), 0, L.stream>>>(L.geo, tb, L.jbar);
+      k_jbar<D_, K_, R><<<dim3(grid_n(L.ncells)), dim3(AFL_NT), 0, L.stream>>>(L.geo, L.dTd, L.jbar);
     });
   }
   return 0;
@@ -462,6 +469,8 @@ void MgT<R>::release() {
     R* bufs[] = {L.rho, L.pfix, L.x, L.b, L.r, L.r2, L.z, L.d, L.t, L.w, L.jbar, L.ev, L.sinv};
     for (R* b : bufs)
       if (b) hipFree(b);
+    if (L.dT) hipFree(L.dT);
+    if (L.dTd) hipFree(L.dTd);
   }
   levels.clear();
 }
@@ -489,7 +498,7 @@ void MgT<R>::set_coefficients(const double* rho_fine, const double* pfix_fine, b
                                                                                             pfix_fine, L.pfix);
       });
     }
-    if (rho_changed && L.lean()) launch_cell_rho_mass_inv<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.sinv);
+    if (rho_changed && L.lean()) launch_cell_rho_mass_inv<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.sinv);
   }
 }
 
@@ -540,7 +549,7 @@ template <typename R>
 void MgT<R>::schur_velocity(const OpCtx& fine, const R* p, R* w) {
   const MgLevelT<R>& L = levels[0];
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
-  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.T, L.stream, L.rho, L.lean() ? L.sinv : nullptr,
+  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr,
                       (R)(coeff * fine.dt / (fine.Ma * fine.Ma)),
                       (R)L.cell_tol, L.cell_maxit, L.N, p, w);
 }

@@ -51,8 +51,9 @@ struct FacePts {
 // E1: vmult_rhs_rho_update (euler_operator.h:342-473 cell, 476-692 faces, boundary none).
 // src = {rho_j, u_j} for the NS explicit states; w = stage weights (a21 | a31,a32 | b1,b2,b3).
 template <int DIM, int K, int NS>
-__global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, Tab T, Srcs src, double dt, double w0, double w1,
+__global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, const Tab* __restrict__ Tp, Srcs src, double dt, double w0, double w1,
                                                      double w2, int64_t N, double* __restrict__ dst) {
+  const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   AFL_SLOT_PTRS
@@ -119,8 +120,9 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, Tab T, Srcs src, doub
 struct E2Coef { double w[3], wp[3], gw[3]; };
 
 template <int DIM, int K, int NE, int NG>
-__global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, Tab T, Srcs src, E2Coef cf, double dt, double iMa2,
+__global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__ Tp, Srcs src, E2Coef cf, double dt, double iMa2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
+  const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F * DIM];
   AFL_SLOT_PTRS
@@ -221,8 +223,9 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, Tab T, Srcs src, E2Coef
 struct E3Coef { double w[3], wt[3], gw[3]; };
 
 template <int DIM, int K, int NE, int NG, int STAGE>
-__global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, Tab T, Srcs src, E3Coef cf, double dt, double Ma2,
+__global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__ Tp, Srcs src, E3Coef cf, double dt, double Ma2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
+  const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   AFL_SLOT_PTRS
@@ -342,9 +345,10 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, Tab T, Srcs src, E3Coef
 // Mass-type apply: dst = scale * M_rho src, M (euler_operator.h:715-740), D (:1384-1408),
 // A (:2328-2357). rho == nullptr means rho = 1.
 template <int DIM, int K>
-__global__ __launch_bounds__(AFL_NT) void k_mass_apply(Geo g, Tab T, const double* __restrict__ rho, double scale,
+__global__ __launch_bounds__(AFL_NT) void k_mass_apply(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho, double scale,
                                                         int ncomp, int64_t N, const double* __restrict__ src,
                                                         double* __restrict__ dst) {
+  const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   AFL_SLOT_PTRS
   LineGeo<DIM, K> lg;
@@ -366,8 +370,9 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_apply(Geo g, Tab T, const doubl
 
 // Inverse diagonal, euler_operator.h:2456-2626: diag_i = sum_q JxW coef phi_i(q)^2.
 template <int DIM, int K>
-__global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, Tab T, const double* __restrict__ rho, double scale,
+__global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho, double scale,
                                                       int ncomp, int64_t N, double* __restrict__ dst) {
+  const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   AFL_SLOT_PTRS
   LineGeo<DIM, K> lg;
@@ -425,7 +430,7 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, Tab T, const double*
 // B: vmult_pressure (euler_operator.h:1510-1617). Volume -f p div v, faces +f {p} n . v
 // (own side), walls +f p n . v, with f = coeff dt / Ma^2. fn: face-node LDS of this cell.
 template <int DIM, int K, typename R = double>
-__device__ inline void apply_B_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
+__device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
                                     const Nbrs<DIM>& nb, const FacePts<DIM, K, R>& fp, const R* __restrict__ p,
                                     R f, R* t1, R* t2, R* i1, R* i2, R* fn, R* Fq, R (&out)[DIM][K + 1]) {
   using C = Cfg<DIM, K>;
@@ -457,8 +462,9 @@ __device__ inline void apply_B_rows(const Geo& g, const TabT<R>& T, const Slot<D
 }
 
 template <int DIM, int K>
-__global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, Tab T, double f, int64_t N, const double* __restrict__ p,
+__global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, const Tab* __restrict__ Tp, double f, int64_t N, const double* __restrict__ p,
                                                      double* __restrict__ dst) {
+  const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F * DIM];
   AFL_SLOT_PTRS
@@ -474,20 +480,32 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, Tab T, double f, int6
 
 // ---------------------------------------------------------------------------------------
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
-// Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
+// LDS per cell of the C apply. FP32 (BATCH): all fields (p_fix, w, and p for the D term) are
+// evaluated in one batched pass, so the work area holds every field's sweep and face nodes.
+template <int DIM, int K, bool WITH_D, bool BATCH>
+struct LdsC {
+  using C = Cfg<DIM, K>;
+  static constexpr int NF = (WITH_D ? 1 : 0) + 1 + DIM;  // p_fix, w_0..w_{d-1}, (p)
+  static constexpr int NFF = 1 + DIM;                    // fields needed on the faces
+  static constexpr int EV = BATCH ? NF * (C::EV1 + C::EV2) : C::EV1 + C::EV2;
+  static constexpr int IN = C::IN1 + C::IN2;
+  static constexpr int W = EV > IN ? EV : IN;
+  static constexpr int WF = W + (BATCH ? NFF : 1) * FaceLds<DIM, K>::FN;
+};
+
 // Result rows of C (optionally fused with D p) for the thread's node row: out = (WITH_D ?
-// scaleD M pD - C w : C w). sh: the cell's work area (Lds::WF), Fq: its face flux block (Lds::F).
+// scaleD M pD - C w : C w). sh: the cell's work area (LdsC::WF), Fq: its face flux block (Lds::F).
 template <int DIM, int K, bool WITH_D, typename R>
-__device__ inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, R* sh, R* Fq, R cdt,
+__device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, R* sh, R* Fq, R cdt,
                                     const R* __restrict__ pfix, const R* __restrict__ w, int64_t N,
                                     const R* __restrict__ pD, R scaleD, R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ;
+  constexpr bool BATCH = sizeof(R) == 4;
+  using LC = LdsC<DIM, K, WITH_D, BATCH>;
   R* t1 = sh;
-  R* t2 = t1 + C::EV1;
   R* i1 = t1;
   R* i2 = t1 + C::IN1;
-  R* fn = t1 + Lds<DIM, K>::W;
   const R gr = R(kGamma / (kGamma - 1.0)), igm1 = R(1.0 / (kGamma - 1.0)), half = R(0.5);
   const R sgn = WITH_D ? R(-1) : R(1);
   LineGeo<DIM, K, R> lg;
@@ -496,18 +514,45 @@ __device__ inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<D
   const FacePts<DIM, K, R> fp(g, T, s);
   R pq[NQ], G[DIM][NQ], V[NQ];
   R trp[2 * DIM][2], trw[DIM][2 * DIM][2];
-  eval_line_faces<DIM, K, R>(pfix, s, nb, t1, t2, fn, T, pq, trp);
+  if constexpr (BATCH) {
+    const R* fl[LC::NF];
+    fl[0] = pfix;
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    R wq[NQ];
-    eval_line_faces<DIM, K, R>(w + d * N, s, nb, t1, t2, fn, T, wq, trw[d]);
+    for (int d = 0; d < DIM; ++d) fl[1 + d] = w + d * N;
+    if (WITH_D) fl[LC::NF - 1] = pD;
+    R v[LC::NF][NQ], tr[LC::NFF][2 * DIM][2];
+    eval_lines_faces<DIM, K, LC::NF, LC::NFF, R>(fl, s, nb, t1, t1 + LC::NF * C::EV1, sh + LC::W, T, v, tr);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) G[d][q] = sgn * (-cdt * gr * pq[q] * wq[q]);
-  }
-  if (WITH_D) {
-    eval_line<DIM, K, R>(pD, s, t1, t2, T, V);
+    for (int q = 0; q < NQ; ++q) {
+      pq[q] = v[0][q];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
+      for (int d = 0; d < DIM; ++d) G[d][q] = sgn * (-cdt * gr * v[0][q] * v[1 + d][q]);
+      if (WITH_D) V[q] = v[LC::NF - 1][q] * scaleD * lg.jxw(g, T, q);
+    }
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f)
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd) {
+        trp[f][sd] = tr[0][f][sd];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) trw[d][f][sd] = tr[1 + d][f][sd];
+      }
+  } else {
+    R* t2 = t1 + C::EV1;
+    R* fn = t1 + LC::W;
+    eval_line_faces<DIM, K, R>(pfix, s, nb, t1, t2, fn, T, pq, trp);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      R wq[NQ];
+      eval_line_faces<DIM, K, R>(w + d * N, s, nb, t1, t2, fn, T, wq, trw[d]);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) G[d][q] = sgn * (-cdt * gr * pq[q] * wq[q]);
+    }
+    if (WITH_D) {
+      eval_line<DIM, K, R>(pD, s, t1, t2, T, V);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
+    }
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -543,15 +588,17 @@ __device__ inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<D
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
 template <int DIM, int K, bool WITH_D, typename R = double>
-__global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+__global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
                                                      const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
                                                      R* __restrict__ dst) {
-  __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  const auto& T = *Tp;
+  constexpr int WFC = LdsC<DIM, K, WITH_D, sizeof(R) == 4>::WF;
+  __shared__ R shw[Cfg<DIM, K>::CPB * WFC];
   __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   Slot<DIM, K> s(g);
   const int cbs = s.cb < Cfg<DIM, K>::CPB ? s.cb : 0;
   R out[K + 1];
-  apply_C_rows<DIM, K, WITH_D, R>(g, T, s, shw + cbs * Lds<DIM, K>::WF, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w,
+  apply_C_rows<DIM, K, WITH_D, R>(g, T, s, shw + cbs * WFC, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w,
                                   N, pD, scaleD, out);
   store_row<DIM, K, R>(dst, s, out);
 }
@@ -562,17 +609,19 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, TabT<R> T, R cdt, int
 // The block preconditioner is cell-local (LDS transposes of the cell's rows); pD, d and x are
 // read only inside the thread's own cell, so d and x (and pD == d or x) update in place.
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, TabT<R> T, R cdt, int64_t N, const R* __restrict__ pfix,
+__global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
                                                           const R* __restrict__ w, const R* pD, R scaleD,
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
-  __shared__ R shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  const auto& T = *Tp;
+  constexpr int WFC = LdsC<DIM, K, true, sizeof(R) == 4>::WF;
+  __shared__ R shw[Cfg<DIM, K>::CPB * WFC];
   __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
   Slot<DIM, K> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
-  R* sh = shw + cbs * Lds<DIM, K>::WF;
+  R* sh = shw + cbs * WFC;
   R t[N1];
   apply_C_rows<DIM, K, true, R>(g, T, s, sh, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w, N, pD, scaleD, t);
   const bool ro = s.act && s.l < ROWS;
@@ -680,7 +729,7 @@ struct SolvLds {
 
 // In-place inverse of a small SPD matrix (Gauss-Jordan without pivoting, fully unrolled).
 template <int N, typename R>
-__device__ inline void inv_spd(R (&A)[N][N]) {
+__device__ __attribute__((always_inline)) inline void inv_spd(R (&A)[N][N]) {
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const R p = R(1) / A[k][k];
@@ -697,7 +746,7 @@ __device__ inline void inv_spd(R (&A)[N][N]) {
 }
 
 template <int DIM, int K, int NC, typename R = double, bool SEP = true>
-__device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
+__device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
                                  const R (&co)[2 * K + 1], R* sl /* SolvLds::size(NC) per cell */,
                                  R tol, int maxit, const R (&b)[NC][K + 1], R (&x)[NC][K + 1]) {
   using C = Cfg<DIM, K>;
@@ -959,9 +1008,10 @@ __device__ inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const 
 }
 
 template <int DIM, int K, int NC>
-__global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const double* __restrict__ rho, double scale,
+__global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho, double scale,
                                                         int64_t N, const double* __restrict__ b,
                                                         double* __restrict__ x, double tol, int maxit) {
+  const auto& T = *Tp;
   constexpr int PC = Lds<DIM, K>::W > SolvLds<DIM, K>::size(NC) ? Lds<DIM, K>::W : SolvLds<DIM, K>::size(NC);
   __shared__ double shw[Cfg<DIM, K>::CPB * PC];
   using C = Cfg<DIM, K>;
@@ -989,8 +1039,9 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, Tab T, const doubl
 
 // Schur first half: w = A^{-1} B p (fused B apply + cell-local A solve).
 template <int DIM, int K, typename R = double>
-__global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, TabT<R> T, const R* __restrict__ rho, R f, int64_t N,
+__global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ rho, R f, int64_t N,
                                                      const R* __restrict__ p, R* __restrict__ w, R tol, int maxit) {
+  const auto& T = *Tp;
   constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;  // B phase: work + face nodes + face fluxes
   constexpr int PS = SolvLds<DIM, K>::size(DIM);             // solver phase
   constexpr int PC = PB > PS ? PB : PS;
@@ -1027,7 +1078,7 @@ __global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, TabT<R> T, const R* _
 // rows (held by the row owners) <- (MI x MI [x MI]) rows for NC components: the inverse of the
 // reference-cell mass matrix, one LDS transpose per further axis. tmp: 2*NC*NPC values per cell.
 template <int DIM, int K, int NC, typename R>
-__device__ inline void cell_mi_apply(const Slot<DIM, K>& s, const TabT<R>& T, R* tmp, R (&rows)[NC][K + 1]) {
+__device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<DIM, K>& s, const TabT<R>& T, R* tmp, R (&rows)[NC][K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
   R* t2 = tmp + NC * NPC;
@@ -1112,8 +1163,9 @@ __device__ inline void cell_mi_apply(const Slot<DIM, K>& s, const TabT<R>& T, R*
 // JxW of cell c, refreshed with the stage coefficients). P^{-1} A has its spectrum within the
 // cell's relative variation of rho J about the mean, a few percent.
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) void k_schur_w_lvl(Geo g, TabT<R> T, const R* __restrict__ sinv, R f, int64_t N,
+__global__ __launch_bounds__(AFL_NT) void k_schur_w_lvl(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ sinv, R f, int64_t N,
                                                          const R* __restrict__ p, R* __restrict__ w) {
+  const auto& T = *Tp;
   using C = Cfg<DIM, K>;
   constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;
   constexpr int PM = 2 * DIM * C::NPC;
@@ -1143,8 +1195,9 @@ __global__ __launch_bounds__(AFL_NT) void k_schur_w_lvl(Geo g, TabT<R> T, const 
 
 // sinv[c] = 1 / sum_q rho(q) JxW(q) of cell c (one thread per cell)
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, TabT<R> T, const R* __restrict__ rho,
+__global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ rho,
                                                                R* __restrict__ sinv) {
+  const auto& T = *Tp;
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ, N1 = C::N1, NPC = C::NPC;
   for (int64_t c = blockIdx.x * (int64_t)AFL_NT + threadIdx.x; c < g.ncells; c += (int64_t)gridDim.x * AFL_NT) {
@@ -1212,11 +1265,11 @@ void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double*
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
     if (stage == 1)
-      k_rhs_rho<D_, K_, 1><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, o.dt, t.a21, 0.0, 0.0, o.N, dst);
+      k_rhs_rho<D_, K_, 1><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a21, 0.0, 0.0, o.N, dst);
     else if (stage == 2)
-      k_rhs_rho<D_, K_, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, o.dt, t.a31, t.a32, 0.0, o.N, dst);
+      k_rhs_rho<D_, K_, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a31, t.a32, 0.0, o.N, dst);
     else
-      k_rhs_rho<D_, K_, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, o.dt, t.b1, t.b2, t.b3, o.N, dst);
+      k_rhs_rho<D_, K_, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.b1, t.b2, t.b3, o.N, dst);
   });
 }
 
@@ -1238,11 +1291,11 @@ void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* d
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
     if (stage == 1)
-      k_rhs_u<D_, K_, 1, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      k_rhs_u<D_, K_, 1, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
     else if (stage == 2)
-      k_rhs_u<D_, K_, 2, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      k_rhs_u<D_, K_, 2, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
     else
-      k_rhs_u<D_, K_, 3, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      k_rhs_u<D_, K_, 3, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
   });
 }
 
@@ -1266,23 +1319,23 @@ void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* d
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
     if (stage == 1)
-      k_rhs_p<D_, K_, 1, 2, 1><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      k_rhs_p<D_, K_, 1, 2, 1><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else if (stage == 2)
-      k_rhs_p<D_, K_, 2, 3, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      k_rhs_p<D_, K_, 2, 3, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else
-      k_rhs_p<D_, K_, 3, 3, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      k_rhs_p<D_, K_, 3, 3, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
   });
 }
 
 void launch_mass_apply(const OpCtx& o, const double* rho, double scale, int ncomp, const double* src, double* dst) {
   AFL_DISPATCH(o.dim, o.k, {
-    k_mass_apply<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, scale, ncomp, o.N, src, dst);
+    k_mass_apply<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, ncomp, o.N, src, dst);
   });
 }
 
 void launch_inv_diag(const OpCtx& o, const double* rho, double scale, int ncomp, double* dst) {
   AFL_DISPATCH(o.dim, o.k, {
-    k_inv_diag<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, scale, ncomp, o.N, dst);
+    k_inv_diag<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, ncomp, o.N, dst);
   });
 }
 
@@ -1290,7 +1343,7 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst) {
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   AFL_DISPATCH(o.dim, o.k, {
-    k_apply_B<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, f, o.N, p, dst);
+    k_apply_B<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, f, o.N, p, dst);
   });
 }
 
@@ -1299,9 +1352,9 @@ void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const d
   const double cdt = coeff * o.dt;
   AFL_DISPATCH(o.dim, o.k, {
     if (pD)
-      k_apply_C<D_, K_, true, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
+      k_apply_C<D_, K_, true, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
     else
-      k_apply_C<D_, K_, false, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, cdt, o.N, pfix, w, nullptr, 0.0, dst);
+      k_apply_C<D_, K_, false, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, nullptr, 0.0, dst);
   });
 }
 
@@ -1309,9 +1362,9 @@ void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncom
   const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
     if (ncomp == 1)
-      k_mass_solve<D_, K_, 1><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, scale, o.N, b, x, tol2, o.cell_maxit);
+      k_mass_solve<D_, K_, 1><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
     else
-      k_mass_solve<D_, K_, D_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, scale, o.N, b, x, tol2, o.cell_maxit);
+      k_mass_solve<D_, K_, D_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
   });
 }
 
@@ -1320,13 +1373,13 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
-    k_schur_w<D_, K_, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, rho, f, o.N, p, w, tol2, o.cell_maxit);
+    k_schur_w<D_, K_, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, f, o.N, p, w, tol2, o.cell_maxit);
   });
 }
 
 // Level Schur apply y = D p - C A^{-1} B p in precision R (multigrid levels).
 template <typename R>
-void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
   AFL_DISPATCH(dim, k, {
     const dim3 gr = grid_for<D_, K_>(geo.ncells);
@@ -1337,7 +1390,7 @@ void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>& T, hipStre
 }
 // w = A^{-1} B p alone in precision R (the first half of the level operator)
 template <typename R>
-void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                       R f, R tol, int maxit, int64_t N, const R* p, R* w) {
   AFL_DISPATCH(dim, k, {
     if (sinv)
@@ -1347,23 +1400,23 @@ void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>& T, hipStrea
   });
 }
 template <typename R>
-void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho,
+void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
                               R* sinv) {
   const int nb = (int)((geo.ncells + AFL_NT - 1) / AFL_NT);
   AFL_DISPATCH(dim, k, { k_cell_rho_mass_inv<D_, K_, R><<<dim3(nb), dim3(AFL_NT), 0, st>>>(geo, T, rho, sinv); });
 }
-template void launch_cell_rho_mass_inv<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+template void launch_cell_rho_mass_inv<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
                                               float*);
-template void launch_cell_rho_mass_inv<double>(int, int, const Geo&, const TabT<double>&, hipStream_t,
+template void launch_cell_rho_mass_inv<double>(int, int, const Geo&, const TabT<double>*, hipStream_t,
                                                const double*, double*);
-template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
                                       const float*, float, float, int, int64_t, const float*, float*);
-template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
+template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
                                        const double*, double, double, int, int64_t, const double*, double*);
 
 // w = A^{-1} B p followed by the fused C/D apply and Chebyshev step (see k_apply_C_cheb)
 template <typename R>
-void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStream_t st, const R* rho, const R* sinv,
+void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
                        const R* rin, R* rout, R c1, R c2, R* d, R* x) {
   AFL_DISPATCH(dim, k, {
@@ -1374,16 +1427,16 @@ void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>& T, hipStre
                                                            rin, rout, c1, c2, d, x);
   });
 }
-template void launch_schur_cheb<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+template void launch_schur_cheb<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
                                        const float*, const float*, float, float, float, int, int64_t, const float*, float*,
                                        const float*, const float*, float*, float, float, float*, float*);
-template void launch_schur_cheb<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
+template void launch_schur_cheb<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
                                         const double*, const double*, double, double, double, int, int64_t, const double*, double*,
                                         const double*, const double*, double*, double, double, double*, double*);
-template void launch_schur_pair<float>(int, int, const Geo&, const TabT<float>&, hipStream_t, const float*,
+template void launch_schur_pair<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
                                        const float*, const float*, float, float, float, int, int64_t, const float*, float*,
                                        float*);
-template void launch_schur_pair<double>(int, int, const Geo&, const TabT<double>&, hipStream_t, const double*,
+template void launch_schur_pair<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
                                         const double*, const double*, double, double, double, int, int64_t,
                                         const double*, double*, double*);
 

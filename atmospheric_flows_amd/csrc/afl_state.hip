@@ -38,8 +38,9 @@ __device__ inline void node_xyz(const Geo& g, const Tab& T, int n1, int64_t c, i
 }
 
 template <int DIM>
-__global__ void k_init_state(Geo g, Tab T, int n1, int npc, int64_t N, HillParams hp, double* __restrict__ rho,
+__global__ void k_init_state(Geo g, const Tab* __restrict__ Tp, int n1, int npc, int64_t N, HillParams hp, double* __restrict__ rho,
                              double* __restrict__ u, double* __restrict__ p, double* __restrict__ th_out, bool noise) {
+  const auto& T = *Tp;
   const double cp = kGamma * kR / (kGamma - 1.0);
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < N; idx += (int64_t)gridDim.x * blockDim.x) {
     double X[3];
@@ -65,8 +66,9 @@ __global__ void k_init_state(Geo g, Tab T, int n1, int npc, int64_t N, HillParam
 }
 
 template <int DIM>
-__global__ void k_sponge(Geo g, Tab T, int n1, int npc, int64_t N, double zs, double adt, double u0, double u1,
+__global__ void k_sponge(Geo g, const Tab* __restrict__ Tp, int n1, int npc, int64_t N, double zs, double adt, double u0, double u1,
                          double u2, double* __restrict__ u) {
+  const auto& T = *Tp;
   const double ur[3] = {u0, u1, u2};
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < N; idx += (int64_t)gridDim.x * blockDim.x) {
     double X[3];
@@ -87,25 +89,25 @@ static int grid_nodes(int64_t n) {
 void launch_init_state(const OpCtx& o, const HillParams& hp, double* rho, double* u, double* p) {
   const int n1 = o.k + 1, npc = o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
   if (o.dim == 2)
-    k_init_state<2><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, n1, npc, o.N, hp, rho, u, p, nullptr, true);
+    k_init_state<2><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, n1, npc, o.N, hp, rho, u, p, nullptr, true);
   else
-    k_init_state<3><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, n1, npc, o.N, hp, rho, u, p, nullptr, true);
+    k_init_state<3><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, n1, npc, o.N, hp, rho, u, p, nullptr, true);
 }
 
 void launch_background(const OpCtx& o, const HillParams& hp, double* rho, double* p, double* th) {
   const int n1 = o.k + 1, npc = o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
   if (o.dim == 2)
-    k_init_state<2><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, n1, npc, o.N, hp, rho, nullptr, p, th, false);
+    k_init_state<2><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, n1, npc, o.N, hp, rho, nullptr, p, th, false);
   else
-    k_init_state<3><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, n1, npc, o.N, hp, rho, nullptr, p, th, false);
+    k_init_state<3><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, n1, npc, o.N, hp, rho, nullptr, p, th, false);
 }
 
 void launch_sponge(const OpCtx& o, double z_start, double alpha_dt, const double* uref, double* u) {
   const int n1 = o.k + 1, npc = o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
   if (o.dim == 2)
-    k_sponge<2><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, n1, npc, o.N, z_start, alpha_dt, uref[0], uref[1], uref[2], u);
+    k_sponge<2><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, n1, npc, o.N, z_start, alpha_dt, uref[0], uref[1], uref[2], u);
   else
-    k_sponge<3><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.T, n1, npc, o.N, z_start, alpha_dt, uref[0], uref[1], uref[2], u);
+    k_sponge<3><<<dim3(grid_nodes(o.N)), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, n1, npc, o.N, z_start, alpha_dt, uref[0], uref[1], uref[2], u);
 }
 
 }  // namespace afl

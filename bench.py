@@ -186,9 +186,11 @@ def main():
            for name, a, b in (("rho", s32[0], s64[0]), ("u", s32[1], s64[1]), ("p", s32[2], s64[2]))}
     dofs = (2 + dim) * N
 
-    # roofline of the dominant kernel of the FP32-preconditioned step, the FP32 w = A^{-1} B p:
-    # algorithmic bytes = (p, rho_fix read; dim velocity components written) * N * 4
-    bytes_sw32 = (2 + dim) * N * 4
+    # roofline of a dominant kernel of the FP32-preconditioned step, the level operator's first
+    # half w = P^{-1} B p (k_schur_w_lvl<float>): algorithmic bytes = p read and dim velocity
+    # components written, N * 4 each, plus one 4-byte density scale per cell
+    ncells = N // (args.degree + 1) ** dim
+    bytes_sw32 = (1 + dim) * N * 4 + ncells * 4
     ach = bytes_sw32 / t_sw32 / 1e9
     field_bytes = N * 8
     res = {
@@ -222,9 +224,10 @@ def main():
                                  "E3_stage3": 16 * field_bytes / t_e3 / 1e9,
                                  "schur_vmult": 9 * field_bytes / t_schur / 1e9,
                                  "schur_velocity_kernel": (2 + dim) * field_bytes / t_sw / 1e9},
-        "roofline": {"kernel": "k_schur_w<float> (FP32 w = A^-1 B p)", "bound": "hbm", "achieved": ach,
+        "roofline": {"kernel": "k_schur_w_lvl<float> (FP32 w = P^-1 B p, multigrid level operator)",
+                     "bound": "hbm", "achieved": ach, "bytes_per_launch": bytes_sw32,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("k_schur_w<2, 4, float>", args)},
+                     "traffic": profiled_traffic("k_schur_w_lvl<2, 4, float>", args)},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
