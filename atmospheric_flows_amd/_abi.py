@@ -26,6 +26,7 @@ class SolverParams(C.Structure):
 
 
 AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32, AFL_SCHUR_MG_FP64 = 0, 1, 2
+AFL_TIME_LEVEL_W, AFL_TIME_LEVEL_C_CHEB = 1, 2
 
 
 class StepInfo(C.Structure):

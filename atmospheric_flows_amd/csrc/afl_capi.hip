@@ -191,6 +191,7 @@ struct afl_operator {
   DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
   double* host_sc = nullptr;  // pinned scalars
   Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
+  KernelTimer timer;          // per-kernel HIP-event timing (afl_kernel_timer_*)
   ~afl_operator() {
     DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
@@ -198,6 +199,7 @@ struct afl_operator {
     for (DevBuf* b : all) b->release();
     if (host_sc) hipHostFree(host_sc);
     if (dtab) hipFree(dtab);
+    for (hipEvent_t e : timer.ev) hipEventDestroy(e);
   }
   int64_t N() const { return o.N; }
   int dim() const { return o.dim; }
@@ -766,6 +768,43 @@ int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {
   if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
   op->mg32.schur_velocity(op->o, p, w);
   return check_device("vmult_schur_velocity_f32");
+}
+
+int afl_kernel_timer_start(afl_operator* op, int kernel, int max_launches) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  if (kernel != AFL_TIME_LEVEL_W && kernel != AFL_TIME_LEVEL_C_CHEB)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "unknown timed kernel");
+  if (max_launches < 1 || max_launches > (1 << 20)) return fail(AFL_ERR_INVALID_ARGUMENT, "max_launches out of range");
+  KernelTimer& t = op->timer;
+  while ((int)t.ev.size() < 2 * max_launches) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipEventCreate failed");
+    t.ev.push_back(e);
+  }
+  t.which = kernel;
+  t.used = 0;
+  t.dropped = 0;
+  op->mg32.timer = &t;
+  op->mg64.timer = &t;
+  return AFL_OK;
+}
+
+int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms) {
+  if (!op || !launches || !total_ms) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  KernelTimer& t = op->timer;
+  t.which = 0;
+  if (hipStreamSynchronize(op->o.stream) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
+  double sum = 0.0;
+  for (int i = 0; i < t.used; ++i) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]) != hipSuccess)
+      return fail(AFL_ERR_DEVICE, "hipEventElapsedTime failed");
+    sum += ms;
+  }
+  *launches = t.used;
+  *total_ms = sum;
+  if (t.dropped) return fail(AFL_ERR_STATE, "kernel timer: more launches than max_launches");
+  return AFL_OK;
 }
 
 int afl_compute_diagnostics(afl_operator* op, const double* rho, const double* u, const double* p,

@@ -1,6 +1,10 @@
 // Host/device shared internals of libafl_hip.so (not part of the C-ABI).
 #pragma once
 
+#include <hip/hip_ext.h>
+
+#include <vector>
+
 #include "afl_common.h"
 
 namespace afl {
@@ -11,6 +15,23 @@ struct TableauC {
   double at21, at22, at31, at32, at33;
   double b1, b2, b3;
 };
+
+// HIP-event timing of every launch of one kernel (benchmark instrumentation). The launch itself
+// records the pair (hipExtLaunchKernelGGL start/stop events), so the events bracket exactly the
+// kernel's execution.
+struct KernelTimer {
+  int which = 0;  // 0 off; a KernelId below
+  int used = 0;
+  int dropped = 0;
+  std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  // the event pair for the next launch of kernel id, or nullptr when not timing it
+  hipEvent_t* next(int id) {
+    if (which != id) return nullptr;
+    if (2 * used + 1 >= (int)ev.size()) { ++dropped; return nullptr; }
+    return &ev[2 * used++];
+  }
+};
+enum KernelId { kTimeLevelW = 1, kTimeLevelCCheb = 2 };
 
 // Everything a launcher needs; a plain copy of the operator state.
 struct OpCtx {
@@ -45,11 +66,15 @@ void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStre
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
 template <typename R>
 void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                      R f, R tol, int maxit, int64_t N, const R* p, R* w);
+                      R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev = nullptr);
 template <typename R>
 void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
                        const R* rin, R* rout, R c1, R c2, R* d, R* x);
+template <typename R>
+void launch_apply_C_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* pfix, R cdt,
+                         int64_t N, const R* w, const R* p, const R* jbar, const R* rin, R* rout, R c1, R c2, R* d,
+                         R* x, hipEvent_t* ev = nullptr);
 template <typename R>
 void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
                               R* sinv);

@@ -46,6 +46,7 @@ struct MgT {
   std::vector<MgLevelT<R>> levels;
   int smoother_degree = 3, coarse_degree = 8;
   int warm_eig_iters = 4;  // power iterations when restarting from the previous eigenvector
+  KernelTimer* timer = nullptr;  // optional launch timing of the level kernels
   double smoothing_range = 15.0;
   int setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k);
   void release();

@@ -332,18 +332,19 @@ static void cheb_step(const MgLevelT<R>& L, bool first, const R* rin, const R* t
 // S p followed by one fused Chebyshev step: r' = rin - S p -> rout, d = c1 d + c2 P^{-1} r', x += d
 template <typename R>
 static void level_schur_cheb(const MgLevelT<R>& L, const OpCtx& fine, const R* p, const R* rin, R* rout, double c1,
-                             double c2, R* d, R* x) {
+                             double c2, R* d, R* x, KernelTimer* tm) {
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
-  launch_schur_cheb<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt,
-                       (R)L.cell_tol, L.cell_maxit, L.N, p,
-                       L.w, L.jbar, rin, rout, (R)c1, (R)c2, d, x);
+  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, f, (R)L.cell_tol,
+                      L.cell_maxit, L.N, p, L.w, tm && L.lean() ? tm->next(kTimeLevelW) : nullptr);
+  launch_apply_C_cheb<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.pfix, cdt, L.N, L.w, p, L.jbar, rin, rout, (R)c1, (R)c2,
+                         d, x, tm ? tm->next(kTimeLevelCCheb) : nullptr);
 }
 
 template <typename R>
 static void smooth(const MgLevelT<R>& L, const OpCtx& fine, const R* b, R* x, bool zero_init, int degree,
-                   double lo, double hi) {
+                   double lo, double hi, KernelTimer* tm) {
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma1 = theta / delta;
   const R* rcur = b;
   R* rnext = L.r;
@@ -351,14 +352,14 @@ static void smooth(const MgLevelT<R>& L, const OpCtx& fine, const R* b, R* x, bo
   if (zero_init) {
     cheb_step(L, true, b, (const R*)nullptr, (R*)nullptr, 0.0, 1.0 / theta, L.d, x);
   } else {
-    level_schur_cheb(L, fine, x, b, rnext, 0.0, 1.0 / theta, L.d, x);
+    level_schur_cheb(L, fine, x, b, rnext, 0.0, 1.0 / theta, L.d, x, tm);
     rcur = rnext;
     std::swap(rnext, ralt);
   }
   double rho = 1.0 / sigma1;
   for (int k = 1; k < degree; ++k) {
     const double rn = 1.0 / (2.0 * sigma1 - rho);
-    level_schur_cheb(L, fine, L.d, rcur, rnext, rn * rho, 2.0 * rn / delta, L.d, x);
+    level_schur_cheb(L, fine, L.d, rcur, rnext, rn * rho, 2.0 * rn / delta, L.d, x, tm);
     rcur = rnext;
     std::swap(rnext, ralt);
     rho = rn;
@@ -384,18 +385,18 @@ static void vcycle(MgT<R>& mg, const OpCtx& fine, int l) {
   MgLevelT<R>& L = mg.levels[l];
   const int nl = (int)mg.levels.size();
   if (l == nl - 1) {
-    smooth(L, fine, L.b, L.x, true, mg.coarse_degree, 0.9, 1.2 * L.lam_max);
+    smooth(L, fine, L.b, L.x, true, mg.coarse_degree, 0.9, 1.2 * L.lam_max, mg.timer);
     return;
   }
   const double hi = 1.2 * L.lam_max, lo = L.lam_max / mg.smoothing_range;
-  smooth(L, fine, L.b, L.x, true, mg.smoother_degree, lo, hi);
+  smooth(L, fine, L.b, L.x, true, mg.smoother_degree, lo, hi, mg.timer);
   // residual b - S x, restricted to the coarser level (subtraction fused into the restriction)
   level_schur(L, fine, L.x, L.t);
   MgLevelT<R>& Cl = mg.levels[l + 1];
   tensor_map<R>(L.dim, Cl.k, L.k, false, L.ncells, L.N, Cl.N, Cl.Pt, L.b, Cl.b, L.stream, L.t);
   vcycle(mg, fine, l + 1);
   tensor_map<R>(L.dim, L.k, Cl.k, true, L.ncells, Cl.N, L.N, Cl.P, Cl.x, L.x, L.stream);
-  smooth(L, fine, L.b, L.x, false, mg.smoother_degree, lo, hi);
+  smooth(L, fine, L.b, L.x, false, mg.smoother_degree, lo, hi, mg.timer);
 }
 
 // host 1D tables of a level (same node family as the fine level)

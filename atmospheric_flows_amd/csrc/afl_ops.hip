@@ -1391,9 +1391,12 @@ void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStre
 // w = A^{-1} B p alone in precision R (the first half of the level operator)
 template <typename R>
 void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                      R f, R tol, int maxit, int64_t N, const R* p, R* w) {
+                      R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev) {
   AFL_DISPATCH(dim, k, {
-    if (sinv)
+    if (sinv && ev)
+      hipExtLaunchKernelGGL(k_schur_w_lvl<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st, ev[0], ev[1],
+                            0, geo, T, sinv, f, N, p, w);
+    else if (sinv)
       k_schur_w_lvl<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
     else
       k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
@@ -1410,9 +1413,10 @@ template void launch_cell_rho_mass_inv<float>(int, int, const Geo&, const TabT<f
 template void launch_cell_rho_mass_inv<double>(int, int, const Geo&, const TabT<double>*, hipStream_t,
                                                const double*, double*);
 template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
-                                      const float*, float, float, int, int64_t, const float*, float*);
+                                      const float*, float, float, int, int64_t, const float*, float*, hipEvent_t*);
 template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
-                                       const double*, double, double, int, int64_t, const double*, double*);
+                                       const double*, double, double, int, int64_t, const double*, double*,
+                                       hipEvent_t*);
 
 // w = A^{-1} B p followed by the fused C/D apply and Chebyshev step (see k_apply_C_cheb)
 template <typename R>
@@ -1427,6 +1431,26 @@ void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStre
                                                            rin, rout, c1, c2, d, x);
   });
 }
+template <typename R>
+void launch_apply_C_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* pfix, R cdt,
+                         int64_t N, const R* w, const R* p, const R* jbar, const R* rin, R* rout, R c1, R c2, R* d,
+                         R* x, hipEvent_t* ev) {
+  const R sD = (R)(1.0 / (kGamma - 1.0));
+  AFL_DISPATCH(dim, k, {
+    if (ev)
+      hipExtLaunchKernelGGL(k_apply_C_cheb<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st, ev[0], ev[1],
+                            0, geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
+    else
+      k_apply_C_cheb<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(
+          geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
+  });
+}
+template void launch_apply_C_cheb<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*, float,
+                                         int64_t, const float*, const float*, const float*, const float*, float*,
+                                         float, float, float*, float*, hipEvent_t*);
+template void launch_apply_C_cheb<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
+                                          double, int64_t, const double*, const double*, const double*,
+                                          const double*, double*, double, double, double*, double*, hipEvent_t*);
 template void launch_schur_cheb<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
                                        const float*, const float*, float, float, float, int, int64_t, const float*, float*,
                                        const float*, const float*, float*, float, float, float*, float*);

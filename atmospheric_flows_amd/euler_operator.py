@@ -59,6 +59,8 @@ def load_library():
     L.afl_imex_step.argtypes = [vp, dp, dp, dp, C.POINTER(StepInfo)]
     L.afl_init_hill_state.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
     L.afl_background.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
+    L.afl_kernel_timer_start.argtypes = [vp, C.c_int, C.c_int]
+    L.afl_kernel_timer_stop.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.afl_compute_diagnostics.argtypes = [vp, dp, dp, dp, C.POINTER(Diagnostics)]
     L.afl_l2_difference.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     for n in ("afl_pack_dofs", "afl_unpack_dofs"):
@@ -160,6 +162,17 @@ class EulerOperator:
         """FP32 fine-level w = A^-1 B p of the FP32 V-cycle (float32 tensors)."""
         self._chk(self.L.afl_vmult_schur_velocity_f32(self.h, _ptr(w, 4), _ptr(p, 4)))
         return w
+
+    # ---- benchmark instrumentation -------------------------------------------------------------
+    def kernel_timer_start(self, kernel, max_launches=100000):
+        """HIP events around every launch of one multigrid level kernel (AFL_TIME_* constants)."""
+        self._chk(self.L.afl_kernel_timer_start(self.h, kernel, max_launches))
+
+    def kernel_timer_stop(self):
+        """(launches, total milliseconds) since kernel_timer_start."""
+        n, ms = C.c_int(), C.c_double()
+        self._chk(self.L.afl_kernel_timer_stop(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     # ---- diagnostics (Atmospheric_Flow.cc:925-1087, 1225-1235) ---------------------------------
     def diagnostics(self, rho, u=None, p=None):

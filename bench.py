@@ -154,7 +154,7 @@ def main():
     # the FP32 level operator of the preconditioner: w = A^-1 B p in single precision (the kernel
     # that dominates the FP32-preconditioned step) and one full V-cycle
     from atmospheric_flows_amd import make_solver_params
-    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32
+    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32, AFL_TIME_LEVEL_W, AFL_TIME_LEVEL_C_CHEB
     op.set_solver_params(make_solver_params(gmres_eps=1e-10, gmres_restart=28, fp_tol=1e-10,
                                             schur_precond=AFL_SCHUR_MG_FP32))
     p32 = p.float()
@@ -178,20 +178,31 @@ def main():
         for _ in range(args.steps):
             infos.append(o.imex_step(r0, u0, p0))
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / args.steps, infos, (r0, u0, p0)
+        return (time.perf_counter() - t0) / args.steps, infos, (r0, u0, p0), o
 
-    step_fp64, infos64, s64 = run_path(AFL_SCHUR_JACOBI)
-    step_s, infos, s32 = run_path(AFL_SCHUR_MG_FP32)
+    step_fp64, infos64, s64, _ = run_path(AFL_SCHUR_JACOBI)
+    step_s, infos, s32, o32 = run_path(AFL_SCHUR_MG_FP32)
     dev = {name: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
            for name, a, b in (("rho", s32[0], s64[0]), ("u", s32[1], s64[1]), ("p", s32[2], s64[2]))}
     dofs = (2 + dim) * N
 
-    # roofline of a dominant kernel of the FP32-preconditioned step, the level operator's first
-    # half w = P^{-1} B p (k_schur_w_lvl<float>): algorithmic bytes = p read and dim velocity
-    # components written, N * 4 each, plus one 4-byte density scale per cell
+    # per-kernel time of the two multigrid level kernels inside the same FP32 steps (HIP events
+    # recorded by the library on its stream around every launch; a second pass of args.steps
+    # steps continuing from the timed state)
     ncells = N // (args.degree + 1) ** dim
-    bytes_sw32 = (1 + dim) * N * 4 + ncells * 4
-    ach = bytes_sw32 / t_sw32 / 1e9
+    level = {}
+    for kid, name, fields in ((AFL_TIME_LEVEL_W, "k_schur_w_lvl<float>", 1 + dim),
+                              (AFL_TIME_LEVEL_C_CHEB, "k_apply_C_cheb<float>", 7 + dim)):
+        o32.kernel_timer_start(kid)
+        for _ in range(args.steps):
+            o32.imex_step(*s32)
+        n, ms = o32.kernel_timer_stop()
+        # algorithmic bytes: each distinct field once (FP32), plus one 4-byte scale per cell
+        level[name] = {"launches_per_step": n / args.steps, "avg_us": 1e3 * ms / max(n, 1),
+                       "ms_per_step": ms / args.steps, "bytes_per_launch": fields * N * 4 + ncells * 4}
+    dom = max(level, key=lambda k: level[k]["ms_per_step"])
+    bytes_dom = level[dom]["bytes_per_launch"]
+    ach = bytes_dom / (level[dom]["avg_us"] * 1e-6) / 1e9
     field_bytes = N * 8
     res = {
         "metric": "IMEX step wall-time, FP32-MG vs FP64 path at equal residual; DoF/s, % HBM peak",
@@ -224,10 +235,11 @@ def main():
                                  "E3_stage3": 16 * field_bytes / t_e3 / 1e9,
                                  "schur_vmult": 9 * field_bytes / t_schur / 1e9,
                                  "schur_velocity_kernel": (2 + dim) * field_bytes / t_sw / 1e9},
-        "roofline": {"kernel": "k_schur_w_lvl<float> (FP32 w = P^-1 B p, multigrid level operator)",
-                     "bound": "hbm", "achieved": ach, "bytes_per_launch": bytes_sw32,
+        "level_kernels_in_step": level,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": ach, "bytes_per_launch": bytes_dom,
+                     "avg_launch_us": level[dom]["avg_us"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("k_schur_w_lvl<2, 4, float>", args)},
+                     "traffic": profiled_traffic(dom.replace("<float>", f"<{dim}, {args.degree}, float>"), args)},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -94,6 +94,13 @@ int afl_mg_vmult(afl_operator* op, double* dst, const double* src);
  * FP32 copies of rho_for_fixed (requires schur_precond = AFL_SCHUR_MG_FP32). */
 int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p);
 
+/* Benchmark instrumentation, no counterpart in the reference (deal.II's TimerOutput times whole
+ * phases, not kernels): HIP events recorded on the operator's stream around every launch of one
+ * multigrid level kernel (enum afl_timed_kernel) until afl_kernel_timer_stop, at most max_launches
+ * launches. Stop synchronises the stream and returns the launch count and summed duration. */
+int afl_kernel_timer_start(afl_operator* op, int kernel, int max_launches);
+int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms);
+
 /* ---- diagnostics ------------------------------------------------------------------------ */
 /* Density bounds, maximum celerity and speed of a state and the two Courant numbers
  * (Atmospheric_Flow.cc:992-1087 get_max_velocity / get_min_density / get_max_density /

@@ -90,6 +90,12 @@ typedef struct afl_sponge {
   int32_t pad_;
 } afl_sponge;
 
+/* Kernels that afl_kernel_timer_start can time (benchmark instrumentation). */
+enum afl_timed_kernel {
+  AFL_TIME_LEVEL_W = 1,      /* multigrid level operator, first half: w = P^-1 B p */
+  AFL_TIME_LEVEL_C_CHEB = 2  /* level C/D apply fused with one Chebyshev smoothing step */
+};
+
 /* Diagnostics of a state (EulerSolver::get_min_density / get_max_density /
  * compute_max_celerity / get_max_velocity and the Courant numbers printed in run,
  * Atmospheric_Flow.cc:992-1087, 1225-1235). */

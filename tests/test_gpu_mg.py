@@ -145,3 +145,23 @@ def test_schur_velocity_f32_matches_fp64(dim, k, nc):
     say(f"  mean-density block: rel L2 vs exact {err:.2e}")
     hz = 25e3 / nc[2]
     assert 1e-7 < err < 1.5 * (np.exp(hz / 8.5e3) - 1.0)
+
+
+def test_kernel_timer_counts_level_launches():
+    """One V-cycle with one level and Chebyshev degree 16 launches each level kernel 15 times
+    (the first smoothing step needs no operator apply)."""
+    say("[gpu mg] kernel timer")
+    from atmospheric_flows_amd._abi import AFL_TIME_LEVEL_W, AFL_TIME_LEVEL_C_CHEB
+    gpu, p, v = _stage_operator(2, 4, (24, 1, 10), AFL_SCHUR_MG_FP32, mg_coarse_degree=16)
+    z = gpu.new_scalar()
+    gpu.mg_vmult(z, v)  # set-up and eigenvalue estimate outside the timed launches
+    for kid in (AFL_TIME_LEVEL_W, AFL_TIME_LEVEL_C_CHEB):
+        gpu.kernel_timer_start(kid, 64)
+        gpu.mg_vmult(z, v)
+        gpu.mg_vmult(z, v)
+        n, ms = gpu.kernel_timer_stop()
+        say(f"  kernel {kid}: {n} launches, {ms:.3f} ms")
+        assert n == 30 and ms > 0.0
+    gpu.kernel_timer_start(AFL_TIME_LEVEL_W, 8)
+    n, ms = gpu.kernel_timer_stop()
+    assert n == 0 and ms == 0.0
