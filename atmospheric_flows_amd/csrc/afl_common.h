@@ -75,7 +75,11 @@ struct Cfg {
   static constexpr int N1 = K + 1;
   static constexpr int NQ = 2 * K + 1;
   static constexpr int LT = (DIM == 2) ? NQ : NQ * NQ;        // threads per cell
-  static constexpr int CPB = AFL_NT / LT;                       // cells per block
+  // Cells that fit in one 64-wide wavefront are packed whole into waves (CPW per wave, the few
+  // spare lanes idle): all exchanges inside a cell are then wave-local and need no block barrier.
+  static constexpr bool WAVE_LOCAL = LT <= 64;
+  static constexpr int CPW = WAVE_LOCAL ? 64 / LT : 0;         // cells per wavefront
+  static constexpr int CPB = WAVE_LOCAL ? (AFL_NT / 64) * CPW : AFL_NT / LT;  // cells per block
   static constexpr int NPC = (DIM == 2) ? N1 * N1 : N1 * N1 * N1;
   static constexpr int NFQ = (DIM == 2) ? NQ : NQ * NQ;         // points per face
   static constexpr int ROWS = (DIM == 2) ? N1 : N1 * N1;        // node rows per cell
@@ -97,8 +101,14 @@ struct Slot {
   int ic[3];     // cell coordinates
   __device__ Slot(const Geo& g) {
     using C = Cfg<DIM, K>;
-    cb = threadIdx.x / C::LT;
-    l = threadIdx.x % C::LT;
+    if (C::WAVE_LOCAL) {
+      const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, lc = lane / C::LT;
+      l = lane % C::LT;
+      cb = lc < C::CPW ? wave * C::CPW + lc : C::CPB;  // spare lanes: no cell
+    } else {
+      cb = threadIdx.x / C::LT;
+      l = threadIdx.x % C::LT;
+    }
     cell = (int64_t)blockIdx.x * C::CPB + cb;
     act = (cb < C::CPB) && (cell < g.ncells);
     const int c = act ? (int)cell : 0;
@@ -107,6 +117,19 @@ struct Slot {
     else { ic[1] = (c / g.nc[0]) % g.nc[1]; ic[2] = c / (g.nc[0] * g.nc[1]); }
   }
 };
+
+// Barrier between the phases of a cell's computation (LDS written by some of the cell's threads,
+// read by others). Wave-local cells need only ordering inside the wavefront; otherwise the block.
+template <int DIM, int K>
+__device__ __attribute__((always_inline)) inline void cell_sync() {
+  if constexpr (Cfg<DIM, K>::WAVE_LOCAL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
 
 template <int DIM>
 __device__ __attribute__((always_inline)) inline int vaxis() { return DIM - 1; }
@@ -280,7 +303,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         t1[s.l * NQ + q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
       R c[N1];
 #pragma unroll
@@ -293,7 +316,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         v[q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     if (s.act && s.l < N1 * N1) {
       R d[N1];
@@ -307,7 +330,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         t1[s.l * NQ + q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
       R c[N1];
@@ -321,7 +344,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         t2[(iz * NQ + q) * NQ + qx] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
       R c[N1];
 #pragma unroll
@@ -334,7 +357,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         v[q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 
@@ -405,7 +428,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
       if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
     }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   // traces at face point l (every thread of an active cell owns one face point per face)
   if (s.act) {
 #pragma unroll
@@ -445,7 +468,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
         v[q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
@@ -460,7 +483,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
         t2[(iz * NQ + q) * NQ + qx] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
       R c[N1];
 #pragma unroll
@@ -473,7 +496,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
         v[q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 
@@ -529,7 +552,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
       }
     }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   if (s.act) {
     // the thread's rows of the interpolation table (face point l), read once
     const int qa = DIM == 2 ? s.l : s.l % NQ, qb = DIM == 2 ? 0 : s.l / NQ;
@@ -578,7 +601,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
@@ -596,7 +619,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
@@ -612,7 +635,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 
@@ -632,7 +655,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         t1[s.l * NQ + q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -642,7 +665,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         v[q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
@@ -653,7 +676,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         t1[s.l * NQ + q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -664,7 +687,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         t2[(iz * NQ + q) * NQ + qx] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -674,7 +697,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         v[q] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 
@@ -697,7 +720,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
           t1[f * E1 + s.l * NQ + q] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -713,7 +736,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
@@ -726,7 +749,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
           t1[f * E1 + s.l * NQ + q] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -743,7 +766,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -759,7 +782,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 
@@ -790,7 +813,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           i1[f * A1 + i * NQ + s.l] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
@@ -802,7 +825,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           out[f][i] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     if (s.act) {
 #pragma unroll
@@ -815,7 +838,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           i1[f * A1 + i * NQ * NQ + s.l] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -828,7 +851,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           i2[f * A2 + (iz * N1 + i) * NQ + qx] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
@@ -840,7 +863,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           out[f][i] = a;
         }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 
@@ -869,7 +892,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         if (HAS_G) tb[i * NQ + s.l] = b;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
@@ -882,7 +905,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         out[i] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   } else {
     R* ta = i1;
     R* tx = i1 + N1 * NQ * NQ;
@@ -902,7 +925,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         if (HAS_G) { tx[i * NQ * NQ + s.l] = bx; ty[i * NQ * NQ + s.l] = by; }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -918,7 +941,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         if (HAS_G) ux[(iz * N1 + i) * NQ + qx] = bx;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
@@ -931,7 +954,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         out[i] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   }
 }
 

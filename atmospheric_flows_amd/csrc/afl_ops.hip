@@ -106,7 +106,7 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, const Tab* __restrict
   if (s.act)
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = -F[f] * fp.jxw[f];
-  __syncthreads();
+  cell_sync<DIM, K>();
   double out[N1];
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
   add_faces<DIM, K>(s, Fq, T, out, 0xffu);
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
     for (int f = 0; f < 2 * DIM; ++f)
 #pragma unroll
       for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + f) * C::NFQ + s.l] = -F[f][a] * fp.jxw[f];
-  __syncthreads();
+  cell_sync<DIM, K>();
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     double Va[NQ], Ga[DIM][NQ];
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__
   if (s.act)
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = -F[f] * fp.jxw[f];
-  __syncthreads();
+  cell_sync<DIM, K>();
   double out[N1];
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
   add_faces<DIM, K>(s, Fq, T, out, 0xffu);
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, const Tab* __restric
         i1[i * NQ + s.l] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1)
       for (int i = 0; i < N1; ++i) {
         double a = 0.0;
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, const Tab* __restric
         i1[i * NQ * NQ + s.l] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
       for (int i = 0; i < N1; ++i) {
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, const Tab* __restric
         i2[(iz * N1 + i) * NQ + qx] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1 * N1)
       for (int i = 0; i < N1; ++i) {
         double a = 0.0;
@@ -445,7 +445,7 @@ __device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g,
       for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + fc) * C::NFQ + s.l] = f * avg * fp.n[fc][a] * fp.jxw[fc];
     }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     R V[NQ], G[DIM][NQ];
@@ -580,7 +580,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       Fq[f * C::NFQ + s.l] = F;
     }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   integrate_line<DIM, K, WITH_D, true, R>(s, i1, i2, T, V, G, out);
   add_faces<DIM, K, R>(s, Fq, T, out, 0xffu);
 }
@@ -643,7 +643,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* _
       tmp[s.l * N1 + i] = a;
     }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   const R* zr;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* _
         tmp2[i * N1 + s.l] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     zr = tmp2;
   } else {
     if (s.act && s.l < N1 * N1) {
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* _
         tmp2[(iz * N1 + i) * N1 + ix] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1 * N1) {
       const int ix = s.l % N1, iy = s.l / N1;
       R v[N1];
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* _
         tmp[(i * N1 + iy) * N1 + ix] = a;
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     zr = tmp;
   }
   if (ro) {
@@ -775,7 +775,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
     for (int q = 0; q < NQ; ++q) sco[s.l * NQ + q] = co[q];
     if (DIM == 3) sL[s.l] = lsum;
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   if (s.act && s.l < NQ) {
     // vertical marginal: sum over the lines of co at qz = l (and its reciprocal)
     R c = R(0);
@@ -792,7 +792,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
       smg[NQ + s.l] = by;  // sum over qx of the lines (qx, qy = l)
     }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   R total = R(0);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) total += smg[q];
@@ -834,7 +834,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
 #pragma unroll
       for (int j = 0; j < N1; ++j) Mi[(s.l * N1 + i) * N1 + j] = Am[i][j];
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   R cmin = R(1), cmax = R(1);
   if (s.act) {
     R mn = R(1e30), mx = R(0);
@@ -848,6 +848,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
   const R theta = R(0.5) * (alpha + beta);
   const R delta = fmax(R(0.5) * (beta - alpha), R(1e-14) * theta);
   const R sigma1 = theta / delta;
+  int nit_lane = 1;
   {
     // sigma = (sqrt(kappa) - 1) / (sqrt(kappa) + 1) = (kappa - 1) / (sqrt(kappa) + 1)^2, with
     // kappa - 1 formed in R so that kappa close to one keeps its digits
@@ -857,10 +858,20 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
     int n = 1;
     if (sg > 1e-30f) n = (int)ceilf(logf(0.5f * (float)tol) / logf(sg));
     n = n < 1 ? 1 : (n > maxit ? maxit : n);
-    if (s.act && s.l == 0) atomicMax(&nit_blk, n);
+    if constexpr (!C::WAVE_LOCAL)
+      if (s.act && s.l == 0) atomicMax(&nit_blk, n);
+    nit_lane = (s.act && s.l == 0) ? n : 1;
   }
-  __syncthreads();
-  const int nit = nit_blk;
+  int nit;
+  if constexpr (C::WAVE_LOCAL) {
+    // the cells of one wavefront iterate together: maximum over its lanes
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nit_lane = max(nit_lane, __shfl_xor(nit_lane, off, 64));
+    nit = nit_lane;
+  } else {
+    __syncthreads();
+    nit = nit_blk;
+  }
   const R zsc = SEP ? R(1) : itot;
   // entry (i, j) of the 1D inverse along axis ax: per-cell matrices in LDS (SEP) or the
   // reference table (uniform, no address taken of the kernel argument)
@@ -886,7 +897,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (DIM == 2) {
       if (s.act && s.l < N1) {
         R v[NC][N1];
@@ -908,7 +919,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
           }
         }
       }
-      __syncthreads();
+      cell_sync<DIM, K>();
       if (row_owner)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
@@ -931,7 +942,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
           }
         }
       }
-      __syncthreads();
+      cell_sync<DIM, K>();
       if (s.act && s.l < N1 * N1) {
         const int ix = s.l % N1, iy = s.l / N1;
 #pragma unroll
@@ -948,14 +959,14 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
           }
         }
       }
-      __syncthreads();
+      cell_sync<DIM, K>();
       if (row_owner)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
           for (int i = 0; i < N1; ++i) z[c][i] = zsc * dl[c * NPC + s.l * N1 + i];
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
   };
   // y = A v for all components
   auto apply = [&](const R (&v)[NC][N1], R (&y)[NC][N1]) {
@@ -964,7 +975,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int i = 0; i < N1; ++i) dl[c * NPC + s.l * N1 + i] = v[c][i];
-    __syncthreads();
+    cell_sync<DIM, K>();
     R vq[NC][NQ];
     eval_lines_lds<DIM, K, NC, R>(dl, s, t1, t2, T, vq);
 #pragma unroll
@@ -1094,7 +1105,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         tmp[c * NPC + s.l * N1 + i] = a;
       }
   }
-  __syncthreads();
+  cell_sync<DIM, K>();
   const R* res;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
@@ -1112,7 +1123,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     res = t2;
   } else {
     if (s.act && s.l < N1 * N1) {
@@ -1131,7 +1142,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     if (s.act && s.l < N1 * N1) {
       const int ix = s.l % N1, iy = s.l / N1;
 #pragma unroll
@@ -1148,7 +1159,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         }
       }
     }
-    __syncthreads();
+    cell_sync<DIM, K>();
     res = tmp;
   }
   if (ro)
