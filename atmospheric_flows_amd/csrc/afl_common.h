@@ -18,6 +18,10 @@
 #include <stdint.h>
 
 #define AFL_NT 256
+// wavefronts per block of the cell kernels whose cells are wave-local
+#ifndef AFL_CELL_WAVES
+#define AFL_CELL_WAVES 4
+#endif
 
 namespace afl {
 
@@ -79,7 +83,9 @@ struct Cfg {
   // spare lanes idle): all exchanges inside a cell are then wave-local and need no block barrier.
   static constexpr bool WAVE_LOCAL = LT <= 64;
   static constexpr int CPW = WAVE_LOCAL ? 64 / LT : 0;         // cells per wavefront
-  static constexpr int CPB = WAVE_LOCAL ? (AFL_NT / 64) * CPW : AFL_NT / LT;  // cells per block
+  static constexpr int WPB = AFL_CELL_WAVES;                   // wavefronts per block (wave-local)
+  static constexpr int NT = WAVE_LOCAL ? 64 * WPB : AFL_NT;    // threads per block
+  static constexpr int CPB = WAVE_LOCAL ? WPB * CPW : AFL_NT / LT;  // cells per block
   static constexpr int NPC = (DIM == 2) ? N1 * N1 : N1 * N1 * N1;
   static constexpr int NFQ = (DIM == 2) ? NQ : NQ * NQ;         // points per face
   static constexpr int ROWS = (DIM == 2) ? N1 : N1 * N1;        // node rows per cell

@@ -21,7 +21,7 @@ __device__ inline double diag_block_reduce(double v) {
   double r = 0.0;
   if (threadIdx.x == 0) {
     r = sh[0];
-    for (int i = 1; i < AFL_NT / 64; ++i) r = MAX ? fmax(r, sh[i]) : r + sh[i];
+    for (int i = 1; i < (int)(blockDim.x / 64); ++i) r = MAX ? fmax(r, sh[i]) : r + sh[i];
   }
   __syncthreads();
   return r;
@@ -130,7 +130,7 @@ void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const doub
   case D_ * 10 + K_: {                                                                                 \
     using C = Cfg<D_, K_>;                                                                             \
     const int64_t nb = (o.geo.ncells + C::CPB - 1) / C::CPB;                                           \
-    k_l2_partial<D_, K_><<<dim3((unsigned)nb), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, ncomp, o.N, a, b, nb, \
+    k_l2_partial<D_, K_><<<dim3((unsigned)nb), dim3(C::NT), 0, o.stream>>>(o.geo, o.dT, ncomp, o.N, a, b, nb, \
                                                                             work);                     \
     k_diag_finish<false><<<dim3(2), dim3(AFL_NT), 0, o.stream>>>(work, nb, nb, out2);                  \
   } break;

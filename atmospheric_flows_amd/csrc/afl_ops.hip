@@ -1276,11 +1276,11 @@ void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double*
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
     if (stage == 1)
-      k_rhs_rho<D_, K_, 1><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a21, 0.0, 0.0, o.N, dst);
+      k_rhs_rho<D_, K_, 1><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a21, 0.0, 0.0, o.N, dst);
     else if (stage == 2)
-      k_rhs_rho<D_, K_, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a31, t.a32, 0.0, o.N, dst);
+      k_rhs_rho<D_, K_, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a31, t.a32, 0.0, o.N, dst);
     else
-      k_rhs_rho<D_, K_, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.b1, t.b2, t.b3, o.N, dst);
+      k_rhs_rho<D_, K_, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.b1, t.b2, t.b3, o.N, dst);
   });
 }
 
@@ -1302,11 +1302,11 @@ void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* d
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
     if (stage == 1)
-      k_rhs_u<D_, K_, 1, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      k_rhs_u<D_, K_, 1, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
     else if (stage == 2)
-      k_rhs_u<D_, K_, 2, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      k_rhs_u<D_, K_, 2, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
     else
-      k_rhs_u<D_, K_, 3, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      k_rhs_u<D_, K_, 3, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
   });
 }
 
@@ -1330,23 +1330,23 @@ void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* d
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
     if (stage == 1)
-      k_rhs_p<D_, K_, 1, 2, 1><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      k_rhs_p<D_, K_, 1, 2, 1><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else if (stage == 2)
-      k_rhs_p<D_, K_, 2, 3, 2><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      k_rhs_p<D_, K_, 2, 3, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else
-      k_rhs_p<D_, K_, 3, 3, 3><<<gr, dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      k_rhs_p<D_, K_, 3, 3, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
   });
 }
 
 void launch_mass_apply(const OpCtx& o, const double* rho, double scale, int ncomp, const double* src, double* dst) {
   AFL_DISPATCH(o.dim, o.k, {
-    k_mass_apply<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, ncomp, o.N, src, dst);
+    k_mass_apply<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, ncomp, o.N, src, dst);
   });
 }
 
 void launch_inv_diag(const OpCtx& o, const double* rho, double scale, int ncomp, double* dst) {
   AFL_DISPATCH(o.dim, o.k, {
-    k_inv_diag<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, ncomp, o.N, dst);
+    k_inv_diag<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, ncomp, o.N, dst);
   });
 }
 
@@ -1354,7 +1354,7 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst) {
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   AFL_DISPATCH(o.dim, o.k, {
-    k_apply_B<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, f, o.N, p, dst);
+    k_apply_B<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, f, o.N, p, dst);
   });
 }
 
@@ -1363,9 +1363,9 @@ void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const d
   const double cdt = coeff * o.dt;
   AFL_DISPATCH(o.dim, o.k, {
     if (pD)
-      k_apply_C<D_, K_, true, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
+      k_apply_C<D_, K_, true, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
     else
-      k_apply_C<D_, K_, false, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, nullptr, 0.0, dst);
+      k_apply_C<D_, K_, false, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, nullptr, 0.0, dst);
   });
 }
 
@@ -1373,9 +1373,9 @@ void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncom
   const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
     if (ncomp == 1)
-      k_mass_solve<D_, K_, 1><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
+      k_mass_solve<D_, K_, 1><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
     else
-      k_mass_solve<D_, K_, D_><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
+      k_mass_solve<D_, K_, D_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
   });
 }
 
@@ -1384,7 +1384,7 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   const double tol2 = o.cell_tol;
   AFL_DISPATCH(o.dim, o.k, {
-    k_schur_w<D_, K_, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(AFL_NT), 0, o.stream>>>(o.geo, o.dT, rho, f, o.N, p, w, tol2, o.cell_maxit);
+    k_schur_w<D_, K_, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, f, o.N, p, w, tol2, o.cell_maxit);
   });
 }
 
@@ -1394,9 +1394,9 @@ void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStre
                        const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
   AFL_DISPATCH(dim, k, {
     const dim3 gr = grid_for<D_, K_>(geo.ncells);
-    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
-    else k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
-    k_apply_C<D_, K_, true, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
+    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+    else k_schur_w<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    k_apply_C<D_, K_, true, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
   });
 }
 // w = A^{-1} B p alone in precision R (the first half of the level operator)
@@ -1405,12 +1405,12 @@ void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStrea
                       R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev) {
   AFL_DISPATCH(dim, k, {
     if (sinv && ev)
-      hipExtLaunchKernelGGL(k_schur_w_lvl<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st, ev[0], ev[1],
+      hipExtLaunchKernelGGL(k_schur_w_lvl<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st, ev[0], ev[1],
                             0, geo, T, sinv, f, N, p, w);
     else if (sinv)
-      k_schur_w_lvl<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+      k_schur_w_lvl<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, sinv, f, N, p, w);
     else
-      k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+      k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
   });
 }
 template <typename R>
@@ -1436,9 +1436,9 @@ void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStre
                        const R* rin, R* rout, R c1, R c2, R* d, R* x) {
   AFL_DISPATCH(dim, k, {
     const dim3 gr = grid_for<D_, K_>(geo.ncells);
-    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, sinv, f, N, p, w);
-    else k_schur_w<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
-    k_apply_C_cheb<D_, K_, R><<<gr, dim3(AFL_NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), jbar,
+    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+    else k_schur_w<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    k_apply_C_cheb<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), jbar,
                                                            rin, rout, c1, c2, d, x);
   });
 }
@@ -1449,10 +1449,10 @@ void launch_apply_C_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipSt
   const R sD = (R)(1.0 / (kGamma - 1.0));
   AFL_DISPATCH(dim, k, {
     if (ev)
-      hipExtLaunchKernelGGL(k_apply_C_cheb<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st, ev[0], ev[1],
+      hipExtLaunchKernelGGL(k_apply_C_cheb<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st, ev[0], ev[1],
                             0, geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
     else
-      k_apply_C_cheb<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(AFL_NT), 0, st>>>(
+      k_apply_C_cheb<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st>>>(
           geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
   });
 }
