@@ -807,6 +807,158 @@ int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms) {
   return AFL_OK;
 }
 
+// ---- VTU output (EulerSolver::output_results, Atmospheric_Flow.cc:886-918) ------------------
+extern "C++" {
+namespace {
+void b64_append(std::string& out, const unsigned char* d, size_t n) {
+  static const char* tb = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  size_t i = 0;
+  for (; i + 2 < n; i += 3) {
+    const unsigned v = (d[i] << 16) | (d[i + 1] << 8) | d[i + 2];
+    out += tb[(v >> 18) & 63]; out += tb[(v >> 12) & 63]; out += tb[(v >> 6) & 63]; out += tb[v & 63];
+  }
+  if (i < n) {
+    const unsigned v = (d[i] << 16) | ((i + 1 < n ? d[i + 1] : 0) << 8);
+    out += tb[(v >> 18) & 63]; out += tb[(v >> 12) & 63];
+    out += i + 1 < n ? tb[(v >> 6) & 63] : '=';
+    out += '=';
+  }
+}
+// one binary DataArray payload: base64 of (UInt64 byte count, raw little-endian data)
+template <typename T>
+std::string vtu_binary(const std::vector<T>& v) {
+  std::vector<unsigned char> buf(8 + v.size() * sizeof(T));
+  const uint64_t nb = v.size() * sizeof(T);
+  std::memcpy(buf.data(), &nb, 8);
+  if (nb) std::memcpy(buf.data() + 8, v.data(), nb);
+  std::string o;
+  o.reserve(buf.size() * 4 / 3 + 8);
+  b64_append(o, buf.data(), buf.size());
+  return o;
+}
+}  // namespace
+}  // extern "C++"
+
+int afl_write_vtu(afl_operator* op, const char* path, const double* rho, const double* u, const double* p,
+                  int high_order) {
+  if (!op || !path || !rho || !u || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const OpCtx& o = op->o;
+  const int dim = o.dim, k = o.k, n1 = k + 1;
+  if (high_order && dim != 2) return fail(AFL_ERR_INVALID_ARGUMENT, "high-order VTU output is written for 2D meshes");
+  const int64_t N = o.N, ncells = o.geo.ncells;
+  const int npc = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  std::vector<double> hr(N), hu(N * dim), hp(N);
+  if (hipStreamSynchronize(o.stream) != hipSuccess || hipMemcpy(hr.data(), rho, N * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hu.data(), u, N * dim * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hp.data(), p, N * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(AFL_ERR_DEVICE, "write_vtu: device to host copy failed");
+  // patch points: k+1 equidistant points per direction of every cell (DataOut::build_patches
+  // with n_subdivisions = degree); E[i][j] = phi_j(i/k) on the support points
+  double E[5][5];
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j) E[i][j] = lag(o.T.ND, n1, j, (double)i / k);
+  const Geo& g = o.geo;
+  const int nx = g.nc[0], ny = dim == 3 ? g.nc[1] : 1;
+  auto zmap = [&](double x, double y, double zeta) {
+    const double h = g.h0 / (1.0 + (x * x + y * y) * g.inv_a2);
+    return zeta + h * (1.0 - zeta / g.H);
+  };
+  const int64_t npts = ncells * npc;
+  std::vector<double> X(npts * 3), R(npts), U(npts * 3, 0.0), P(npts);
+  std::vector<double> tmp(npc);
+  auto interp = [&](const double* f, double* out) {  // nodal -> patch values of one cell
+    for (int q = 0; q < npc; ++q) {
+      const int qx = q % n1, qy = (q / n1) % n1, qz = dim == 3 ? q / (n1 * n1) : 0;
+      double a = 0.0;
+      for (int i = 0; i < npc; ++i) {
+        const int ix = i % n1, iy = (i / n1) % n1, iz = dim == 3 ? i / (n1 * n1) : 0;
+        double w = E[qx][ix] * E[qy][iy];
+        if (dim == 3) w *= E[qz][iz];
+        a += w * f[i];
+      }
+      out[q] = a;
+    }
+  };
+  for (int64_t c = 0; c < ncells; ++c) {
+    const int ix = (int)(c % nx), iy = dim == 3 ? (int)((c / nx) % ny) : 0;
+    const int iz = dim == 3 ? (int)(c / ((int64_t)nx * ny)) : (int)(c / nx);
+    const int64_t base = c * npc;
+    interp(&hr[base], &R[base]);
+    interp(&hp[base], &P[base]);
+    for (int a = 0; a < dim; ++a) {
+      interp(&hu[a * N + base], tmp.data());
+      for (int q = 0; q < npc; ++q) U[(base + q) * 3 + a] = tmp[q];
+    }
+    for (int q = 0; q < npc; ++q) {
+      const int qx = q % n1, qy = (q / n1) % n1, qz = dim == 3 ? q / (n1 * n1) : qy;
+      const double x = g.lo[0] + (ix + (double)qx / k) * g.hc[0];
+      const double y = dim == 3 ? g.lo[1] + (iy + (double)qy / k) * g.hc[1] : 0.0;
+      const double zeta = (iz + (double)qz / k) * g.hc[dim - 1];
+      double* xp = &X[(base + q) * 3];
+      xp[0] = x;
+      if (dim == 2) { xp[1] = zmap(x, 0.0, zeta); xp[2] = 0.0; }
+      else { xp[1] = y; xp[2] = zmap(x, y, zeta); }
+    }
+  }
+  std::vector<int64_t> conn, offs;
+  std::vector<uint8_t> types;
+  auto pid = [&](int64_t c, int a, int b, int cc) { return c * npc + (int64_t)(cc * n1 + b) * n1 + a; };
+  for (int64_t c = 0; c < ncells; ++c) {
+    if (high_order) {
+      // VTK_LAGRANGE_QUADRILATERAL: corners, then the edges (0,1), (1,2), (3,2), (0,3), then the interior
+      conn.push_back(pid(c, 0, 0, 0)); conn.push_back(pid(c, k, 0, 0));
+      conn.push_back(pid(c, k, k, 0)); conn.push_back(pid(c, 0, k, 0));
+      for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, 0, 0));
+      for (int j = 1; j < k; ++j) conn.push_back(pid(c, k, j, 0));
+      for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, k, 0));
+      for (int j = 1; j < k; ++j) conn.push_back(pid(c, 0, j, 0));
+      for (int j = 1; j < k; ++j)
+        for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, j, 0));
+      offs.push_back((int64_t)conn.size());
+      types.push_back(70);
+      continue;
+    }
+    for (int zc = 0; zc < (dim == 3 ? k : 1); ++zc)
+      for (int b = 0; b < k; ++b)
+        for (int a = 0; a < k; ++a) {
+          if (dim == 2) {
+            conn.push_back(pid(c, a, b, 0)); conn.push_back(pid(c, a + 1, b, 0));
+            conn.push_back(pid(c, a + 1, b + 1, 0)); conn.push_back(pid(c, a, b + 1, 0));
+            types.push_back(9);
+          } else {
+            for (int lz = 0; lz < 2; ++lz) {
+              conn.push_back(pid(c, a, b, zc + lz)); conn.push_back(pid(c, a + 1, b, zc + lz));
+              conn.push_back(pid(c, a + 1, b + 1, zc + lz)); conn.push_back(pid(c, a, b + 1, zc + lz));
+            }
+            types.push_back(12);
+          }
+          offs.push_back((int64_t)conn.size());
+        }
+  }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(AFL_ERR_INVALID_ARGUMENT, std::string("write_vtu: cannot open ") + path);
+  std::string hdr = "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
+                    "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\"" +
+                    std::to_string(npts) + "\" NumberOfCells=\"" + std::to_string(types.size()) + "\">\n";
+  auto arr = [&](const char* type, const char* name, int ncomp, const std::string& data) {
+    std::string a = std::string("<DataArray type=\"") + type + "\"";
+    if (name) a += std::string(" Name=\"") + name + "\"";
+    if (ncomp > 1) a += " NumberOfComponents=\"" + std::to_string(ncomp) + "\"";
+    a += " format=\"binary\">\n" + data + "\n</DataArray>\n";
+    return a;
+  };
+  std::string body = hdr + "<PointData Scalars=\"rho\" Vectors=\"u\">\n" + arr("Float64", "rho", 1, vtu_binary(R)) +
+                     arr("Float64", "u", 3, vtu_binary(U)) + arr("Float64", "p", 1, vtu_binary(P)) +
+                     "</PointData>\n<Points>\n" + arr("Float64", "Points", 3, vtu_binary(X)) +
+                     "</Points>\n<Cells>\n" + arr("Int64", "connectivity", 1, vtu_binary(conn)) +
+                     arr("Int64", "offsets", 1, vtu_binary(offs)) + arr("UInt8", "types", 1, vtu_binary(types)) +
+                     "</Cells>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+  const size_t wr = std::fwrite(body.data(), 1, body.size(), f);
+  std::fclose(f);
+  if (wr != body.size()) return fail(AFL_ERR_DEVICE, "write_vtu: short write");
+  return AFL_OK;
+}
+
 int afl_compute_diagnostics(afl_operator* op, const double* rho, const double* u, const double* p,
                             afl_diagnostics* out) {
   if (!op || !rho || !out) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");

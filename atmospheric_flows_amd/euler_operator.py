@@ -59,6 +59,7 @@ def load_library():
     L.afl_imex_step.argtypes = [vp, dp, dp, dp, C.POINTER(StepInfo)]
     L.afl_init_hill_state.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
     L.afl_background.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
+    L.afl_write_vtu.argtypes = [vp, C.c_char_p, dp, dp, dp, C.c_int]
     L.afl_kernel_timer_start.argtypes = [vp, C.c_int, C.c_int]
     L.afl_kernel_timer_stop.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.afl_compute_diagnostics.argtypes = [vp, dp, dp, dp, C.POINTER(Diagnostics)]
@@ -180,6 +181,10 @@ class EulerOperator:
         d = Diagnostics()
         self._chk(self.L.afl_compute_diagnostics(self.h, _ptr(rho), _ptr(u), _ptr(p), C.byref(d)))
         return d.as_dict()
+
+    def write_vtu(self, path, rho, u, p, high_order=False):
+        """VTU file of the state (point data rho, u, p on the degree-k patch points of each cell)."""
+        self._chk(self.L.afl_write_vtu(self.h, os.fsencode(path), _ptr(rho), _ptr(u), _ptr(p), 1 if high_order else 0))
 
     def l2_difference(self, ncomp, a, b=None):
         """(||a - b||_L2, ||b||_L2) of ncomp-component fields."""

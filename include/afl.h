@@ -102,6 +102,15 @@ int afl_kernel_timer_start(afl_operator* op, int kernel, int max_launches);
 int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms);
 
 /* ---- diagnostics ------------------------------------------------------------------------ */
+/* VTU output of a state (EulerSolver::output_results, Atmospheric_Flow.cc:886-918): point data
+ * "rho", "u" (3 components) and "p" on k+1 equidistant patch points per direction of every cell
+ * (DataOut::build_patches with n_subdivisions = degree, mapped with the terrain-following map),
+ * written as k^dim linear sub-cells (high_order = 0) or, in 2D, one VTK Lagrange quadrilateral
+ * per cell (high_order = 1, the reference's second file). Binary (base64) XML, UInt64 headers.
+ * Copies the fields to the host; synchronises the stream. */
+int afl_write_vtu(afl_operator* op, const char* path, const double* rho, const double* u, const double* p,
+                  int high_order);
+
 /* Density bounds, maximum celerity and speed of a state and the two Courant numbers
  * (Atmospheric_Flow.cc:992-1087 get_max_velocity / get_min_density / get_max_density /
  * compute_max_celerity; CFL_c and CFL_u as printed at :1225-1235). The reference evaluates
