@@ -485,12 +485,17 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, const Tab* __restrict
 template <int DIM, int K, bool WITH_D, bool BATCH>
 struct LdsC {
   using C = Cfg<DIM, K>;
-  static constexpr int NF = (WITH_D ? 1 : 0) + 1 + DIM;  // p_fix, w_0..w_{d-1}, (p)
+  static constexpr int NF = 1 + DIM;                     // batched: p_fix, w_0..w_{d-1} (p apart)
   static constexpr int NFF = 1 + DIM;                    // fields needed on the faces
   static constexpr int EV = BATCH ? NF * (C::EV1 + C::EV2) : C::EV1 + C::EV2;
   static constexpr int IN = C::IN1 + C::IN2;
   static constexpr int W = EV > IN ? EV : IN;
-  static constexpr int WF = W + (BATCH ? NFF : 1) * FaceLds<DIM, K>::FN;
+  static constexpr int WF0 = W + (BATCH ? NFF : 1) * FaceLds<DIM, K>::FN;
+  // the face-flux block is written once the evaluations are done: inside the work area after the
+  // integration scratch when it fits there, else appended
+  static constexpr int F = 2 * DIM * C::NFQ;
+  static constexpr int FOFF = IN + F <= WF0 ? IN : WF0;
+  static constexpr int WF = IN + F <= WF0 ? WF0 : WF0 + F;
 };
 
 // Result rows of C (optionally fused with D p) for the thread's node row: out = (WITH_D ?
@@ -519,7 +524,6 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
     fl[0] = pfix;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) fl[1 + d] = w + d * N;
-    if (WITH_D) fl[LC::NF - 1] = pD;
     R v[LC::NF][NQ], tr[LC::NFF][2 * DIM][2];
     eval_lines_faces<DIM, K, LC::NF, LC::NFF, R>(fl, s, nb, t1, t1 + LC::NF * C::EV1, sh + LC::W, T, v, tr);
 #pragma unroll
@@ -527,7 +531,11 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       pq[q] = v[0][q];
 #pragma unroll
       for (int d = 0; d < DIM; ++d) G[d][q] = sgn * (-cdt * gr * v[0][q] * v[1 + d][q]);
-      if (WITH_D) V[q] = v[LC::NF - 1][q] * scaleD * lg.jxw(g, T, q);
+    }
+    if (WITH_D) {
+      eval_line<DIM, K, R>(pD, s, t1, t1 + C::EV1, T, V);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
     }
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f)
@@ -592,14 +600,13 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __rest
                                                      const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
                                                      R* __restrict__ dst) {
   const auto& T = *Tp;
-  constexpr int WFC = LdsC<DIM, K, WITH_D, sizeof(R) == 4>::WF;
-  __shared__ R shw[Cfg<DIM, K>::CPB * WFC];
-  __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4>;
+  __shared__ R shw[Cfg<DIM, K>::CPB * LC::WF];
   Slot<DIM, K> s(g);
   const int cbs = s.cb < Cfg<DIM, K>::CPB ? s.cb : 0;
   R out[K + 1];
-  apply_C_rows<DIM, K, WITH_D, R>(g, T, s, shw + cbs * WFC, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w,
-                                  N, pD, scaleD, out);
+  R* sh = shw + cbs * LC::WF;
+  apply_C_rows<DIM, K, WITH_D, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, out);
   store_row<DIM, K, R>(dst, s, out);
 }
 
@@ -614,16 +621,17 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* _
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
   const auto& T = *Tp;
-  constexpr int WFC = LdsC<DIM, K, true, sizeof(R) == 4>::WF;
-  __shared__ R shw[Cfg<DIM, K>::CPB * WFC];
-  __shared__ R shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+  using LC = LdsC<DIM, K, true, sizeof(R) == 4>;
+  __shared__ R shw[Cfg<DIM, K>::CPB * LC::WF];
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
   Slot<DIM, K> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
-  R* sh = shw + cbs * WFC;
+  R* sh = shw + cbs * LC::WF;
+  static_assert(2 * C::NPC <= LC::FOFF || LC::FOFF >= LC::W, "epilogue scratch must not overlap the face fluxes");
   R t[N1];
-  apply_C_rows<DIM, K, true, R>(g, T, s, sh, shF + cbs * Lds<DIM, K>::F, cdt, pfix, w, N, pD, scaleD, t);
+  apply_C_rows<DIM, K, true, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, t);
+  cell_sync<DIM, K>();  // the face fluxes and work area are read until here; the epilogue reuses them
   const bool ro = s.act && s.l < ROWS;
   const int64_t base = (s.act ? s.cell : 0) * NPC + s.l * N1;
   R* tmp = sh;
