@@ -7,6 +7,14 @@ namespace afl {
 
 struct Srcs { const double* p[12]; };
 
+// src.p[i] for a runtime i without indexing the kernel argument through memory
+__device__ __attribute__((always_inline)) inline const double* src_at(const Srcs& s, int i) {
+  const double* v = s.p[0];
+#pragma unroll
+  for (int j = 1; j < 12; ++j) v = (i == j) ? s.p[j] : v;
+  return v;
+}
+
 template <int DIM, int K>
 struct Lds {
   using C = Cfg<DIM, K>;
@@ -51,7 +59,7 @@ struct FacePts {
 // E1: vmult_rhs_rho_update (euler_operator.h:342-473 cell, 476-692 faces, boundary none).
 // src = {rho_j, u_j} for the NS explicit states; w = stage weights (a21 | a31,a32 | b1,b2,b3).
 template <int DIM, int K, int NS>
-__global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, const Tab* __restrict__ Tp, Srcs src, double dt, double w0, double w1,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 2 : 1))) void k_rhs_rho(Geo g, const Tab* __restrict__ Tp, Srcs src, double dt, double w0, double w1,
                                                      double w2, int64_t N, double* __restrict__ dst) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
@@ -61,7 +69,6 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, const Tab* __restrict
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K> fp(g, T, s);
   double V[NQ], G[DIM][NQ], F[2 * DIM];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) { V[q] = 0.0; for (int d = 0; d < DIM; ++d) G[d][q] = 0.0; }
@@ -82,13 +89,15 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, const Tab* __restrict
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
       if (nb.c[f] < 0) continue;
+      double nfv[3], jfv;  // face geometry recomputed here: fewer live registers
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nfv, jfv);
       const double rp = trr[f][0], rm = trr[f][1];
       double avg = 0.0, unp = 0.0, unm = 0.0;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
-        avg += 0.5 * (rp * tru[d][f][0] + rm * tru[d][f][1]) * fp.n[f][d];
-        unp += tru[d][f][0] * fp.n[f][d];
-        unm += tru[d][f][1] * fp.n[f][d];
+        avg += 0.5 * (rp * tru[d][f][0] + rm * tru[d][f][1]) * nfv[d];
+        unp += tru[d][f][0] * nfv[d];
+        unm += tru[d][f][1] * nfv[d];
       }
       const double lam = fmax(fabs(unp), fabs(unm));
       F[f] += wj[j] * dt * (avg + 0.5 * lam * (rp - rm));
@@ -105,7 +114,11 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_rho(Geo g, const Tab* __restrict
   double* Fq = shF + cbs * Lds<DIM, K>::F;
   if (s.act)
 #pragma unroll
-    for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = -F[f] * fp.jxw[f];
+    for (int f = 0; f < 2 * DIM; ++f) {
+      double nfv[3], jfv;
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nfv, jfv);
+      Fq[f * C::NFQ + s.l] = -F[f] * jfv;
+    }
   cell_sync<DIM, K>();
   double out[N1];
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
@@ -130,53 +143,55 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K> fp(g, T, s);
-  double V[DIM][NQ], Sym[NSYM][NQ], P[NQ], F[2 * DIM][DIM];
+  // Sym: symmetric flux tensor rho u (x) u with the pressure term on its diagonal
+  double V[DIM][NQ], Sym[NSYM][NQ], F[2 * DIM][DIM];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     for (int d = 0; d < DIM; ++d) V[d][q] = 0.0;
     for (int d = 0; d < NSYM; ++d) Sym[d][q] = 0.0;
-    P[q] = 0.0;
   }
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f)
     for (int a = 0; a < DIM; ++a) F[f][a] = 0.0;
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
+    const double wj = pick<3>(cf.w, j), wpj = pick<3>(cf.wp, j);
     double r[NQ], u[DIM][NQ], pj[NQ];
     double trr[2 * DIM][2], tru[DIM][2 * DIM][2], trp[2 * DIM][2];
-    eval_line_faces<DIM, K>(src.p[3 * j], s, nb, t1, t2, fn, T, r, trr);
+    eval_line_faces<DIM, K>(src_at(src, 3 * j), s, nb, t1, t2, fn, T, r, trr);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src.p[3 * j + 1] + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
-    eval_line_faces<DIM, K>(src.p[3 * j + 2], s, nb, t1, t2, fn, T, pj, trp);
+    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
+    eval_line_faces<DIM, K>(src_at(src, 3 * j + 2), s, nb, t1, t2, fn, T, pj, trp);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       if (j == 0) for (int d = 0; d < DIM; ++d) V[d][q] = r[q] * u[d][q];
+      const double pw = wpj * dt * iMa2 * pj[q];
       int m = 0;
       for (int a = 0; a < DIM; ++a)
-        for (int b = a; b < DIM; ++b) Sym[m++][q] += cf.w[j] * dt * r[q] * u[a][q] * u[b][q];
-      P[q] += cf.wp[j] * dt * iMa2 * pj[q];
+        for (int b = a; b < DIM; ++b) Sym[m++][q] += wj * dt * r[q] * u[a][q] * u[b][q] + (a == b ? pw : 0.0);
     }
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
+      double nfv[3], jfv;  // face geometry recomputed here: fewer live registers
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nfv, jfv);
       const double pp = trp[f][0];
       if (nb.c[f] >= 0) {
         const double rp = trr[f][0], rm = trr[f][1], pm = trp[f][1];
         double unp = 0.0, unm = 0.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) { unp += tru[d][f][0] * fp.n[f][d]; unm += tru[d][f][1] * fp.n[f][d]; }
+        for (int d = 0; d < DIM; ++d) { unp += tru[d][f][0] * nfv[d]; unm += tru[d][f][1] * nfv[d]; }
         const double lam = fmax(fabs(unp), fabs(unm));
         const double avg_p = 0.5 * (pp + pm);
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
           const double avg_tn = 0.5 * (rp * tru[a][f][0] * unp + rm * tru[a][f][1] * unm);
           const double jump = rp * tru[a][f][0] - rm * tru[a][f][1];
-          F[f][a] += cf.w[j] * dt * avg_tn + cf.wp[j] * dt * iMa2 * avg_p * fp.n[f][a] + cf.w[j] * dt * 0.5 * lam * jump;
+          F[f][a] += wj * dt * avg_tn + wpj * dt * iMa2 * avg_p * nfv[a] + wj * dt * 0.5 * lam * jump;
         }
       } else {
         // wall: p_D = p, flux (p + p_D)/2 n (euler_operator.h:2213-2218)
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) F[f][a] += cf.wp[j] * dt * iMa2 * (0.5 * (pp + pp)) * fp.n[f][a];
+        for (int a = 0; a < DIM; ++a) F[f][a] += wpj * dt * iMa2 * (0.5 * (pp + pp)) * nfv[a];
       }
     }
   }
@@ -190,9 +205,12 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
   double* Fq = shF + cbs * Lds<DIM, K>::F * DIM;
   if (s.act)
 #pragma unroll
-    for (int f = 0; f < 2 * DIM; ++f)
+    for (int f = 0; f < 2 * DIM; ++f) {
+      double nfv[3], jfv;
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nfv, jfv);
 #pragma unroll
-      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + f) * C::NFQ + s.l] = -F[f][a] * fp.jxw[f];
+      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + f) * C::NFQ + s.l] = -F[f][a] * jfv;
+    }
   cell_sync<DIM, K>();
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -203,7 +221,7 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
       for (int b = 0; b < DIM; ++b) {
         const int lo = a < b ? a : b, hi = a < b ? b : a;
         const int m = lo * DIM - lo * (lo - 1) / 2 + (hi - lo);
-        Gp[b] = Sym[m][q] + (a == b ? P[q] : 0.0);
+        Gp[b] = Sym[m][q];
       }
       lg.gref(g, T, s.ic, q, Gp, R);
       for (int b = 0; b < DIM; ++b) Ga[b][q] = R[b];
@@ -223,7 +241,7 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
 struct E3Coef { double w[3], wt[3], gw[3]; };
 
 template <int DIM, int K, int NE, int NG, int STAGE>
-__global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__ Tp, Srcs src, E3Coef cf, double dt, double Ma2,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 2 : 1))) void k_rhs_p(Geo g, const Tab* __restrict__ Tp, Srcs src, E3Coef cf, double dt, double Ma2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
@@ -233,21 +251,23 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K> fp(g, T, s);
   double V[NQ], G[DIM][NQ], F[2 * DIM];
   double tu0[DIM][2 * DIM][2];  // u_old traces (stage-2 kinetic jump as written)
 #pragma unroll
   for (int q = 0; q < NQ; ++q) { V[q] = 0.0; for (int d = 0; d < DIM; ++d) G[d][q] = 0.0; }
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f) F[f] = 0.0;
-#pragma unroll
+  // 2D: one explicit state per iteration, not unrolled, so the states' fields are never live
+  // together (3D unrolls: its register allocation spills when the loop is kept)
+#pragma unroll(DIM == 2 ? 1 : NE)
   for (int j = 0; j < NE; ++j) {
+    const double wj = pick<3>(cf.w, j), wpj = pick<3>(cf.wt, j);
     double r[NQ], u[DIM][NQ], p[NQ];
     double trr[2 * DIM][2], tru[DIM][2 * DIM][2], trp[2 * DIM][2];
-    eval_line_faces<DIM, K>(src.p[3 * j], s, nb, t1, t2, fn, T, r, trr);
+    eval_line_faces<DIM, K>(src_at(src, 3 * j), s, nb, t1, t2, fn, T, r, trr);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src.p[3 * j + 1] + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
-    eval_line_faces<DIM, K>(src.p[3 * j + 2], s, nb, t1, t2, fn, T, p, trp);
+    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
+    eval_line_faces<DIM, K>(src_at(src, 3 * j + 2), s, nb, t1, t2, fn, T, p, trp);
     if (STAGE == 2 && j == 0)
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
@@ -259,33 +279,35 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__
       for (int d = 0; d < DIM; ++d) uu += u[d][q] * u[d][q];
       const double E = 1.0 / gm1 * (p[q] / r[q]) + 0.5 * Ma2 * uu;
       if (j == 0) V[q] += r[q] * E;
-      const double kin = 0.5 * cf.w[j] * dt * Ma2 * uu * r[q];
-      const double ent = cf.wt[j] * dt * (r[q] * (E - 0.5 * Ma2 * uu) + p[q]);
+      const double kin = 0.5 * wj * dt * Ma2 * uu * r[q];
+      const double ent = wpj * dt * (r[q] * (E - 0.5 * Ma2 * uu) + p[q]);
       for (int d = 0; d < DIM; ++d) G[d][q] += (kin + ent) * u[d][q];
     }
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
       if (nb.c[f] < 0) continue;
+      double nfv[3], jfv;  // face geometry recomputed here: fewer live registers
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nfv, jfv);
       const double rp = trr[f][0], rm = trr[f][1], pp = trp[f][0], pm = trp[f][1];
       double uup = 0.0, uum = 0.0, unp = 0.0, unm = 0.0;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
         uup += tru[d][f][0] * tru[d][f][0]; uum += tru[d][f][1] * tru[d][f][1];
-        unp += tru[d][f][0] * fp.n[f][d]; unm += tru[d][f][1] * fp.n[f][d];
+        unp += tru[d][f][0] * nfv[d]; unm += tru[d][f][1] * nfv[d];
       }
       const double Ep = 1.0 / gm1 * pp / rp + 0.5 * Ma2 * uup;
       const double Em = 1.0 / gm1 * pm / rm + 0.5 * Ma2 * uum;
       double avg_kin = 0.0, avg_ent = 0.0;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
-        avg_kin += 0.5 * (0.5 * uup * rp * tru[d][f][0] + 0.5 * uum * rm * tru[d][f][1]) * fp.n[f][d];
+        avg_kin += 0.5 * (0.5 * uup * rp * tru[d][f][0] + 0.5 * uum * rm * tru[d][f][1]) * nfv[d];
         avg_ent += 0.5 * (((Ep - 0.5 * Ma2 * uup) * rp + pp) * tru[d][f][0] +
-                          ((Em - 0.5 * Ma2 * uum) * rm + pm) * tru[d][f][1]) * fp.n[f][d];
+                          ((Em - 0.5 * Ma2 * uum) * rm + pm) * tru[d][f][1]) * nfv[d];
       }
       const double lam = fmax(fabs(unp), fabs(unm));
       if (STAGE == 3) {
         const double jump = rp * Ep - rm * Em;
-        F[f] += cf.w[j] * dt * Ma2 * avg_kin + cf.w[j] * dt * avg_ent + cf.w[j] * dt * 0.5 * lam * jump;
+        F[f] += wj * dt * Ma2 * avg_kin + wj * dt * avg_ent + wj * dt * 0.5 * lam * jump;
       } else {
         double jk;
         if (STAGE == 2 && j == 1) {  // euler_operator.h:1182-1183, u_tmp_2 . u_old as written
@@ -297,7 +319,7 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__
           jk = rp * 0.5 * uup - rm * 0.5 * uum;
         }
         const double je = rp * (Ep - 0.5 * Ma2 * uup) - rm * (Em - 0.5 * Ma2 * uum);
-        F[f] += cf.w[j] * dt * Ma2 * (avg_kin + 0.5 * lam * jk) + cf.wt[j] * dt * (avg_ent + 0.5 * lam * je);
+        F[f] += wj * dt * Ma2 * (avg_kin + 0.5 * lam * jk) + wpj * dt * (avg_ent + 0.5 * lam * je);
       }
     }
   }
@@ -333,7 +355,11 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_p(Geo g, const Tab* __restrict__
   double* Fq = shF + cbs * Lds<DIM, K>::F;
   if (s.act)
 #pragma unroll
-    for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = -F[f] * fp.jxw[f];
+    for (int f = 0; f < 2 * DIM; ++f) {
+      double nfv[3], jfv;
+      face_geo<DIM, K>(g, T, s.ic, f, s.l, nfv, jfv);
+      Fq[f * C::NFQ + s.l] = -F[f] * jfv;
+    }
   cell_sync<DIM, K>();
   double out[N1];
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
