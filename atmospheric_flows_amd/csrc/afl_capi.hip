@@ -189,14 +189,19 @@ struct afl_operator {
   bool dinv_ready = false;
   // IMEX work vectors
   DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
+  // adjoint work vectors (Schur tangent and its transpose; step adjoint)
+  DevBuf adjW0, adjT1, adjT2, adjC;
+  static constexpr int kAdjBufs = 34;
+  DevBuf aj[kAdjBufs];
   double* host_sc = nullptr;  // pinned scalars
   Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
   KernelTimer timer;          // per-kernel HIP-event timing (afl_kernel_timer_*)
   ~afl_operator() {
     DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
-                     &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix};
+                     &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix, &adjW0, &adjT1, &adjT2, &adjC};
     for (DevBuf* b : all) b->release();
+    for (DevBuf& b : aj) b.release();
     if (host_sc) hipHostFree(host_sc);
     if (dtab) hipFree(dtab);
     for (hipEvent_t e : timer.ev) hipEventDestroy(e);
@@ -282,7 +287,10 @@ int ensure_dinv(afl_operator* op) {
 // Right-preconditioned restarted GMRES, Jacobi(D), classical Gram-Schmidt applied twice.
 // (The reference's left-preconditioned deal.II GMRES compares the preconditioned residual
 // with eps*||rhs||; the Arnoldi estimate here is the true residual ||b - S x||.)
-int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+// The operator is a callable apply(in, out); what names the solve in errors.
+template <class Apply>
+int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double tol, int max_it, int m, int* iters,
+             double* res_out, const char* what) {
   int rc;
   const int64_t N = op->N();
   hipStream_t st = op->o.stream;
@@ -299,7 +307,7 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
   int it = 0;
   double res = 0.0;
   while (true) {
-    schur_apply(op, x, w);
+    apply(x, w);
     vec_copy(st, N, b, V);
     vec_axpby(st, N, -1.0, w, 1.0, V);
     vec_norm2(st, N, V, op->red.p, sc + kScNrm);
@@ -309,7 +317,7 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
     if (!(beta > tol)) break;
     if (it >= max_it) {
       *iters = it; *res_out = res;
-      return fail(AFL_ERR_NOT_CONVERGED, "Schur GMRES did not converge");
+      return fail(AFL_ERR_NOT_CONVERGED, what);
     }
     vec_scale_by_inv(st, N, sc + kScNrm, V);
     std::fill(g.begin(), g.end(), 0.0);
@@ -321,7 +329,7 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
       const double* vj = V + (int64_t)j * N;
       double* vn = V + (int64_t)(j + 1) * N;
       vec_mul(st, N, vj, op->dinv_D.p, z);
-      schur_apply(op, z, vn);
+      apply(z, vn);
       vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH1);
       vec_maxpy(st, N, V, N, j + 1, sc + kScH1, -1.0, vn);
       vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH2);
@@ -364,12 +372,18 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
     if (conv) break;
     if (it >= max_it) {
       *iters = it; *res_out = res;
-      return fail(AFL_ERR_NOT_CONVERGED, "Schur GMRES did not converge");
+      return fail(AFL_ERR_NOT_CONVERGED, what);
     }
   }
   *iters = it;
   *res_out = res;
   return check_device("gmres");
+}
+
+int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+  return gmres_op(
+      op, [op](const double* in, double* out) { schur_apply(op, in, out); }, x, b, tol, max_it, m, iters, res_out,
+      "Schur GMRES did not converge");
 }
 
 // p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
@@ -746,6 +760,99 @@ int afl_vmult_schur_velocity(afl_operator* op, double* w, const double* p) {
   if (!op || !w || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   launch_schur_w(op->o, op->rho_fix.p, p, w);
   return check_device("vmult_schur_velocity");
+}
+
+// ---- discrete-adjoint building blocks ----------------------------------------------------
+
+int afl_vjp_rhs_rho_update(afl_operator* op, const double* rbar, double* const* src_bar, const double* const* src,
+                           int n_src) {
+  if (!op || !rbar || !src_bar) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int st = (int)op->o.hyp;
+  int rc;
+  if ((rc = check_src(src, n_src, 2 * st, "vjp_rhs_rho_update"))) return rc;
+  for (int i = 0; i < n_src; ++i)
+    if (!src_bar[i]) return fail(AFL_ERR_INVALID_ARGUMENT, "vjp_rhs_rho_update: null output vector");
+  launch_vjp_rhs_rho(op->o, st, rbar, src, src_bar);
+  return check_device("vjp_rhs_rho_update");
+}
+
+int afl_vjp_rhs_velocity_fixed(afl_operator* op, const double* rbar, double* const* src_bar, const double* const* src,
+                               int n_src) {
+  if (!op || !rbar || !src_bar) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int st = (int)op->o.hyp;
+  int rc;
+  if ((rc = check_src(src, n_src, st == 1 ? 4 : (st == 2 ? 7 : 9), "vjp_rhs_velocity_fixed"))) return rc;
+  for (int i = 0; i < n_src; ++i)
+    if (!src_bar[i]) return fail(AFL_ERR_INVALID_ARGUMENT, "vjp_rhs_velocity_fixed: null output vector");
+  launch_vjp_rhs_u(op->o, st, rbar, src, src_bar);
+  return check_device("vjp_rhs_velocity_fixed");
+}
+
+int afl_vjp_rhs_pressure(afl_operator* op, const double* rbar, double* const* src_bar, const double* const* src,
+                         int n_src) {
+  if (!op || !rbar || !src_bar) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int st = (int)op->o.hyp;
+  int rc;
+  if ((rc = check_src(src, n_src, st == 1 ? 5 : (st == 2 ? 8 : 11), "vjp_rhs_pressure"))) return rc;
+  for (int i = 0; i < n_src; ++i)
+    if (!src_bar[i]) return fail(AFL_ERR_INVALID_ARGUMENT, "vjp_rhs_pressure: null output vector");
+  launch_vjp_rhs_p(op->o, st, rbar, src, src_bar);
+  return check_device("vjp_rhs_pressure");
+}
+
+int afl_vmult_pressure_transpose(afl_operator* op, double* dst, const double* src) {
+  if (!op || !dst || !src) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_apply_Bt(op->o, src, dst);
+  return check_device("vmult_pressure_transpose");
+}
+
+int afl_vmult_enthalpy_tangent(afl_operator* op, double* dst, const double* dw, const double* w0) {
+  if (!op || !dst || !dw || !w0) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_apply_C_tangent(op->o, op->p_fix.p, dw, w0, dst);
+  return check_device("vmult_enthalpy_tangent");
+}
+
+int afl_vmult_enthalpy_tangent_transpose(afl_operator* op, double* dst, const double* cbar, const double* w0) {
+  if (!op || !dst || !cbar || !w0) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_apply_C_tangent_t(op->o, op->p_fix.p, cbar, w0, dst);
+  return check_device("vmult_enthalpy_tangent_transpose");
+}
+
+namespace {
+// S'(p0) x = D x - C'(w0) A^{-1} B x, or its transpose D x - B^T A^{-1} C'(w0)^T x, w0 = A^{-1} B p0
+int schur_tangent(afl_operator* op, double* dst, const double* src, const double* p0, bool transpose) {
+  if (!op || !dst || !src || !p0) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int64_t N = op->N(), NU = N * op->dim();
+  int rc;
+  if ((rc = op->adjW0.alloc(NU)) || (rc = op->adjT1.alloc(NU)) || (rc = op->adjT2.alloc(NU)) ||
+      (rc = op->adjC.alloc(N)))
+    return rc;
+  const OpCtx& o = op->o;
+  launch_schur_w(o, op->rho_fix.p, p0, op->adjW0.p);
+  if (!transpose) {
+    launch_schur_w(o, op->rho_fix.p, src, op->adjT2.p);
+    launch_apply_C_tangent(o, op->p_fix.p, op->adjT2.p, op->adjW0.p, op->adjC.p);
+  } else {
+    launch_apply_C_tangent_t(o, op->p_fix.p, src, op->adjW0.p, op->adjT1.p);
+    launch_mass_solve(o, op->rho_fix.p, 1.0, op->dim(), op->adjT1.p, op->adjT2.p);
+    launch_apply_Bt(o, op->adjT2.p, op->adjC.p);
+  }
+  launch_mass_apply(o, nullptr, 1.0 / (kGamma - 1.0), 1, src, dst);
+  vec_axpby(o.stream, N, -1.0, op->adjC.p, 1.0, dst);
+  return AFL_OK;
+}
+}  // namespace
+
+int afl_vmult_schur_tangent(afl_operator* op, double* dst, const double* src, const double* p0) {
+  int rc;
+  if ((rc = schur_tangent(op, dst, src, p0, false))) return rc;
+  return check_device("vmult_schur_tangent");
+}
+
+int afl_vmult_schur_tangent_transpose(afl_operator* op, double* dst, const double* src, const double* p0) {
+  int rc;
+  if ((rc = schur_tangent(op, dst, src, p0, true))) return rc;
+  return check_device("vmult_schur_tangent_transpose");
 }
 
 int afl_mg_vmult(afl_operator* op, double* dst, const double* src) {
@@ -1149,6 +1256,213 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
     launch_sponge(o, op->sponge.z_start, op->sponge.alpha_max * o.dt, op->sponge.u_ref, u);
   if (info) *info = inf;
   return check_device("imex_step");
+}
+
+// ---- adjoint of one IMEX step ----------------------------------------------------------------
+// Reverse of afl_imex_step: the forward step is recomputed from (rho, u, p) to restore the stage
+// vectors, then every stage is differentiated backwards. The implicit stages are differentiated at
+// their converged fixed point (p*, u*) by the implicit-function theorem applied to
+//   R_u = A(rho_new) u* + B p* - E2(x) = 0,
+//   R_p = D p* - C(A^{-1} B p*; p*) + C(A^{-1} E2(x); p*) - E3(x, rho_new, u*) = 0,
+// the equations the fixed-point loop of Atmospheric_Flow.cc:545-781 converges to (C is applied to the
+// two velocities separately, as the loop does). The transposed system is reduced to the pressure
+// multiplier, Z lam_p = pbar - B^T A^{-1} ubar with
+//   Z = D - B^T A^{-1} C'(w_B)^T - C_pf(w_B)^T + C_pf(w_E)^T + B^T A^{-1} E3_u^T,
+// solved by GMRES with the Jacobi(D) preconditioner, and lam_u = A^{-1} (ubar + E3_u^T lam_p).
+extern "C++" {
+namespace {
+
+struct StepAdj {
+  afl_operator* op;
+  int64_t N, NU;
+  double* b[afl_operator::kAdjBufs];
+  hipStream_t st;
+  void acc(double* dst, const double* src, int64_t n) { vec_axpby(st, n, 1.0, src, 1.0, dst); }
+};
+
+// adjoint of one converged implicit stage; tgt[i] receives the adjoint of E3 source i (i < n3 - 1,
+// E2 source i for i < n2); the E3 source n3 - 1 is u_fix = u*. rnew_pos: position of rho_new.
+int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, const double* us, const double* pbar,
+               const double* ubar, const double* const* e2s, int n2, const double* const* e3s, int n3,
+               double* const* tgt, int rnew_pos) {
+  afl_operator* op = S.op;
+  OpCtx& o = op->o;
+  const int64_t N = S.N, NU = S.NU;
+  hipStream_t st = S.st;
+  double* const* B = S.b;
+  double *wB = B[22], *wE = B[23], *lamp = B[24], *lamu = B[25], *t1 = B[26], *t2 = B[27], *t3 = B[28], *r = B[30];
+  double* outs[11];
+  for (int i = 0; i < 11; ++i) outs[i] = B[9 + i];
+  o.hyp = (unsigned)stage;
+  vec_copy(st, N, rho_new, op->rho_fix.p);
+  vec_copy(st, N, ps, op->p_fix.p);
+  op->mg_coef_stale = true;
+  op->mg_eig_stale = true;
+  const TableauC& t = o.tab;
+  const double Ma2 = o.Ma * o.Ma, iFr2 = 1.0 / (o.Fr * o.Fr);
+  const double cgz = (stage == 1 ? t.at22 : t.at33) * o.dt * Ma2 * iFr2;  // gravity weight of (rho_new, u_fix)
+  launch_schur_w(o, op->rho_fix.p, ps, wB);
+  launch_rhs_u(o, stage, e2s, t1);
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, wE);
+  // r = pbar - B^T A^{-1} ubar
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, ubar, t1);
+  launch_apply_Bt(o, t1, r);
+  vec_axpby(st, N, 1.0, pbar, -1.0, r);
+  auto Z = [&](const double* in, double* out) {
+    launch_apply_C_tangent_t(o, op->p_fix.p, in, wB, t1);
+    launch_scaled_velocity_mass(o, in, rho_new, us, -Ma2, -cgz, t2);
+    vec_axpby(st, NU, -1.0, t2, 1.0, t1);
+    launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, t2);
+    launch_apply_Bt(o, t2, out);
+    launch_mass_apply(o, nullptr, 1.0 / (kGamma - 1.0), 1, in, t3);
+    vec_axpby(st, N, 1.0, t3, -1.0, out);
+    launch_apply_C_coef_t(o, wB, in, t3);
+    vec_axpby(st, N, -1.0, t3, 1.0, out);
+    launch_apply_C_coef_t(o, wE, in, t3);
+    vec_axpby(st, N, 1.0, t3, 1.0, out);
+  };
+  int rc;
+  if ((rc = ensure_common(op))) return rc;
+  vec_norm2(st, N, r, op->red.p, op->dsc.p + kScA);
+  if ((rc = fetch(op, kScA, 1))) return rc;
+  const double tol = op->sp.gmres_eps * op->host_sc[kScA];
+  vec_fill(st, N, 0.0, lamp);
+  int it = 0;
+  double res = 0.0;
+  if (op->host_sc[kScA] > 0.0 &&
+      (rc = gmres_op(op, Z, lamp, r, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res,
+                     "adjoint stage GMRES did not converge")))
+    return rc;
+  // lam_u = A^{-1} (ubar + E3_u^T lam_p)
+  launch_scaled_velocity_mass(o, lamp, rho_new, us, -Ma2, -cgz, t1);
+  vec_axpby(st, NU, 1.0, ubar, 1.0, t1);
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, lamu);
+  // E2 sources: VJP of lam_u - A^{-1} C'(w_E)^T lam_p
+  launch_apply_C_tangent_t(o, op->p_fix.p, lamp, wE, t1);
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, t2);  // t2 = A^{-1} C'(w_E)^T lam_p (kept below)
+  vec_copy(st, NU, lamu, B[29]);
+  vec_axpby(st, NU, -1.0, t2, 1.0, B[29]);
+  launch_vjp_rhs_u(o, stage, B[29], e2s, outs);
+  for (int i = 0; i < n2; ++i) S.acc(tgt[i], outs[i], i % 3 == 1 ? NU : N);
+  // rho_new through A(rho_new): + int phi (A^{-1} C'(w_E)^T lam_p) . w_E - int phi lam_u . u*
+  launch_dot_mass(o, t2, wE, t3);
+  S.acc(tgt[rnew_pos], t3, N);
+  launch_dot_mass(o, lamu, us, t3);
+  vec_axpby(st, N, -1.0, t3, 1.0, tgt[rnew_pos]);
+  //                     - int phi (A^{-1} C'(w_B)^T lam_p) . w_B
+  launch_apply_C_tangent_t(o, op->p_fix.p, lamp, wB, t1);
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, t2);
+  launch_dot_mass(o, t2, wB, t3);
+  vec_axpby(st, N, -1.0, t3, 1.0, tgt[rnew_pos]);
+  // E3 sources except u_fix: VJP of lam_p
+  launch_vjp_rhs_p(o, stage, lamp, e3s, outs);
+  for (int i = 0; i < n3 - 1; ++i) {
+    const bool vel = i < 3 * stage ? (i % 3 == 1) : false;
+    S.acc(tgt[i], outs[i], vel ? NU : N);
+  }
+  return AFL_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, const double* p, const double* rho_bar_out,
+                      const double* u_bar_out, const double* p_bar_out, double* rho_bar, double* u_bar, double* p_bar,
+                      afl_step_info* info) {
+  if (!op || !rho || !u || !p || !rho_bar_out || !u_bar_out || !p_bar_out || !rho_bar || !u_bar || !p_bar)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  OpCtx& o = op->o;
+  const int64_t N = o.N, NU = N * o.dim;
+  hipStream_t st = o.stream;
+  int rc;
+  for (DevBuf& b : op->aj)
+    if ((rc = b.alloc(NU))) return rc;
+  StepAdj S{op, N, NU, {}, st};
+  for (int i = 0; i < afl_operator::kAdjBufs; ++i) S.b[i] = op->aj[i].p;
+  double* const* B = S.b;
+  // forward recompute from copies of the input state: restores the stage vectors
+  vec_copy(st, N, rho, B[31]);
+  vec_copy(st, NU, u, B[32]);
+  vec_copy(st, N, p, B[33]);
+  if ((rc = afl_imex_step(op, B[31], B[32], B[33], info))) return rc;
+  const double *r2 = op->rho2.p, *u2 = op->u2.p, *p2 = op->p2.p, *r3 = op->rho3.p, *u3 = op->u3.p, *p3 = op->p3.p;
+  const double *rc_ = op->rhoc.p, *uc = op->uc.p;
+  double *rb2 = B[0], *ub2 = B[1], *pb2 = B[2], *rb3 = B[3], *ub3 = B[4], *pb3 = B[5], *rbc = B[6], *ubc = B[7],
+         *pbc = B[8], *e = B[20], *eU = B[21], *t3 = B[28];
+  double* outs[11];
+  for (int i = 0; i < 11; ++i) outs[i] = B[9 + i];
+  vec_fill(st, N, 0.0, rho_bar);
+  vec_fill(st, NU, 0.0, u_bar);
+  vec_fill(st, N, 0.0, p_bar);
+  for (double* z : {rb2, pb2, rb3, pb3}) vec_fill(st, N, 0.0, z);
+  for (double* z : {ub2, ub3}) vec_fill(st, NU, 0.0, z);
+  vec_copy(st, N, rho_bar_out, rbc);
+  vec_copy(st, NU, u_bar_out, ubc);
+  vec_copy(st, N, p_bar_out, pbc);
+  if (op->sponge.enabled) {  // u <- (u + a u_ref) / (1 + a): the transpose divides by 1 + a
+    const double zero[3] = {0.0, 0.0, 0.0};
+    launch_sponge(o, op->sponge.z_start, op->sponge.alpha_max * o.dt, zero, ubc);
+  }
+  auto acc = [&](double* dst, const double* src, int64_t n) { S.acc(dst, src, n); };
+  // ---- stage 3: pc = D^{-1} E3, uc = A(rho_c)^{-1} E2, rho_c = M^{-1} E1
+  o.hyp = 3;
+  vec_copy(st, N, rc_, op->rho_fix.p);
+  op->mg_coef_stale = true;
+  op->mg_eig_stale = true;
+  launch_mass_solve(o, nullptr, 1.0 / (kGamma - 1.0), 1, pbc, e);
+  {
+    const double* s11[11] = {rho, u, p, r2, u2, p2, r3, u3, p3, rc_, uc};
+    launch_vjp_rhs_p(o, 3, e, s11, outs);
+    double* tg[11] = {rho_bar, u_bar, p_bar, rb2, ub2, pb2, rb3, ub3, pb3, rbc, ubc};
+    for (int i = 0; i < 11; ++i) acc(tg[i], outs[i], (i % 3 == 1 && i < 9) || i == 10 ? NU : N);
+  }
+  launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, ubc, eU);
+  launch_dot_mass(o, eU, uc, t3);
+  vec_axpby(st, N, -1.0, t3, 1.0, rbc);
+  {
+    const double* s9[9] = {rho, u, p, r2, u2, p2, r3, u3, p3};
+    launch_vjp_rhs_u(o, 3, eU, s9, outs);
+    double* tg[9] = {rho_bar, u_bar, p_bar, rb2, ub2, pb2, rb3, ub3, pb3};
+    for (int i = 0; i < 9; ++i) acc(tg[i], outs[i], i % 3 == 1 ? NU : N);
+  }
+  launch_mass_solve(o, nullptr, 1.0, 1, rbc, e);
+  {
+    const double* s6[6] = {rho, u, r2, u2, r3, u3};
+    launch_vjp_rhs_rho(o, 3, e, s6, outs);
+    double* tg[6] = {rho_bar, u_bar, rb2, ub2, rb3, ub3};
+    for (int i = 0; i < 6; ++i) acc(tg[i], outs[i], i % 2 == 1 ? NU : N);
+  }
+  // ---- stage 2: fixed point (p3, u3) with rho_new = rho3, then rho3 = M^{-1} E1
+  {
+    const double* e2s[7] = {rho, u, p, r2, u2, p2, r3};
+    const double* e3s[8] = {rho, u, p, r2, u2, p2, r3, u3};
+    double* tg[7] = {rho_bar, u_bar, p_bar, rb2, ub2, pb2, rb3};
+    if ((rc = fp_adjoint(S, 2, r3, p3, u3, pb3, ub3, e2s, 7, e3s, 8, tg, 6))) return rc;
+  }
+  o.hyp = 2;
+  launch_mass_solve(o, nullptr, 1.0, 1, rb3, e);
+  {
+    const double* s4[4] = {rho, u, r2, u2};
+    launch_vjp_rhs_rho(o, 2, e, s4, outs);
+    double* tg[4] = {rho_bar, u_bar, rb2, ub2};
+    for (int i = 0; i < 4; ++i) acc(tg[i], outs[i], i % 2 == 1 ? NU : N);
+  }
+  // ---- stage 1: fixed point (p2, u2) with rho_new = rho2, then rho2 = M^{-1} E1
+  {
+    const double* e2s[4] = {rho, u, p, r2};
+    const double* e3s[5] = {rho, u, p, r2, u2};
+    double* tg[4] = {rho_bar, u_bar, p_bar, rb2};
+    if ((rc = fp_adjoint(S, 1, r2, p2, u2, pb2, ub2, e2s, 4, e3s, 5, tg, 3))) return rc;
+  }
+  o.hyp = 1;
+  launch_mass_solve(o, nullptr, 1.0, 1, rb2, e);
+  {
+    const double* s2[2] = {rho, u};
+    launch_vjp_rhs_rho(o, 1, e, s2, outs);
+    acc(rho_bar, outs[0], N);
+    acc(u_bar, outs[1], NU);
+  }
+  return check_device("imex_step_vjp");
 }
 
 int afl_init_hill_state(afl_operator* op, const afl_hill_state* s, double* rho, double* u, double* p) {

@@ -199,6 +199,22 @@ struct LineGeo {
       Rf[2] = (-dx * hy * G[0] - dy * hx * G[1] + hx * hy * G[DIM - 1]) * w;
     }
   }
+  // transpose of gref: reference gradient gr -> physical gradient * JxW (P = Mq^T gr)
+  __device__ void gref_t(const Geo& g, const TabT<R>& T, int qz, const R* gr, R* P) const {
+    const R zeta = zeta0 + T.QP[qz] * hz;
+    const R fz = R(1) - zeta * iH;
+    const R w = wxy * T.W[qz];
+    if (DIM == 2) {
+      const R dx = dhx * fz * hx;
+      P[0] = (gr[0] * c - dx * gr[1]) * w;
+      P[1] = hx * gr[1] * w;
+    } else {
+      const R dx = dhx * fz * hx, dy = dhy * fz * hy;
+      P[0] = (gr[0] * c - dx * gr[2]) * hy * w;
+      P[1] = (gr[1] * c - dy * gr[2]) * hx * w;
+      P[2] = hx * hy * gr[2] * w;
+    }
+  }
 };
 
 // Geometry of face f = 2*dir + side of cell ic at face point fq (2D: fq = q; 3D: fq = qa + NQ qb,
@@ -361,6 +377,83 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
 #pragma unroll
         for (int i = 0; i < N1; ++i) a += T.S[q][i] * c[i];
         v[q] = a;
+      }
+    }
+    cell_sync<DIM, K>();
+  }
+}
+
+// Reference-coordinate gradient of a scalar field at the quadrature points of the thread's line:
+// gr[b][q] = d/dxi_b (D along axis b, S along the others). Work area per cell: 2D 2*N1*NQ,
+// 3D 2*N1*N1*NQ + 3*N1*NQ*NQ values from t1 on (no t2). Contains barriers.
+template <int DIM, int K, typename R = double>
+__device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __restrict__ f, const Slot<DIM, K>& s, R* t1,
+                                      const TabT<R>& T, R (&gr)[DIM][2 * K + 1]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ;
+  const R* u = f + (s.act ? s.cell : 0) * C::NPC;
+  constexpr int X1 = (DIM == 2) ? N1 * NQ : N1 * N1 * NQ;  // one x-sweep array
+  R* tS = t1;
+  R* tD = t1 + X1;
+  if (s.act && s.l < C::ROWS) {
+    R d[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      R a = 0.0, b = 0.0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) { a += T.S[q][i] * d[i]; b += T.D[q][i] * d[i]; }
+      tS[s.l * NQ + q] = a;
+      tD[s.l * NQ + q] = b;
+    }
+  }
+  cell_sync<DIM, K>();
+  if (DIM == 2) {
+    if (s.act) {
+      R cs[N1], cd[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) { cs[i] = tS[i * NQ + s.l]; cd[i] = tD[i * NQ + s.l]; }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        R a = 0.0, b = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) { a += T.S[q][i] * cd[i]; b += T.D[q][i] * cs[i]; }
+        gr[0][q] = a;
+        gr[DIM - 1][q] = b;
+      }
+    }
+    cell_sync<DIM, K>();
+  } else {
+    constexpr int Y1 = N1 * NQ * NQ;
+    R* uSS = t1 + 2 * X1;  // S_x S_y: for the z derivative
+    R* uSD = uSS + Y1;     // S_x D_y: y derivative
+    R* uDS = uSD + Y1;     // D_x S_y: x derivative
+    if (s.act && s.l < NQ * N1) {
+      const int qx = s.l % NQ, iz = s.l / NQ;
+      R cs[N1], cd[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) { cs[i] = tS[(iz * N1 + i) * NQ + qx]; cd[i] = tD[(iz * N1 + i) * NQ + qx]; }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        R a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) { a += T.S[q][i] * cs[i]; b += T.D[q][i] * cs[i]; c += T.S[q][i] * cd[i]; }
+        const int o = (iz * NQ + q) * NQ + qx;
+        uSS[o] = a; uSD[o] = b; uDS[o] = c;
+      }
+    }
+    cell_sync<DIM, K>();
+    if (s.act) {
+      R a0[N1], a1[N1], a2[N1];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) { a0[i] = uDS[i * NQ * NQ + s.l]; a1[i] = uSD[i * NQ * NQ + s.l]; a2[i] = uSS[i * NQ * NQ + s.l]; }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        R x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) { x += T.S[q][i] * a0[i]; y += T.S[q][i] * a1[i]; z += T.D[q][i] * a2[i]; }
+        gr[0][q] = x; gr[1][q] = y; gr[DIM - 1][q] = z;
       }
     }
     cell_sync<DIM, K>();

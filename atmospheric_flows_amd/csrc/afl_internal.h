@@ -58,6 +58,19 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst);
 void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst);
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x);
 void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* w);
+// discrete-adjoint building blocks (afl_adjoint.hip)
+void launch_vjp_rhs_rho(const OpCtx& o, int stage, const double* rbar, const double* const* src, double* const* out);
+void launch_vjp_rhs_u(const OpCtx& o, int stage, const double* rbar, const double* const* src, double* const* out);
+void launch_vjp_rhs_p(const OpCtx& o, int stage, const double* rbar, const double* const* src, double* const* out);
+void launch_apply_Bt(const OpCtx& o, const double* y, double* dst);
+void launch_apply_C_coef_t(const OpCtx& o, const double* w, const double* cb, double* dst);
+// dst = int phi (a . b) over DIM-vector fields a, b
+void launch_dot_mass(const OpCtx& o, const double* a, const double* b, double* dst);
+// dst_d = int phi lam rho (alpha u_d + beta [d = vertical])
+void launch_scaled_velocity_mass(const OpCtx& o, const double* lam, const double* rho, const double* u, double alpha,
+                                 double beta, double* dst);
+void launch_apply_C_tangent(const OpCtx& o, const double* pf, const double* dw, const double* w0, double* dst);
+void launch_apply_C_tangent_t(const OpCtx& o, const double* pf, const double* cb, const double* w0, double* dst);
 // Multigrid level operator in precision R. sinv != nullptr selects the mean-density block
 // solve (w = P^{-1} B p, sinv per cell from launch_cell_rho_mass_inv); otherwise the cell-local
 // A solve runs to tol with rho.

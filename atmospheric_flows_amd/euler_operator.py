@@ -64,6 +64,13 @@ def load_library():
     L.afl_kernel_timer_stop.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.afl_compute_diagnostics.argtypes = [vp, dp, dp, dp, C.POINTER(Diagnostics)]
     L.afl_l2_difference.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    for n in ("afl_vjp_rhs_rho_update", "afl_vjp_rhs_velocity_fixed", "afl_vjp_rhs_pressure"):
+        getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
+    L.afl_vmult_pressure_transpose.argtypes = [vp, dp, dp]
+    L.afl_imex_step_vjp.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.POINTER(StepInfo)]
+    for n in ("afl_vmult_enthalpy_tangent", "afl_vmult_enthalpy_tangent_transpose", "afl_vmult_schur_tangent",
+              "afl_vmult_schur_tangent_transpose"):
+        getattr(L, n).argtypes = [vp, dp, dp, dp]
     for n in ("afl_pack_dofs", "afl_unpack_dofs"):
         getattr(L, n).argtypes = [vp, C.c_int, dp, dp, C.c_int64, C.c_int64, C.c_int64, dp]
     _lib = L
@@ -153,6 +160,59 @@ class EulerOperator:
     def vmult_schur_velocity(self, w, p):
         self._chk(self.L.afl_vmult_schur_velocity(self.h, _ptr(w), _ptr(p)))
         return w
+
+    # ---- discrete-adjoint building blocks -------------------------------------------------------
+    def _vjp(self, fn, rbar, srcs, outs):
+        for t in list(srcs) + list(outs):
+            _ptr(t)
+        if len(outs) != len(srcs):
+            raise AflError("vector-Jacobian product: one output per source")
+        sarr = (C.c_void_p * len(srcs))(*[t.data_ptr() for t in srcs])
+        oarr = (C.c_void_p * len(outs))(*[t.data_ptr() for t in outs])
+        self._chk(fn(self.h, _ptr(rbar), oarr, sarr, len(srcs)))
+        return outs
+
+    def vjp_rhs_rho_update(self, rbar, srcs, outs):
+        """outs[i] = (d rhs_rho / d srcs[i])^T rbar for the E1 sources of the current stage."""
+        return self._vjp(self.L.afl_vjp_rhs_rho_update, rbar, srcs, outs)
+
+    def vjp_rhs_pressure(self, rbar, srcs, outs):
+        """outs[i] = (d rhs_p / d srcs[i])^T rbar for the E3 sources of the current stage."""
+        return self._vjp(self.L.afl_vjp_rhs_pressure, rbar, srcs, outs)
+
+    def vjp_rhs_velocity_fixed(self, rbar, srcs, outs):
+        """outs[i] = (d rhs_u / d srcs[i])^T rbar for the E2 sources of the current stage."""
+        return self._vjp(self.L.afl_vjp_rhs_velocity_fixed, rbar, srcs, outs)
+
+    def imex_step_vjp(self, rho, u, p, rho_bar_out, u_bar_out, p_bar_out):
+        """Adjoint of one IMEX step at the input state (rho, u, p): returns (rho_bar, u_bar, p_bar),
+        the transposed step Jacobian applied to the output adjoint, and the recomputed step's info."""
+        import torch
+        rb, ub, pb = torch.empty_like(rho), torch.empty_like(u), torch.empty_like(p)
+        info = StepInfo()
+        self._chk(self.L.afl_imex_step_vjp(self.h, _ptr(rho), _ptr(u), _ptr(p), _ptr(rho_bar_out), _ptr(u_bar_out),
+                                           _ptr(p_bar_out), _ptr(rb), _ptr(ub), _ptr(pb), C.byref(info)))
+        return rb, ub, pb, info
+
+    def vmult_pressure_transpose(self, dst, src):
+        self._chk(self.L.afl_vmult_pressure_transpose(self.h, _ptr(dst), _ptr(src)))
+        return dst
+
+    def vmult_enthalpy_tangent(self, dst, dw, w0):
+        self._chk(self.L.afl_vmult_enthalpy_tangent(self.h, _ptr(dst), _ptr(dw), _ptr(w0)))
+        return dst
+
+    def vmult_enthalpy_tangent_transpose(self, dst, cbar, w0):
+        self._chk(self.L.afl_vmult_enthalpy_tangent_transpose(self.h, _ptr(dst), _ptr(cbar), _ptr(w0)))
+        return dst
+
+    def vmult_schur_tangent(self, dst, src, p0):
+        self._chk(self.L.afl_vmult_schur_tangent(self.h, _ptr(dst), _ptr(src), _ptr(p0)))
+        return dst
+
+    def vmult_schur_tangent_transpose(self, dst, src, p0):
+        self._chk(self.L.afl_vmult_schur_tangent_transpose(self.h, _ptr(dst), _ptr(src), _ptr(p0)))
+        return dst
 
     def mg_vmult(self, dst, src):
         """Schur preconditioner apply (p-multigrid V-cycle of the selected schur_precond)."""

@@ -94,6 +94,49 @@ int afl_mg_vmult(afl_operator* op, double* dst, const double* src);
  * FP32 copies of rho_for_fixed (requires schur_precond = AFL_SCHUR_MG_FP32). */
 int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p);
 
+/* ---- discrete-adjoint building blocks ------------------------------------------------------
+ * The reference has no adjoint; these are the "adjoint entry points: transpose applies" that a
+ * gradient of a final-time misfit with respect to the initial state is assembled from. Each is
+ * the exact transpose (or tangent) of the linearisation of the forward operator named. */
+/* E1 vector-Jacobian product (vmult_rhs_rho_update, euler_operator.h:342-712):
+ * src_bar[i] = (d rhs_rho / d src[i])^T rbar for the E1 sources of the current HYPERBOLIC_stage
+ * in their positional order (rho_j: N, u_j: dim*N). Outputs are overwritten. */
+int afl_vjp_rhs_rho_update(afl_operator* op, const double* rbar, double* const* src_bar, const double* const* src,
+                           int n_src);
+/* E2 vector-Jacobian product (vmult_rhs_velocity_fixed, euler_operator.h:1620-2325): rbar has dim*N
+ * entries; src_bar[i] matches src[i] (rho, p: N; u: dim*N), E2 positional order of the stage. */
+int afl_vjp_rhs_velocity_fixed(afl_operator* op, const double* rbar, double* const* src_bar, const double* const* src,
+                               int n_src);
+/* E3 vector-Jacobian product (vmult_rhs_pressure, euler_operator.h:743-1381): rbar has N entries;
+ * src_bar[i] matches src[i] in the E3 positional order of the stage. Sources that alias one vector
+ * (u_old read twice at stage 2, rho_new / u_fix read by the correction and by gravity) get the
+ * adjoint of every use at their position. */
+int afl_vjp_rhs_pressure(afl_operator* op, const double* rbar, double* const* src_bar, const double* const* src,
+                         int n_src);
+/* B^T (vmult_pressure, :2420-2435): dim*N -> N. */
+int afl_vmult_pressure_transpose(afl_operator* op, double* dst, const double* src);
+/* Tangent of vmult_enthalpy (:2438-2453) at w0 with the current pres_fixed: C'(w0) dw, N <- dim*N.
+ * C is nonlinear in w through the Rusanov speed max(|w+.n|, |w-.n|); at ties the subgradient is
+ * split evenly between the two traces. */
+int afl_vmult_enthalpy_tangent(afl_operator* op, double* dst, const double* dw, const double* w0);
+/* C'(w0)^T cbar: N -> dim*N. */
+int afl_vmult_enthalpy_tangent_transpose(afl_operator* op, double* dst, const double* cbar, const double* w0);
+/* Tangent of the Schur apply (apply_add NS 2, :2374-2408) at p0: S'(p0) x = D x - C'(w0) A^{-1} B x
+ * with w0 = A^{-1} B p0, and its transpose D x - B^T A^{-1} C'(w0)^T x (rho_for_fixed, pres_fixed). */
+int afl_vmult_schur_tangent(afl_operator* op, double* dst, const double* src, const double* p0);
+int afl_vmult_schur_tangent_transpose(afl_operator* op, double* dst, const double* src, const double* p0);
+
+/* Adjoint of one IMEX step (afl_imex_step, i.e. EulerSolver::run's step, Atmospheric_Flow.cc:1110-1251):
+ * given the step's input state (rho, u, p) and the adjoint of its output (rho_bar_out, ...), writes
+ * the adjoint of its input (rho_bar, ...): the transpose of the step's Jacobian applied to the output
+ * adjoint. Recomputes the forward step internally (info: that step's solver counts), then runs the
+ * stages backwards; the implicit stages are differentiated at their converged fixed point and need
+ * one GMRES solve each (gmres_eps, gmres_restart of the solver parameters; Jacobi(D) preconditioner).
+ * Uses rho_for_fixed, pres_fixed and the stage as scratch, like afl_imex_step. */
+int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, const double* p, const double* rho_bar_out,
+                      const double* u_bar_out, const double* p_bar_out, double* rho_bar, double* u_bar, double* p_bar,
+                      afl_step_info* info);
+
 /* Benchmark instrumentation, no counterpart in the reference (deal.II's TimerOutput times whole
  * phases, not kernels): HIP events recorded on the operator's stream around every launch of one
  * multigrid level kernel (enum afl_timed_kernel) until afl_kernel_timer_stop, at most max_launches

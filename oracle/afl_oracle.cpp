@@ -293,6 +293,28 @@ struct orc_operator {
       ext[ax] = nq; in = o;
     }
   }
+  // physical gradient of a scalar field at the cell quadrature points: out[d*nqc + q]
+  // (reference derivatives D along one axis and S along the others, then J^{-T});
+  // the transpose of the gradient part of integrate_cell without JxW.
+  void eval_cell_grad(const double* field, int c, double* out, Scratch& s) const {
+    const double* u = field + (int64_t)c * npc;
+    std::vector<double> ref((size_t)dim * nqc), a((size_t)std::max(nqc, npc) * 2), b2((size_t)std::max(nqc, npc) * 2);
+    for (int bd = 0; bd < dim; ++bd) {
+      int ext[3] = {n1, n1, n1};
+      const double* in = u;
+      for (int ax = 0; ax < dim; ++ax) {
+        double* o = (ax == dim - 1) ? ref.data() + (size_t)bd * nqc : ((ax & 1) ? b2.data() : a.data());
+        apply_axis(dim, ext, ax, (ax == bd ? B.D : B.S).data(), nq, n1, in, o, false, false);
+        ext[ax] = nq; in = o;
+      }
+    }
+    for (int q = 0; q < nqc; ++q)
+      for (int d = 0; d < dim; ++d) {
+        double v = 0.0;
+        for (int bd = 0; bd < dim; ++bd) v += s.Jinv[(size_t)q * 9 + bd * 3 + d] * ref[(size_t)bd * nqc + q];
+        out[(size_t)d * nqc + q] = v;
+      }
+  }
   // dst[cell] += sum_q JxW (V phi_i + Gphys . grad phi_i). V, G may be null.
   // G layout: G[d*nqc + q] physical components.
   void integrate_cell(const double* V, const double* G, int c, double* dst, Scratch& s) const {
@@ -795,6 +817,486 @@ void apply_C(const Op& op, double* dst, const double* src) {
   });
 }
 
+// ---------------------------------------------------------------------------------------
+// Adjoint (transpose) operators. The reference has none: these are the exact transposes of
+// the linearisations of the forward weak forms above (E1, B, C, S), the building blocks of
+// a discrete adjoint. Each is pinned by the tests against finite differences of the forward
+// oracle (tangents) and by <y, L x> = <L^T y, x> (transposes).
+
+// Rusanov speed lambda = max(|a|, |b|): d lambda = wa da + wb db. At a tie the subgradient is
+// split evenly, so the two cells of a face take the same one.
+inline void lam_weights(double a, double b, double& wa, double& wb) {
+  const double fa = std::fabs(a), fb = std::fabs(b);
+  const double sa = a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0), sb = b > 0.0 ? 1.0 : (b < 0.0 ? -1.0 : 0.0);
+  if (fa > fb) { wa = sa; wb = 0.0; }
+  else if (fa < fb) { wa = 0.0; wb = sb; }
+  else { wa = 0.5 * sa; wb = 0.5 * sb; }
+}
+
+// E1 adjoint: sbar[i] = (d rhs_rho / d src[i])^T rbar for the sources of assemble_rhs_rho
+// (euler_operator.h:342-473 cell, 476-692 faces), in the same order (rho_j, u_j).
+int vjp_rhs_rho(Op& op, const double* rbar, double* const* sbar, const double* const* src, int n_src) {
+  const int st = op.hyp;
+  const int need = st == 1 ? 2 : (st == 2 ? 4 : 6);
+  if (n_src != need) { g_err = "vjp_rhs_rho_update: wrong number of source vectors"; return 1; }
+  double w[3];
+  int ns;
+  if (st == 1) { w[0] = op.T.a21; ns = 1; }
+  else if (st == 2) { w[0] = op.T.a31; w[1] = op.T.a32; ns = 2; }
+  else { w[0] = op.T.b1; w[1] = op.T.b2; w[2] = op.T.b3; ns = 3; }
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const int64_t N = op.N;
+  const double dt = op.dt;
+  for (int j = 0; j < ns; ++j) {
+    std::fill(sbar[2 * j], sbar[2 * j] + N, 0.0);
+    std::fill(sbar[2 * j + 1], sbar[2 * j + 1] + (int64_t)dim * N, 0.0);
+  }
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> R((size_t)nqc), GR((size_t)nqc * dim), rho((size_t)nqc), u((size_t)nqc * dim),
+        V((size_t)nqc);
+    op.eval_cell(rbar, c, R.data(), s);
+    op.eval_cell_grad(rbar, c, GR.data(), s);
+    for (int j = 0; j < ns; ++j) {
+      const double cj = w[j] * dt;
+      op.eval_cell(src[2 * j], c, rho.data(), s);
+      eval_cell_vec(op, src[2 * j + 1], c, u.data(), s);
+      for (int q = 0; q < nqc; ++q) {
+        double v = j == 0 ? R[q] : 0.0;
+        for (int d = 0; d < dim; ++d) v += cj * u[(size_t)d * nqc + q] * GR[(size_t)d * nqc + q];
+        V[q] = v;
+      }
+      op.integrate_cell(V.data(), nullptr, c, sbar[2 * j], s);
+      for (int d = 0; d < dim; ++d) {
+        for (int q = 0; q < nqc; ++q) V[q] = cj * rho[q] * GR[(size_t)d * nqc + q];
+        op.integrate_cell(V.data(), nullptr, c, sbar[2 * j + 1] + (int64_t)d * N, s);
+      }
+    }
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> Ra((size_t)nfq), Rb((size_t)nfq), ra((size_t)nfq), rb((size_t)nfq), ua((size_t)nfq * dim),
+        ub((size_t)nfq * dim), Ga((size_t)nfq), Gb((size_t)nfq), Ua((size_t)nfq * dim), Ub((size_t)nfq * dim);
+    op.eval_face(rbar, cm, dir, 1, Ra.data());
+    op.eval_face(rbar, cp, dir, 0, Rb.data());
+    for (int j = 0; j < ns; ++j) {
+      const double cj = w[j] * dt;
+      op.eval_face(src[2 * j], cm, dir, 1, ra.data());
+      op.eval_face(src[2 * j], cp, dir, 0, rb.data());
+      eval_face_vec(op, src[2 * j + 1], cm, dir, 1, ua.data());
+      eval_face_vec(op, src[2 * j + 1], cp, dir, 0, ub.data());
+      for (int q = 0; q < nfq; ++q) {
+        const double* n = &s.fn[(size_t)q * 3];
+        double uan = 0.0, ubn = 0.0;
+        for (int d = 0; d < dim; ++d) { uan += ua[(size_t)d * nfq + q] * n[d]; ubn += ub[(size_t)d * nfq + q] * n[d]; }
+        const double lam = std::max(std::fabs(uan), std::fabs(ubn));
+        double wa, wb;
+        lam_weights(uan, ubn, wa, wb);
+        // <rbar, faces> = sum_q JxW_f F (R_b - R_a), F = c (1/2 (ra ua + rb ub).n + 1/2 lam (ra - rb))
+        const double W = cj * (Rb[q] - Ra[q]);
+        Ga[q] = W * 0.5 * (uan + lam);
+        Gb[q] = W * 0.5 * (ubn - lam);
+        const double jr = ra[q] - rb[q];
+        for (int d = 0; d < dim; ++d) {
+          Ua[(size_t)d * nfq + q] = W * 0.5 * (ra[q] + jr * wa) * n[d];
+          Ub[(size_t)d * nfq + q] = W * 0.5 * (rb[q] + jr * wb) * n[d];
+        }
+      }
+      op.integrate_face(Ga.data(), s.fJxW.data(), cm, dir, 1, sbar[2 * j]);
+      op.integrate_face(Gb.data(), s.fJxW.data(), cp, dir, 0, sbar[2 * j]);
+      for (int d = 0; d < dim; ++d) {
+        op.integrate_face(Ua.data() + (size_t)d * nfq, s.fJxW.data(), cm, dir, 1, sbar[2 * j + 1] + (int64_t)d * N);
+        op.integrate_face(Ub.data() + (size_t)d * nfq, s.fJxW.data(), cp, dir, 0, sbar[2 * j + 1] + (int64_t)d * N);
+      }
+    }
+  });
+  return 0;
+}
+
+// E2 adjoint: sbar[i] = (d rhs_u / d src[i])^T rbar for the sources of assemble_rhs_u
+// (euler_operator.h:1620-1886 cell, 1889-2184 faces, 2187-2305 walls), same order and sizes.
+int vjp_rhs_u(Op& op, const double* rbar, double* const* sbar, const double* const* src, int n_src) {
+  const int st = op.hyp;
+  const int need = st == 1 ? 4 : (st == 2 ? 7 : 9);
+  if (n_src != need) { g_err = "vjp_rhs_velocity_fixed: wrong number of source vectors"; return 1; }
+  const Tableau& T = op.T;
+  int nexp, ng;
+  double w[3], wp[3], gw[3];
+  if (st == 1) { nexp = 1; w[0] = T.a21; wp[0] = T.at21; ng = 2; gw[0] = T.at21; gw[1] = T.at22; }
+  else if (st == 2) {
+    nexp = 2; w[0] = T.a31; w[1] = T.a32; wp[0] = T.at31; wp[1] = T.at32;
+    ng = 3; gw[0] = T.at31; gw[1] = T.at32; gw[2] = T.at33;
+  } else {
+    nexp = 3; w[0] = T.b1; w[1] = T.b2; w[2] = T.b3; wp[0] = T.b1; wp[1] = T.b2; wp[2] = T.b3;
+    ng = 3; gw[0] = T.b1; gw[1] = T.b2; gw[2] = T.b3;
+  }
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const int64_t N = op.N;
+  const double dt = op.dt, iMa2 = 1.0 / (op.Ma * op.Ma), iFr2 = 1.0 / (op.Fr * op.Fr);
+  // source i is a density (i % 3 == 0), a velocity (1) or a pressure (2)
+  for (int i = 0; i < n_src; ++i) std::fill(sbar[i], sbar[i] + (i % 3 == 1 ? (int64_t)dim * N : N), 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> R((size_t)nqc * dim), P((size_t)nqc * dim * dim), rho((size_t)nqc), u((size_t)nqc * dim),
+        p((size_t)nqc), V((size_t)nqc);
+    eval_cell_vec(op, rbar, c, R.data(), s);
+    for (int a0 = 0; a0 < dim; ++a0) op.eval_cell_grad(rbar + (int64_t)a0 * N, c, P.data() + (size_t)a0 * dim * nqc, s);
+    auto Pab = [&](int a0, int b, int q) { return P[((size_t)a0 * dim + b) * nqc + q]; };
+    for (int j = 0; j < nexp; ++j) {
+      const double cj = w[j] * dt, cpj = wp[j] * dt * iMa2;
+      op.eval_cell(src[3 * j], c, rho.data(), s);
+      eval_cell_vec(op, src[3 * j + 1], c, u.data(), s);
+      // density: j == 0 momentum rho u . R, flux u (x) u : grad R
+      for (int q = 0; q < nqc; ++q) {
+        double v = 0.0;
+        for (int a0 = 0; a0 < dim; ++a0) {
+          if (j == 0) v += u[(size_t)a0 * nqc + q] * R[(size_t)a0 * nqc + q];
+          for (int b = 0; b < dim; ++b) v += cj * u[(size_t)a0 * nqc + q] * u[(size_t)b * nqc + q] * Pab(a0, b, q);
+        }
+        V[q] = v;
+      }
+      op.integrate_cell(V.data(), nullptr, c, sbar[3 * j], s);
+      // velocity component cc
+      for (int cc = 0; cc < dim; ++cc) {
+        for (int q = 0; q < nqc; ++q) {
+          double v = j == 0 ? rho[q] * R[(size_t)cc * nqc + q] : 0.0;
+          for (int b = 0; b < dim; ++b)
+            v += cj * rho[q] * (u[(size_t)b * nqc + q] * Pab(cc, b, q) + u[(size_t)b * nqc + q] * Pab(b, cc, q));
+          V[q] = v;
+        }
+        op.integrate_cell(V.data(), nullptr, c, sbar[3 * j + 1] + (int64_t)cc * N, s);
+      }
+      // pressure: divergence of R
+      for (int q = 0; q < nqc; ++q) {
+        double v = 0.0;
+        for (int a0 = 0; a0 < dim; ++a0) v += cpj * Pab(a0, a0, q);
+        V[q] = v;
+      }
+      op.integrate_cell(V.data(), nullptr, c, sbar[3 * j + 2], s);
+    }
+    // gravity densities src[3g]
+    for (int g = 0; g < ng; ++g) {
+      for (int q = 0; q < nqc; ++q) V[q] = -gw[g] * dt * iFr2 * R[(size_t)(dim - 1) * nqc + q];
+      op.integrate_cell(V.data(), nullptr, c, sbar[3 * g], s);
+    }
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> Ra((size_t)nfq * dim), Rb((size_t)nfq * dim), ra((size_t)nfq), rb((size_t)nfq), pa((size_t)nfq),
+        pb((size_t)nfq), ua((size_t)nfq * dim), ub((size_t)nfq * dim), Ga((size_t)nfq), Gb((size_t)nfq), Ha((size_t)nfq),
+        Hb((size_t)nfq), Ua((size_t)nfq * dim), Ub((size_t)nfq * dim);
+    eval_face_vec(op, rbar, cm, dir, 1, Ra.data());
+    eval_face_vec(op, rbar, cp, dir, 0, Rb.data());
+    for (int j = 0; j < nexp; ++j) {
+      const double cj = w[j] * dt, cpj = wp[j] * dt * iMa2;
+      op.eval_face(src[3 * j], cm, dir, 1, ra.data()); op.eval_face(src[3 * j], cp, dir, 0, rb.data());
+      eval_face_vec(op, src[3 * j + 1], cm, dir, 1, ua.data()); eval_face_vec(op, src[3 * j + 1], cp, dir, 0, ub.data());
+      op.eval_face(src[3 * j + 2], cm, dir, 1, pa.data()); op.eval_face(src[3 * j + 2], cp, dir, 0, pb.data());
+      for (int q = 0; q < nfq; ++q) {
+        const double* n = &s.fn[(size_t)q * 3];
+        double W[3] = {0, 0, 0}, uav[3] = {0, 0, 0}, ubv[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+          W[d] = Rb[(size_t)d * nfq + q] - Ra[(size_t)d * nfq + q];
+          uav[d] = ua[(size_t)d * nfq + q]; ubv[d] = ub[(size_t)d * nfq + q];
+        }
+        const double una = dotv(uav, n, dim), unb = dotv(ubv, n, dim);
+        const double lam = std::max(std::fabs(una), std::fabs(unb));
+        double la, lb;
+        lam_weights(una, unb, la, lb);
+        const double Wn = dotv(W, n, dim), Wua = dotv(W, uav, dim), Wub = dotv(W, ubv, dim);
+        double Wj = 0.0;  // W . jump(rho u)
+        for (int d = 0; d < dim; ++d) Wj += W[d] * (ra[q] * uav[d] - rb[q] * ubv[d]);
+        Ga[q] = cj * 0.5 * (una + lam) * Wua;
+        Gb[q] = cj * 0.5 * (unb - lam) * Wub;
+        Ha[q] = cpj * 0.5 * Wn;
+        Hb[q] = cpj * 0.5 * Wn;
+        for (int d = 0; d < dim; ++d) {
+          Ua[(size_t)d * nfq + q] = cj * 0.5 * (ra[q] * (W[d] * una + Wua * n[d]) + la * n[d] * Wj + lam * ra[q] * W[d]);
+          Ub[(size_t)d * nfq + q] = cj * 0.5 * (rb[q] * (W[d] * unb + Wub * n[d]) + lb * n[d] * Wj - lam * rb[q] * W[d]);
+        }
+      }
+      op.integrate_face(Ga.data(), s.fJxW.data(), cm, dir, 1, sbar[3 * j]);
+      op.integrate_face(Gb.data(), s.fJxW.data(), cp, dir, 0, sbar[3 * j]);
+      op.integrate_face(Ha.data(), s.fJxW.data(), cm, dir, 1, sbar[3 * j + 2]);
+      op.integrate_face(Hb.data(), s.fJxW.data(), cp, dir, 0, sbar[3 * j + 2]);
+      for (int d = 0; d < dim; ++d) {
+        op.integrate_face(Ua.data() + (size_t)d * nfq, s.fJxW.data(), cm, dir, 1, sbar[3 * j + 1] + (int64_t)d * N);
+        op.integrate_face(Ub.data() + (size_t)d * nfq, s.fJxW.data(), cp, dir, 0, sbar[3 * j + 1] + (int64_t)d * N);
+      }
+    }
+  });
+  op.boundary_loop([&](int c, int dir, int side, Scratch& s) {
+    std::vector<double> Rw((size_t)nfq * dim), H((size_t)nfq);
+    eval_face_vec(op, rbar, c, dir, side, Rw.data());
+    for (int j = 0; j < nexp; ++j) {
+      for (int q = 0; q < nfq; ++q) {
+        double v = 0.0;
+        for (int d = 0; d < dim; ++d) v += Rw[(size_t)d * nfq + q] * s.fn[(size_t)q * 3 + d];
+        H[q] = -wp[j] * dt * iMa2 * v;
+      }
+      op.integrate_face(H.data(), s.fJxW.data(), c, dir, side, sbar[3 * j + 2]);
+    }
+  });
+  return 0;
+}
+
+// E3 adjoint: sbar[i] = (d rhs_p / d src[i])^T rbar for the sources of assemble_rhs_p
+// (euler_operator.h:743-1007 cell, 1010-1361 faces), same order and sizes. With
+// rho (E - 1/2 Ma^2 |u|^2) = p / (gamma - 1) the flux of state j is
+//   G = (1/2 w dt Ma^2 rho |u|^2 + wt dt gamma/(gamma-1) p) u,
+// and every face flux is c1 (avg_kin.n + lam/2 jump_kin) + c2 (avg_ent.n + lam/2 jump_rhoe)
+// with c1 = w dt Ma^2, c2 = wt dt (stage 3's jump of rho E is the sum of the two jumps).
+int vjp_rhs_p(Op& op, const double* rbar, double* const* sbar, const double* const* src, int n_src) {
+  const int st = op.hyp;
+  const int need = st == 1 ? 5 : (st == 2 ? 8 : 11);
+  if (n_src != need) { g_err = "vjp_rhs_pressure: wrong number of source vectors"; return 1; }
+  const Tableau& T = op.T;
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const int64_t N = op.N;
+  const double dt = op.dt, Ma2 = op.Ma * op.Ma, iFr2 = 1.0 / (op.Fr * op.Fr), gm1 = kGamma - 1.0;
+  const double gr = kGamma / gm1;
+  int nexp, ki, ng;
+  double w[3], wt[3], gw[3];
+  if (st == 1) { nexp = 1; w[0] = T.a21; wt[0] = T.at21; ki = 3; ng = 2; gw[0] = T.at21; gw[1] = T.at22; }
+  else if (st == 2) {
+    nexp = 2; w[0] = T.a31; wt[0] = T.at31; w[1] = T.a32; wt[1] = T.at32; ki = 6;
+    ng = 3; gw[0] = T.at31; gw[1] = T.at32; gw[2] = T.at33;
+  } else {
+    nexp = 3; w[0] = T.b1; wt[0] = T.b1; w[1] = T.b2; wt[1] = T.b2; w[2] = T.b3; wt[2] = T.b3; ki = 9;
+    ng = 3; gw[0] = T.b1; gw[1] = T.b2; gw[2] = T.b3;
+  }
+  auto is_vel = [&](int i) { return i < 3 * nexp ? (i % 3 == 1) : (i == ki + 1); };
+  for (int i = 0; i < n_src; ++i) std::fill(sbar[i], sbar[i] + (is_vel(i) ? (int64_t)dim * N : N), 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> R((size_t)nqc), GR((size_t)nqc * dim), rho((size_t)nqc), u((size_t)nqc * dim), p((size_t)nqc),
+        Vr((size_t)nqc), Vp((size_t)nqc), Vu((size_t)nqc * dim);
+    op.eval_cell(rbar, c, R.data(), s);
+    op.eval_cell_grad(rbar, c, GR.data(), s);
+    for (int j = 0; j < nexp; ++j) {
+      op.eval_cell(src[3 * j], c, rho.data(), s);
+      eval_cell_vec(op, src[3 * j + 1], c, u.data(), s);
+      op.eval_cell(src[3 * j + 2], c, p.data(), s);
+      for (int q = 0; q < nqc; ++q) {
+        double uv[3], gv[3];
+        for (int d = 0; d < dim; ++d) { uv[d] = u[(size_t)d * nqc + q]; gv[d] = GR[(size_t)d * nqc + q]; }
+        const double uu = dotv(uv, uv, dim), Gu = dotv(uv, gv, dim);
+        const double aj = 0.5 * w[j] * dt * Ma2 * rho[q] * uu + wt[j] * dt * gr * p[q];
+        Vr[q] = 0.5 * w[j] * dt * Ma2 * uu * Gu + (j == 0 ? 0.5 * Ma2 * uu * R[q] : 0.0);
+        Vp[q] = wt[j] * dt * gr * Gu + (j == 0 ? R[q] / gm1 : 0.0);
+        for (int d = 0; d < dim; ++d)
+          Vu[(size_t)d * nqc + q] = aj * gv[d] + w[j] * dt * Ma2 * rho[q] * uv[d] * Gu +
+                                    (j == 0 ? Ma2 * rho[q] * uv[d] * R[q] : 0.0);
+      }
+      op.integrate_cell(Vr.data(), nullptr, c, sbar[3 * j], s);
+      op.integrate_cell(Vp.data(), nullptr, c, sbar[3 * j + 2], s);
+      for (int d = 0; d < dim; ++d) op.integrate_cell(Vu.data() + (size_t)d * nqc, nullptr, c, sbar[3 * j + 1] + (int64_t)d * N, s);
+    }
+    // kinetic-energy correction -1/2 Ma^2 rho |u|^2 of (src[ki], src[ki+1])
+    op.eval_cell(src[ki], c, rho.data(), s);
+    eval_cell_vec(op, src[ki + 1], c, u.data(), s);
+    for (int q = 0; q < nqc; ++q) {
+      double uu = 0.0;
+      for (int d = 0; d < dim; ++d) uu += u[(size_t)d * nqc + q] * u[(size_t)d * nqc + q];
+      Vr[q] = -0.5 * Ma2 * uu * R[q];
+      for (int d = 0; d < dim; ++d) Vu[(size_t)d * nqc + q] = -Ma2 * rho[q] * u[(size_t)d * nqc + q] * R[q];
+    }
+    op.integrate_cell(Vr.data(), nullptr, c, sbar[ki], s);
+    for (int d = 0; d < dim; ++d) op.integrate_cell(Vu.data() + (size_t)d * nqc, nullptr, c, sbar[ki + 1] + (int64_t)d * N, s);
+    // gravity work -gw dt Ma^2/Fr^2 rho u_z of (src[3g], src[3g+1])
+    for (int g = 0; g < ng; ++g) {
+      const double cg = gw[g] * dt * Ma2 * iFr2;
+      op.eval_cell(src[3 * g], c, rho.data(), s);
+      op.eval_cell(src[3 * g + 1] + (int64_t)(dim - 1) * N, c, p.data(), s);
+      for (int q = 0; q < nqc; ++q) { Vr[q] = -cg * p[q] * R[q]; Vp[q] = -cg * rho[q] * R[q]; }
+      op.integrate_cell(Vr.data(), nullptr, c, sbar[3 * g], s);
+      op.integrate_cell(Vp.data(), nullptr, c, sbar[3 * g + 1] + (int64_t)(dim - 1) * N, s);
+    }
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> Ra((size_t)nfq), Rb((size_t)nfq), ra((size_t)nfq), rb((size_t)nfq), pa((size_t)nfq),
+        pb((size_t)nfq), ua((size_t)nfq * dim), ub((size_t)nfq * dim), u0a((size_t)nfq * dim), u0b((size_t)nfq * dim),
+        Gra((size_t)nfq), Grb((size_t)nfq), Gpa((size_t)nfq), Gpb((size_t)nfq), Gua((size_t)nfq * dim),
+        Gub((size_t)nfq * dim), Q0a((size_t)nfq * dim), Q0b((size_t)nfq * dim);
+    op.eval_face(rbar, cm, dir, 1, Ra.data());
+    op.eval_face(rbar, cp, dir, 0, Rb.data());
+    eval_face_vec(op, src[1], cm, dir, 1, u0a.data());
+    eval_face_vec(op, src[1], cp, dir, 0, u0b.data());
+    for (int j = 0; j < nexp; ++j) {
+      const double c1 = w[j] * dt * Ma2, c2 = wt[j] * dt;
+      const bool quirk = (st == 2 && j == 1);  // euler_operator.h:1182-1183: u_tmp_2 . u_old
+      op.eval_face(src[3 * j], cm, dir, 1, ra.data()); op.eval_face(src[3 * j], cp, dir, 0, rb.data());
+      eval_face_vec(op, src[3 * j + 1], cm, dir, 1, ua.data()); eval_face_vec(op, src[3 * j + 1], cp, dir, 0, ub.data());
+      op.eval_face(src[3 * j + 2], cm, dir, 1, pa.data()); op.eval_face(src[3 * j + 2], cp, dir, 0, pb.data());
+      for (int q = 0; q < nfq; ++q) {
+        const double* n = &s.fn[(size_t)q * 3];
+        double uav[3] = {0, 0, 0}, ubv[3] = {0, 0, 0}, zav[3] = {0, 0, 0}, zbv[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+          uav[d] = ua[(size_t)d * nfq + q]; ubv[d] = ub[(size_t)d * nfq + q];
+          zav[d] = u0a[(size_t)d * nfq + q]; zbv[d] = u0b[(size_t)d * nfq + q];
+        }
+        const double uua = dotv(uav, uav, dim), uub = dotv(ubv, ubv, dim);
+        const double una = dotv(uav, n, dim), unb = dotv(ubv, n, dim);
+        const double lam = std::max(std::fabs(una), std::fabs(unb));
+        double la, lb;
+        lam_weights(una, unb, la, lb);
+        // kinetic jump factors: jump_kin = ra ka - rb kb
+        const double ka = quirk ? 0.5 * dotv(uav, zav, dim) : 0.5 * uua;
+        const double kb = quirk ? 0.5 * dotv(ubv, zbv, dim) : 0.5 * uub;
+        const double jk = ra[q] * ka - rb[q] * kb, jre = (pa[q] - pb[q]) / gm1;
+        const double W = Rb[q] - Ra[q];
+        Gra[q] = W * c1 * (0.25 * uua * una + 0.5 * lam * ka);
+        Grb[q] = W * c1 * (0.25 * uub * unb - 0.5 * lam * kb);
+        Gpa[q] = W * c2 * 0.5 * (gr * una + lam / gm1);
+        Gpb[q] = W * c2 * 0.5 * (gr * unb - lam / gm1);
+        for (int d = 0; d < dim; ++d) {
+          const double dka = quirk ? 0.5 * zav[d] : uav[d], dkb = quirk ? 0.5 * zbv[d] : ubv[d];
+          Gua[(size_t)d * nfq + q] =
+              W * (c1 * (0.25 * (2.0 * uav[d] * ra[q] * una + uua * ra[q] * n[d]) + 0.5 * la * n[d] * jk +
+                         0.5 * lam * ra[q] * dka) +
+                   c2 * 0.5 * (gr * pa[q] * n[d] + la * n[d] * jre));
+          Gub[(size_t)d * nfq + q] =
+              W * (c1 * (0.25 * (2.0 * ubv[d] * rb[q] * unb + uub * rb[q] * n[d]) + 0.5 * lb * n[d] * jk -
+                         0.5 * lam * rb[q] * dkb) +
+                   c2 * 0.5 * (gr * pb[q] * n[d] + lb * n[d] * jre));
+          Q0a[(size_t)d * nfq + q] = quirk ? W * c1 * 0.5 * lam * ra[q] * 0.5 * uav[d] : 0.0;
+          Q0b[(size_t)d * nfq + q] = quirk ? -W * c1 * 0.5 * lam * rb[q] * 0.5 * ubv[d] : 0.0;
+        }
+      }
+      op.integrate_face(Gra.data(), s.fJxW.data(), cm, dir, 1, sbar[3 * j]);
+      op.integrate_face(Grb.data(), s.fJxW.data(), cp, dir, 0, sbar[3 * j]);
+      op.integrate_face(Gpa.data(), s.fJxW.data(), cm, dir, 1, sbar[3 * j + 2]);
+      op.integrate_face(Gpb.data(), s.fJxW.data(), cp, dir, 0, sbar[3 * j + 2]);
+      for (int d = 0; d < dim; ++d) {
+        op.integrate_face(Gua.data() + (size_t)d * nfq, s.fJxW.data(), cm, dir, 1, sbar[3 * j + 1] + (int64_t)d * N);
+        op.integrate_face(Gub.data() + (size_t)d * nfq, s.fJxW.data(), cp, dir, 0, sbar[3 * j + 1] + (int64_t)d * N);
+        if (quirk) {
+          op.integrate_face(Q0a.data() + (size_t)d * nfq, s.fJxW.data(), cm, dir, 1, sbar[1] + (int64_t)d * N);
+          op.integrate_face(Q0b.data() + (size_t)d * nfq, s.fJxW.data(), cp, dir, 0, sbar[1] + (int64_t)d * N);
+        }
+      }
+    }
+  });
+  return 0;
+}
+
+// B^T (transpose of apply_B, euler_operator.h:1510-1617): y (dim fields) -> N.
+void apply_Bt(const Op& op, double* dst, const double* y) {
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const int64_t N = op.N;
+  const double coeff = op.hyp == 1 ? op.T.at22 : op.T.at33;
+  const double f = coeff * op.dt / (op.Ma * op.Ma);
+  std::fill(dst, dst + N, 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> GY((size_t)nqc * dim), V((size_t)nqc, 0.0);
+    for (int a0 = 0; a0 < dim; ++a0) {
+      op.eval_cell_grad(y + (int64_t)a0 * N, c, GY.data(), s);
+      for (int q = 0; q < nqc; ++q) V[q] -= f * GY[(size_t)a0 * nqc + q];
+    }
+    op.integrate_cell(V.data(), nullptr, c, dst, s);
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> ya((size_t)nfq * dim), yb((size_t)nfq * dim), F((size_t)nfq);
+    eval_face_vec(op, y, cm, dir, 1, ya.data());
+    eval_face_vec(op, y, cp, dir, 0, yb.data());
+    for (int q = 0; q < nfq; ++q) {
+      double v = 0.0;
+      for (int d = 0; d < dim; ++d) v += (ya[(size_t)d * nfq + q] - yb[(size_t)d * nfq + q]) * s.fn[(size_t)q * 3 + d];
+      F[q] = 0.5 * f * v;
+    }
+    op.integrate_face(F.data(), s.fJxW.data(), cm, dir, 1, dst);
+    op.integrate_face(F.data(), s.fJxW.data(), cp, dir, 0, dst);
+  });
+  op.boundary_loop([&](int c, int dir, int side, Scratch& s) {
+    std::vector<double> yb((size_t)nfq * dim), F((size_t)nfq);
+    eval_face_vec(op, y, c, dir, side, yb.data());
+    for (int q = 0; q < nfq; ++q) {
+      double v = 0.0;
+      for (int d = 0; d < dim; ++d) v += yb[(size_t)d * nfq + q] * s.fn[(size_t)q * 3 + d];
+      F[q] = f * v;
+    }
+    op.integrate_face(F.data(), s.fJxW.data(), c, dir, side, dst);
+  });
+}
+
+// Tangent of apply_C at w0 (with the current p_fix): C'(w0) dw. C depends on w nonlinearly
+// only through lambda = max(|w+.n|, |w-.n|) (euler_operator.h:1411-1507).
+void apply_C_tangent(const Op& op, double* dst, const double* dw, const double* w0) {
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const double coeff = op.hyp == 1 ? op.T.at22 : op.T.at33;
+  const double cdt = coeff * op.dt, gr = kGamma / (kGamma - 1.0), gm1 = kGamma - 1.0;
+  const double* pf = op.p_fix.data();
+  std::fill(dst, dst + op.N, 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> p((size_t)nqc), w((size_t)nqc * dim), G((size_t)nqc * dim);
+    op.eval_cell(pf, c, p.data(), s);
+    eval_cell_vec(op, dw, c, w.data(), s);
+    for (int d = 0; d < dim; ++d)
+      for (int q = 0; q < nqc; ++q) G[(size_t)d * nqc + q] = -cdt * gr * p[q] * w[(size_t)d * nqc + q];
+    op.integrate_cell(nullptr, G.data(), c, dst, s);
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> pa((size_t)nfq), pb((size_t)nfq), da((size_t)nfq * dim), db((size_t)nfq * dim),
+        wa((size_t)nfq * dim), wb((size_t)nfq * dim), Fp((size_t)nfq), Fm((size_t)nfq);
+    op.eval_face(pf, cm, dir, 1, pa.data()); op.eval_face(pf, cp, dir, 0, pb.data());
+    eval_face_vec(op, dw, cm, dir, 1, da.data()); eval_face_vec(op, dw, cp, dir, 0, db.data());
+    eval_face_vec(op, w0, cm, dir, 1, wa.data()); eval_face_vec(op, w0, cp, dir, 0, wb.data());
+    for (int q = 0; q < nfq; ++q) {
+      const double* n = &s.fn[(size_t)q * 3];
+      double avg_n = 0.0, dan = 0.0, dbn = 0.0, wan = 0.0, wbn = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        avg_n += 0.5 * gr * (pa[q] * da[(size_t)d * nfq + q] + pb[q] * db[(size_t)d * nfq + q]) * n[d];
+        dan += da[(size_t)d * nfq + q] * n[d]; dbn += db[(size_t)d * nfq + q] * n[d];
+        wan += wa[(size_t)d * nfq + q] * n[d]; wbn += wb[(size_t)d * nfq + q] * n[d];
+      }
+      double la, lb;
+      lam_weights(wan, wbn, la, lb);
+      const double F = cdt * (avg_n + 0.5 * (la * dan + lb * dbn) * (pa[q] - pb[q]) / gm1);
+      Fp[q] = F; Fm[q] = -F;
+    }
+    op.integrate_face(Fp.data(), s.fJxW.data(), cm, dir, 1, dst);
+    op.integrate_face(Fm.data(), s.fJxW.data(), cp, dir, 0, dst);
+  });
+}
+
+// Transpose of apply_C_tangent: C'(w0)^T cb, N -> dim fields.
+void apply_C_tangent_T(const Op& op, double* dst, const double* cb, const double* w0) {
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const int64_t N = op.N;
+  const double coeff = op.hyp == 1 ? op.T.at22 : op.T.at33;
+  const double cdt = coeff * op.dt, gr = kGamma / (kGamma - 1.0), gm1 = kGamma - 1.0;
+  const double* pf = op.p_fix.data();
+  std::fill(dst, dst + (int64_t)dim * N, 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> p((size_t)nqc), GC((size_t)nqc * dim), V((size_t)nqc);
+    op.eval_cell(pf, c, p.data(), s);
+    op.eval_cell_grad(cb, c, GC.data(), s);
+    for (int d = 0; d < dim; ++d) {
+      for (int q = 0; q < nqc; ++q) V[q] = -cdt * gr * p[q] * GC[(size_t)d * nqc + q];
+      op.integrate_cell(V.data(), nullptr, c, dst + (int64_t)d * N, s);
+    }
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> pa((size_t)nfq), pb((size_t)nfq), Ca((size_t)nfq), Cb((size_t)nfq), wa((size_t)nfq * dim),
+        wb((size_t)nfq * dim), Fa((size_t)nfq * dim), Fb((size_t)nfq * dim);
+    op.eval_face(pf, cm, dir, 1, pa.data()); op.eval_face(pf, cp, dir, 0, pb.data());
+    op.eval_face(cb, cm, dir, 1, Ca.data()); op.eval_face(cb, cp, dir, 0, Cb.data());
+    eval_face_vec(op, w0, cm, dir, 1, wa.data()); eval_face_vec(op, w0, cp, dir, 0, wb.data());
+    for (int q = 0; q < nfq; ++q) {
+      const double* n = &s.fn[(size_t)q * 3];
+      double wan = 0.0, wbn = 0.0;
+      for (int d = 0; d < dim; ++d) { wan += wa[(size_t)d * nfq + q] * n[d]; wbn += wb[(size_t)d * nfq + q] * n[d]; }
+      double la, lb;
+      lam_weights(wan, wbn, la, lb);
+      // <cb, faces> = sum_q JxW_f F (C_a - C_b)
+      const double W = cdt * (Ca[q] - Cb[q]), jump = (pa[q] - pb[q]) / gm1;
+      for (int d = 0; d < dim; ++d) {
+        Fa[(size_t)d * nfq + q] = W * 0.5 * (gr * pa[q] + la * jump) * n[d];
+        Fb[(size_t)d * nfq + q] = W * 0.5 * (gr * pb[q] + lb * jump) * n[d];
+      }
+    }
+    for (int d = 0; d < dim; ++d) {
+      op.integrate_face(Fa.data() + (size_t)d * nfq, s.fJxW.data(), cm, dir, 1, dst + (int64_t)d * N);
+      op.integrate_face(Fb.data() + (size_t)d * nfq, s.fJxW.data(), cp, dir, 0, dst + (int64_t)d * N);
+    }
+  });
+}
+
 // Inverse diagonal by unit-vector probing, euler_operator.h:2456-2626. Per cell and node i:
 // diag_i = sum_q JxW coef phi_i(q)^2 (identical to evaluate(e_i) -> submit -> integrate -> pick i).
 void inverse_diagonal(const Op& op, double* out, int ncomp, double scale, const double* rho) {
@@ -879,6 +1381,27 @@ void schur_apply(Op& op, double* dst, const double* src) {
   solve_cells(op, op.chol_A, w.data(), Bp.data(), op.dim, 1.0);
   apply_C(op, Cw.data(), w.data());
   for (int64_t i = 0; i < N; ++i) dst[i] -= Cw[i];
+}
+
+// Tangent of the Schur apply at p0 (C is nonlinear in w): S'(p0) dp = D dp - C'(w0) A^{-1} B dp
+// with w0 = A^{-1} B p0; its transpose S'(p0)^T y = D y - B^T A^{-1} C'(w0)^T y (A, D symmetric).
+void schur_tangent(Op& op, double* dst, const double* src, const double* p0, bool transpose) {
+  const int64_t N = op.N, NU = (int64_t)op.dim * N;
+  std::vector<double> Bp((size_t)NU), w0((size_t)NU), t1((size_t)NU), t2((size_t)NU), c((size_t)N);
+  ensure_A(op);
+  apply_B(op, Bp.data(), p0);
+  solve_cells(op, op.chol_A, w0.data(), Bp.data(), op.dim, 1.0);
+  apply_mass(op, dst, src, 1, 1.0 / (kGamma - 1.0), nullptr);
+  if (!transpose) {
+    apply_B(op, t1.data(), src);
+    solve_cells(op, op.chol_A, t2.data(), t1.data(), op.dim, 1.0);
+    apply_C_tangent(op, c.data(), t2.data(), w0.data());
+  } else {
+    apply_C_tangent_T(op, t1.data(), src, w0.data());
+    solve_cells(op, op.chol_A, t2.data(), t1.data(), op.dim, 1.0);
+    apply_Bt(op, c.data(), t2.data());
+  }
+  for (int64_t i = 0; i < N; ++i) dst[i] -= c[i];
 }
 
 double dot(const double* a, const double* b, int64_t n) {
@@ -1054,6 +1577,29 @@ int orc_vmult(orc_operator* op, double* dst, const double* src) {
 }
 int orc_vmult_pressure(orc_operator* op, double* dst, const double* src) { apply_B(*op, dst, src); return 0; }
 int orc_vmult_enthalpy(orc_operator* op, double* dst, const double* src) { apply_C(*op, dst, src); return 0; }
+
+int orc_vjp_rhs_rho_update(orc_operator* op, const double* rbar, double* const* sbar, const double* const* src, int n) {
+  return vjp_rhs_rho(*op, rbar, sbar, src, n);
+}
+int orc_vjp_rhs_velocity_fixed(orc_operator* op, const double* rbar, double* const* sbar, const double* const* src, int n) {
+  return vjp_rhs_u(*op, rbar, sbar, src, n);
+}
+int orc_vjp_rhs_pressure(orc_operator* op, const double* rbar, double* const* sbar, const double* const* src, int n) {
+  return vjp_rhs_p(*op, rbar, sbar, src, n);
+}
+int orc_vmult_pressure_transpose(orc_operator* op, double* dst, const double* src) { apply_Bt(*op, dst, src); return 0; }
+int orc_vmult_enthalpy_tangent(orc_operator* op, double* dst, const double* dw, const double* w0) {
+  apply_C_tangent(*op, dst, dw, w0); return 0;
+}
+int orc_vmult_enthalpy_tangent_transpose(orc_operator* op, double* dst, const double* cb, const double* w0) {
+  apply_C_tangent_T(*op, dst, cb, w0); return 0;
+}
+int orc_vmult_schur_tangent(orc_operator* op, double* dst, const double* src, const double* p0) {
+  schur_tangent(*op, dst, src, p0, false); return 0;
+}
+int orc_vmult_schur_tangent_transpose(orc_operator* op, double* dst, const double* src, const double* p0) {
+  schur_tangent(*op, dst, src, p0, true); return 0;
+}
 
 int orc_compute_diagonal(orc_operator* op) {
   const int ns = op->ns;

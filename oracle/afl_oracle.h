@@ -46,6 +46,19 @@ int orc_vmult(orc_operator* op, double* dst, const double* src);          /* NS_
 int orc_vmult_pressure(orc_operator* op, double* dst, const double* src); /* B */
 int orc_vmult_enthalpy(orc_operator* op, double* dst, const double* src); /* C */
 int orc_compute_diagonal(orc_operator* op);
+
+/* Adjoint building blocks (no reference counterpart): transposes of the linearised forward
+ * operators. sbar follows the E1 source order (rho_j: N, u_j: dim*N). */
+int orc_vjp_rhs_rho_update(orc_operator* op, const double* rbar, double* const* sbar, const double* const* src, int n_src);
+int orc_vjp_rhs_velocity_fixed(orc_operator* op, const double* rbar, double* const* sbar, const double* const* src,
+                               int n_src); /* E2; sbar sizes follow the sources (rho, p: N, u: dim*N) */
+int orc_vjp_rhs_pressure(orc_operator* op, const double* rbar, double* const* sbar, const double* const* src,
+                         int n_src); /* E3 */
+int orc_vmult_pressure_transpose(orc_operator* op, double* dst, const double* src);        /* B^T: dim*N -> N */
+int orc_vmult_enthalpy_tangent(orc_operator* op, double* dst, const double* dw, const double* w0);  /* C'(w0) dw */
+int orc_vmult_enthalpy_tangent_transpose(orc_operator* op, double* dst, const double* cb, const double* w0);
+int orc_vmult_schur_tangent(orc_operator* op, double* dst, const double* src, const double* p0);
+int orc_vmult_schur_tangent_transpose(orc_operator* op, double* dst, const double* src, const double* p0);
 int orc_get_matrix_diagonal_inverse(orc_operator* op, double* dst);
 
 int orc_solve_mass(orc_operator* op, double* x, const double* b); /* exact inverse of M / A / D */

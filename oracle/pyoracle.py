@@ -58,6 +58,13 @@ def lib():
         L.orc_node_coords.argtypes = [C.c_void_p, _P]
         L.orc_diagnostics.argtypes = [C.c_void_p, _P, _P, _P, _P]
         L.orc_l2_difference.argtypes = [C.c_void_p, C.c_int, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_vjp_rhs_rho_update.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
+        L.orc_vjp_rhs_velocity_fixed.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
+        L.orc_vjp_rhs_pressure.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
+        L.orc_vmult_pressure_transpose.argtypes = [C.c_void_p, _P, _P]
+        for name in ("orc_vmult_enthalpy_tangent", "orc_vmult_enthalpy_tangent_transpose",
+                     "orc_vmult_schur_tangent", "orc_vmult_schur_tangent_transpose"):
+            getattr(L, name).argtypes = [C.c_void_p, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -174,6 +181,42 @@ class OracleOperator:
         self._chk(self.L.orc_l2_difference(self.h, ncomp, _ptr(np.ascontiguousarray(a)),
                                            None if b is None else _ptr(np.ascontiguousarray(b)), C.byref(e), C.byref(r)))
         return e.value, r.value
+
+    # ---- adjoint building blocks (transposes of the linearised forward operators) ----
+    def _vjp(self, fn, rbar, srcs):
+        srcs = [np.ascontiguousarray(x, dtype=np.float64) for x in srcs]
+        outs = [np.zeros(x.size) for x in srcs]
+        sarr = (_P * len(srcs))(*[_ptr(x) for x in srcs])
+        oarr = (_P * len(outs))(*[_ptr(x) for x in outs])
+        self._chk(fn(self.h, _ptr(np.ascontiguousarray(rbar, dtype=np.float64)), oarr, sarr, len(srcs)))
+        return outs
+
+    def vjp_rhs_rho_update(self, rbar, srcs):
+        """(d rhs_rho / d src_i)^T rbar for each E1 source, in the E1 order."""
+        return self._vjp(self.L.orc_vjp_rhs_rho_update, rbar, srcs)
+
+    def vjp_rhs_pressure(self, rbar, srcs):
+        """(d rhs_p / d src_i)^T rbar for each E3 source, in the E3 order."""
+        return self._vjp(self.L.orc_vjp_rhs_pressure, rbar, srcs)
+
+    def vjp_rhs_velocity_fixed(self, rbar, srcs):
+        """(d rhs_u / d src_i)^T rbar for each E2 source, in the E2 order."""
+        return self._vjp(self.L.orc_vjp_rhs_velocity_fixed, rbar, srcs)
+
+    def vmult_pressure_transpose(self, y): return self._lin(self.L.orc_vmult_pressure_transpose, self.N, y)
+
+    def _lin2(self, fn, n_out, a, b):
+        dst = np.zeros(n_out)
+        self._chk(fn(self.h, _ptr(dst), _ptr(np.ascontiguousarray(a, dtype=np.float64)),
+                     _ptr(np.ascontiguousarray(b, dtype=np.float64))))
+        return dst
+
+    def vmult_enthalpy_tangent(self, dw, w0): return self._lin2(self.L.orc_vmult_enthalpy_tangent, self.N, dw, w0)
+    def vmult_enthalpy_tangent_transpose(self, cb, w0):
+        return self._lin2(self.L.orc_vmult_enthalpy_tangent_transpose, self.dim * self.N, cb, w0)
+    def vmult_schur_tangent(self, dp, p0): return self._lin2(self.L.orc_vmult_schur_tangent, self.N, dp, p0)
+    def vmult_schur_tangent_transpose(self, y, p0):
+        return self._lin2(self.L.orc_vmult_schur_tangent_transpose, self.N, y, p0)
 
     def node_coords(self):
         X = np.zeros(self.dim * self.N)
