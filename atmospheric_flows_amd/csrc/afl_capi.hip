@@ -818,6 +818,29 @@ int afl_vmult_enthalpy_tangent_transpose(afl_operator* op, double* dst, const do
   return check_device("vmult_enthalpy_tangent_transpose");
 }
 
+int afl_vmult_enthalpy_coefficient_transpose(afl_operator* op, double* dst, const double* cbar, const double* w) {
+  if (!op || !dst || !cbar || !w) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_apply_C_coef_t(op->o, w, cbar, dst);
+  return check_device("vmult_enthalpy_coefficient_transpose");
+}
+
+int afl_vjp_rho_for_fixed(afl_operator* op, double* dst, const double* ybar, const double* u) {
+  if (!op || !dst || !ybar || !u) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  launch_dot_mass(op->o, ybar, u, dst);
+  return check_device("vjp_rho_for_fixed");
+}
+
+int afl_vjp_rhs_pressure_u_fixed(afl_operator* op, double* dst, const double* rbar, const double* rho_new,
+                                 const double* u_fixed) {
+  if (!op || !dst || !rbar || !rho_new || !u_fixed) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const OpCtx& o = op->o;
+  if (o.hyp == 3) return fail(AFL_ERR_STATE, "vjp_rhs_pressure_u_fixed: stages 1 and 2 read u_fixed");
+  const double Ma2 = o.Ma * o.Ma, iFr2 = 1.0 / (o.Fr * o.Fr);
+  const double cgz = (o.hyp == 1 ? o.tab.at22 : o.tab.at33) * o.dt * Ma2 * iFr2;
+  launch_scaled_velocity_mass(o, rbar, rho_new, u_fixed, -Ma2, -cgz, dst);
+  return check_device("vjp_rhs_pressure_u_fixed");
+}
+
 namespace {
 // S'(p0) x = D x - C'(w0) A^{-1} B x, or its transpose D x - B^T A^{-1} C'(w0)^T x, w0 = A^{-1} B p0
 int schur_tangent(afl_operator* op, double* dst, const double* src, const double* p0, bool transpose) {

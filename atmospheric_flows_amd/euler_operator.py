@@ -67,8 +67,10 @@ def load_library():
     for n in ("afl_vjp_rhs_rho_update", "afl_vjp_rhs_velocity_fixed", "afl_vjp_rhs_pressure"):
         getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
     L.afl_vmult_pressure_transpose.argtypes = [vp, dp, dp]
+    L.afl_vjp_rhs_pressure_u_fixed.argtypes = [vp, dp, dp, dp, dp]
     L.afl_imex_step_vjp.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.POINTER(StepInfo)]
-    for n in ("afl_vmult_enthalpy_tangent", "afl_vmult_enthalpy_tangent_transpose", "afl_vmult_schur_tangent",
+    for n in ("afl_vmult_enthalpy_coefficient_transpose", "afl_vjp_rho_for_fixed",
+              "afl_vmult_enthalpy_tangent", "afl_vmult_enthalpy_tangent_transpose", "afl_vmult_schur_tangent",
               "afl_vmult_schur_tangent_transpose"):
         getattr(L, n).argtypes = [vp, dp, dp, dp]
     for n in ("afl_pack_dofs", "afl_unpack_dofs"):
@@ -204,6 +206,18 @@ class EulerOperator:
 
     def vmult_enthalpy_tangent_transpose(self, dst, cbar, w0):
         self._chk(self.L.afl_vmult_enthalpy_tangent_transpose(self.h, _ptr(dst), _ptr(cbar), _ptr(w0)))
+        return dst
+
+    def vmult_enthalpy_coefficient_transpose(self, dst, cbar, w):
+        self._chk(self.L.afl_vmult_enthalpy_coefficient_transpose(self.h, _ptr(dst), _ptr(cbar), _ptr(w)))
+        return dst
+
+    def vjp_rho_for_fixed(self, dst, ybar, u):
+        self._chk(self.L.afl_vjp_rho_for_fixed(self.h, _ptr(dst), _ptr(ybar), _ptr(u)))
+        return dst
+
+    def vjp_rhs_pressure_u_fixed(self, dst, rbar, rho_new, u_fixed):
+        self._chk(self.L.afl_vjp_rhs_pressure_u_fixed(self.h, _ptr(dst), _ptr(rbar), _ptr(rho_new), _ptr(u_fixed)))
         return dst
 
     def vmult_schur_tangent(self, dst, src, p0):

@@ -121,6 +121,15 @@ int afl_vmult_pressure_transpose(afl_operator* op, double* dst, const double* sr
 int afl_vmult_enthalpy_tangent(afl_operator* op, double* dst, const double* dw, const double* w0);
 /* C'(w0)^T cbar: N -> dim*N. */
 int afl_vmult_enthalpy_tangent_transpose(afl_operator* op, double* dst, const double* cbar, const double* w0);
+/* Coefficient transposes for the implicit stages: (d C(w; pres_fixed) / d pres_fixed)^T cbar (C is
+ * linear in its coefficient, :1411-1507), N -> N; (d A(rho) u / d rho)^T ybar = int phi ybar . u
+ * (assemble_cell_term_velocity_fixed, :2328-2357), dim*N -> N; and the adjoint of E3 with respect to
+ * u_fixed at stages 1 and 2, whose kinetic-energy correction and gravity work read (rho_new, u_fixed)
+ * (:743-1007), N -> dim*N. */
+int afl_vmult_enthalpy_coefficient_transpose(afl_operator* op, double* dst, const double* cbar, const double* w);
+int afl_vjp_rho_for_fixed(afl_operator* op, double* dst, const double* ybar, const double* u);
+int afl_vjp_rhs_pressure_u_fixed(afl_operator* op, double* dst, const double* rbar, const double* rho_new,
+                                 const double* u_fixed);
 /* Tangent of the Schur apply (apply_add NS 2, :2374-2408) at p0: S'(p0) x = D x - C'(w0) A^{-1} B x
  * with w0 = A^{-1} B p0, and its transpose D x - B^T A^{-1} C'(w0)^T x (rho_for_fixed, pres_fixed). */
 int afl_vmult_schur_tangent(afl_operator* op, double* dst, const double* src, const double* p0);

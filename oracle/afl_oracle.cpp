@@ -1297,6 +1297,60 @@ void apply_C_tangent_T(const Op& op, double* dst, const double* cb, const double
   });
 }
 
+// (d C(w; pf) / d pf)^T cb: C (apply_C) is linear in its coefficient pf for a fixed w.
+void apply_C_coef_T(const Op& op, double* dst, const double* cb, const double* w) {
+  const int dim = op.dim, nqc = op.nqc, nfq = op.nfq;
+  const double coeff = op.hyp == 1 ? op.T.at22 : op.T.at33;
+  const double cdt = coeff * op.dt, gr = kGamma / (kGamma - 1.0), gm1 = kGamma - 1.0;
+  std::fill(dst, dst + op.N, 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> GC((size_t)nqc * dim), wq((size_t)nqc * dim), V((size_t)nqc);
+    op.eval_cell_grad(cb, c, GC.data(), s);
+    eval_cell_vec(op, w, c, wq.data(), s);
+    for (int q = 0; q < nqc; ++q) {
+      double v = 0.0;
+      for (int d = 0; d < dim; ++d) v += wq[(size_t)d * nqc + q] * GC[(size_t)d * nqc + q];
+      V[q] = -cdt * gr * v;
+    }
+    op.integrate_cell(V.data(), nullptr, c, dst, s);
+  });
+  op.face_loop([&](int cm, int cp, int dir, Scratch& s) {
+    std::vector<double> Ca((size_t)nfq), Cb((size_t)nfq), wa((size_t)nfq * dim), wb((size_t)nfq * dim),
+        Ga((size_t)nfq), Gb((size_t)nfq);
+    op.eval_face(cb, cm, dir, 1, Ca.data()); op.eval_face(cb, cp, dir, 0, Cb.data());
+    eval_face_vec(op, w, cm, dir, 1, wa.data()); eval_face_vec(op, w, cp, dir, 0, wb.data());
+    for (int q = 0; q < nfq; ++q) {
+      const double* n = &s.fn[(size_t)q * 3];
+      double wan = 0.0, wbn = 0.0;
+      for (int d = 0; d < dim; ++d) { wan += wa[(size_t)d * nfq + q] * n[d]; wbn += wb[(size_t)d * nfq + q] * n[d]; }
+      const double lam = std::max(std::fabs(wan), std::fabs(wbn));
+      const double W = cdt * (Ca[q] - Cb[q]);
+      Ga[q] = W * 0.5 * (gr * wan + lam / gm1);
+      Gb[q] = W * 0.5 * (gr * wbn - lam / gm1);
+    }
+    op.integrate_face(Ga.data(), s.fJxW.data(), cm, dir, 1, dst);
+    op.integrate_face(Gb.data(), s.fJxW.data(), cp, dir, 0, dst);
+  });
+}
+
+// int phi (a . b) for dim-vector fields: the transpose of rho -> A(rho) b (apply_mass with rho)
+// applied to a.
+void dot_mass(const Op& op, double* dst, const double* a, const double* b) {
+  const int dim = op.dim, nqc = op.nqc;
+  std::fill(dst, dst + op.N, 0.0);
+  op.cell_loop([&](int c, Scratch& s) {
+    std::vector<double> x((size_t)nqc * dim), y((size_t)nqc * dim), V((size_t)nqc);
+    eval_cell_vec(op, a, c, x.data(), s);
+    eval_cell_vec(op, b, c, y.data(), s);
+    for (int q = 0; q < nqc; ++q) {
+      double v = 0.0;
+      for (int d = 0; d < dim; ++d) v += x[(size_t)d * nqc + q] * y[(size_t)d * nqc + q];
+      V[q] = v;
+    }
+    op.integrate_cell(V.data(), nullptr, c, dst, s);
+  });
+}
+
 // Inverse diagonal by unit-vector probing, euler_operator.h:2456-2626. Per cell and node i:
 // diag_i = sum_q JxW coef phi_i(q)^2 (identical to evaluate(e_i) -> submit -> integrate -> pick i).
 void inverse_diagonal(const Op& op, double* out, int ncomp, double scale, const double* rho) {
@@ -1588,6 +1642,12 @@ int orc_vjp_rhs_pressure(orc_operator* op, const double* rbar, double* const* sb
   return vjp_rhs_p(*op, rbar, sbar, src, n);
 }
 int orc_vmult_pressure_transpose(orc_operator* op, double* dst, const double* src) { apply_Bt(*op, dst, src); return 0; }
+int orc_vmult_enthalpy_coefficient_transpose(orc_operator* op, double* dst, const double* cb, const double* w) {
+  apply_C_coef_T(*op, dst, cb, w); return 0;
+}
+int orc_vjp_rho_for_fixed(orc_operator* op, double* dst, const double* a, const double* b) {
+  dot_mass(*op, dst, a, b); return 0;
+}
 int orc_vmult_enthalpy_tangent(orc_operator* op, double* dst, const double* dw, const double* w0) {
   apply_C_tangent(*op, dst, dw, w0); return 0;
 }

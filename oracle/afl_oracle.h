@@ -57,6 +57,8 @@ int orc_vjp_rhs_pressure(orc_operator* op, const double* rbar, double* const* sb
 int orc_vmult_pressure_transpose(orc_operator* op, double* dst, const double* src);        /* B^T: dim*N -> N */
 int orc_vmult_enthalpy_tangent(orc_operator* op, double* dst, const double* dw, const double* w0);  /* C'(w0) dw */
 int orc_vmult_enthalpy_tangent_transpose(orc_operator* op, double* dst, const double* cb, const double* w0);
+int orc_vmult_enthalpy_coefficient_transpose(orc_operator* op, double* dst, const double* cb, const double* w);
+int orc_vjp_rho_for_fixed(orc_operator* op, double* dst, const double* a, const double* b); /* int phi a.b */
 int orc_vmult_schur_tangent(orc_operator* op, double* dst, const double* src, const double* p0);
 int orc_vmult_schur_tangent_transpose(orc_operator* op, double* dst, const double* src, const double* p0);
 int orc_get_matrix_diagonal_inverse(orc_operator* op, double* dst);

@@ -62,7 +62,8 @@ def lib():
         L.orc_vjp_rhs_velocity_fixed.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
         L.orc_vjp_rhs_pressure.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
         L.orc_vmult_pressure_transpose.argtypes = [C.c_void_p, _P, _P]
-        for name in ("orc_vmult_enthalpy_tangent", "orc_vmult_enthalpy_tangent_transpose",
+        for name in ("orc_vmult_enthalpy_coefficient_transpose", "orc_vjp_rho_for_fixed",
+                     "orc_vmult_enthalpy_tangent", "orc_vmult_enthalpy_tangent_transpose",
                      "orc_vmult_schur_tangent", "orc_vmult_schur_tangent_transpose"):
             getattr(L, name).argtypes = [C.c_void_p, _P, _P, _P]
         _lib = L
@@ -214,6 +215,9 @@ class OracleOperator:
     def vmult_enthalpy_tangent(self, dw, w0): return self._lin2(self.L.orc_vmult_enthalpy_tangent, self.N, dw, w0)
     def vmult_enthalpy_tangent_transpose(self, cb, w0):
         return self._lin2(self.L.orc_vmult_enthalpy_tangent_transpose, self.dim * self.N, cb, w0)
+    def vmult_enthalpy_coefficient_transpose(self, cb, w):
+        return self._lin2(self.L.orc_vmult_enthalpy_coefficient_transpose, self.N, cb, w)
+    def vjp_rho_for_fixed(self, a, b): return self._lin2(self.L.orc_vjp_rho_for_fixed, self.N, a, b)
     def vmult_schur_tangent(self, dp, p0): return self._lin2(self.L.orc_vmult_schur_tangent, self.N, dp, p0)
     def vmult_schur_tangent_transpose(self, y, p0):
         return self._lin2(self.L.orc_vmult_schur_tangent_transpose, self.N, y, p0)

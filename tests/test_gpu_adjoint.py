@@ -146,6 +146,35 @@ def test_vjp_rhs_pressure(pair, stage):
 
 
 @pytest.mark.parametrize("stage", [1, 2])
+def test_coefficient_transposes(pair, stage):
+    P = pair
+    pf = P.p * (1 + 1e-3 * P.rng.standard_normal(P.N))
+    P.set_both(stage, p_fix=pf)
+    w, cb = P.rng.standard_normal(P.dim * P.N), P.rng.standard_normal(P.N)
+    ref = P.cpu.vmult_enthalpy_coefficient_transpose(cb, w)
+    out = P.gpu.vmult_enthalpy_coefficient_transpose(P.zeros(P.N), P.dev(cb), P.dev(w))
+    e1 = rel(P.host(out), ref)
+    a, b = P.rng.standard_normal(P.dim * P.N), P.rng.standard_normal(P.dim * P.N)
+    ref = P.cpu.vjp_rho_for_fixed(a, b)
+    out = P.gpu.vjp_rho_for_fixed(P.zeros(P.N), P.dev(a), P.dev(b))
+    e2 = rel(P.host(out), ref)
+    # E3 with respect to u_fixed: the last output of the oracle's E3 adjoint
+    n = {1: 5, 2: 8}[stage]
+    srcs = []
+    for i in range(n):
+        kind = i % 3 if i < 3 * stage else (0 if i == 3 * stage else 1)
+        srcs.append({0: P.rho * (1 + 1e-3 * P.rng.standard_normal(P.N)),
+                     1: P.u + 3.0 * P.rng.standard_normal(P.u.size),
+                     2: P.p * (1 + 1e-4 * P.rng.standard_normal(P.N))}[kind])
+    rbar = P.rng.standard_normal(P.N)
+    ref = P.cpu.vjp_rhs_pressure(rbar, srcs)[-1]
+    out = P.gpu.vjp_rhs_pressure_u_fixed(P.zeros(P.dim * P.N), P.dev(rbar), P.dev(srcs[-2]), P.dev(srcs[-1]))
+    e3 = rel(P.host(out), ref)
+    say(f"  stage {stage}: C_pf^T rel {e1:.2e}; A_rho^T rel {e2:.2e}; E3_u^T rel {e3:.2e}")
+    assert e1 <= TOL and e2 <= TOL and e3 <= TOL
+
+
+@pytest.mark.parametrize("stage", [1, 2])
 def test_pressure_transpose(pair, stage):
     P = pair
     P.set_both(stage)

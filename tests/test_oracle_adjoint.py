@@ -212,3 +212,23 @@ def test_vjp_rhs_pressure_matches_finite_differences(oracle_mod, dim, k, stage):
         li = float(adj[i] @ dirs[i])
         assert abs(li - ri) <= 1e-6 * abs(ri) + 10 * floor, (i, li, ri, floor)
         assert abs(ri) > 1e3 * floor  # the check has resolution
+
+
+@pytest.mark.parametrize("dim,k", [(2, 2), (3, 2)])
+def test_coefficient_transposes(oracle_mod, dim, k):
+    """<cb, C(w; pf)> = <C_pf(w)^T cb, pf> against the forward C; <a, A(rho) b> = <int phi a.b, rho>
+    against the forward A."""
+    op = _op(oracle_mod, dim, k, 1)
+    N = op.N
+    rng = np.random.default_rng(77 + dim)
+    w, cb, pf = rng.standard_normal(dim * N), rng.standard_normal(N), 1e5 * (1 + 0.3 * rng.random(N))
+    op.set_pres_fixed(pf)
+    Cw = op.vmult_enthalpy(w)
+    a1, b1 = float(cb @ Cw), float(op.vmult_enthalpy_coefficient_transpose(cb, w) @ pf)
+    assert abs(a1 - b1) <= 1e-12 * np.linalg.norm(cb) * np.linalg.norm(Cw), (a1, b1)
+    rho, a, b = 1 + 0.2 * rng.random(N), rng.standard_normal(dim * N), rng.standard_normal(dim * N)
+    op.set_rho_for_fixed(rho)
+    op.set_ns_stage(3)
+    Ab = op.vmult(b)
+    a2, b2 = float(a @ Ab), float(op.vjp_rho_for_fixed(a, b) @ rho)
+    assert abs(a2 - b2) <= 1e-12 * np.linalg.norm(a) * np.linalg.norm(Ab), (a2, b2)
