@@ -37,6 +37,11 @@ class StepInfo(C.Structure):
     ]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("forward_steps", C.c_int32), ("adjoint_steps", C.c_int32), ("snapshots_used", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 class Diagnostics(C.Structure):
     _fields_ = [
         ("min_density", C.c_double), ("max_density", C.c_double), ("max_celerity", C.c_double),

@@ -1488,6 +1488,158 @@ int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, cons
   return check_device("imex_step_vjp");
 }
 
+// ---- gradient over many steps with binomial checkpointing ------------------------------------
+// The reverse sweep over steps [a, b) starts from the state at a (held in a snapshot) and s free
+// snapshots: one step is reversed directly; otherwise the state is advanced to a split point m, kept
+// in a snapshot, [m, b) is reversed with s - 1 snapshots and then [a, m) with s. The split minimises
+// the forward steps, T(t, s) = min_m [m + T(t - m, s - 1) + T(m, s)], T(1, s) = 0, T(t, 0) =
+// t (t - 1) / 2 (Griewank's binomial checkpointing; the optimum is found by dynamic programming).
+extern "C++" {
+namespace {
+
+struct Schedule {
+  int T = 0, S = 0;
+  std::vector<int64_t> cost;
+  std::vector<int> split;
+  int64_t& c(int t, int s) { return cost[(size_t)t * (S + 1) + s]; }
+  int& m(int t, int s) { return split[(size_t)t * (S + 1) + s]; }
+  void build(int t_max, int s_max) {
+    T = t_max; S = s_max;
+    cost.assign((size_t)(T + 1) * (S + 1), 0);
+    split.assign((size_t)(T + 1) * (S + 1), 0);
+    for (int t = 1; t <= T; ++t)
+      for (int s = 0; s <= S; ++s) {
+        if (t == 1) { c(t, s) = 0; continue; }
+        if (s == 0) { c(t, s) = (int64_t)t * (t - 1) / 2; continue; }
+        int64_t best = -1;
+        int bm = 1;
+        for (int mm = 1; mm < t; ++mm) {
+          const int64_t v = mm + c(t - mm, s - 1) + c(mm, s);
+          if (best < 0 || v < best) { best = v; bm = mm; }
+        }
+        c(t, s) = best;
+        m(t, s) = bm;
+      }
+  }
+};
+
+struct AdjRun {
+  afl_operator* op;
+  int64_t N, NU;
+  std::vector<DevBuf> snap;  // 3 buffers per slot: rho, u, p
+  DevBuf wr, wu, wp;         // working state
+  DevBuf lam[2][3];          // adjoint ping-pong
+  int cur = 0, used = 0, max_used = 0;
+  std::vector<int> free_slots;
+  Schedule sch;
+  afl_adjoint_stats st{};
+  hipStream_t stream() const { return op->o.stream; }
+  double* sr(int k) { return snap[3 * k].p; }
+  double* su(int k) { return snap[3 * k + 1].p; }
+  double* sp(int k) { return snap[3 * k + 2].p; }
+  int take() {
+    const int k = free_slots.back();
+    free_slots.pop_back();
+    max_used = std::max(max_used, ++used);
+    return k;
+  }
+  void give(int k) { free_slots.push_back(k); --used; }
+  // working state <- state at slot k advanced by n steps
+  int advance(int k, int n) {
+    vec_copy(stream(), N, sr(k), wr.p);
+    vec_copy(stream(), NU, su(k), wu.p);
+    vec_copy(stream(), N, sp(k), wp.p);
+    for (int i = 0; i < n; ++i) {
+      int rc;
+      if ((rc = afl_imex_step(op, wr.p, wu.p, wp.p, nullptr))) return rc;
+      ++st.forward_steps;
+    }
+    return AFL_OK;
+  }
+  int vjp_from_working() {
+    const int nx = 1 - cur;
+    int rc = afl_imex_step_vjp(op, wr.p, wu.p, wp.p, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p, lam[nx][0].p,
+                               lam[nx][1].p, lam[nx][2].p, nullptr);
+    if (rc) return rc;
+    cur = nx;
+    ++st.adjoint_steps;
+    return AFL_OK;
+  }
+  int reverse(int a, int b, int slot, int s) {
+    int rc;
+    const int t = b - a;
+    if (t == 1) {
+      if ((rc = advance(slot, 0))) return rc;
+      return vjp_from_working();
+    }
+    if (s == 0) {
+      for (int k = b - 1; k >= a; --k) {
+        if ((rc = advance(slot, k - a))) return rc;
+        if ((rc = vjp_from_working())) return rc;
+      }
+      return AFL_OK;
+    }
+    const int mm = sch.m(t, s);
+    if ((rc = advance(slot, mm))) return rc;
+    const int k = take();
+    vec_copy(stream(), N, wr.p, sr(k));
+    vec_copy(stream(), NU, wu.p, su(k));
+    vec_copy(stream(), N, wp.p, sp(k));
+    if ((rc = reverse(a + mm, b, k, s - 1))) return rc;
+    give(k);
+    return reverse(a, a + mm, slot, s);
+  }
+};
+
+}  // namespace
+}  // extern "C++"
+
+int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, const double* p0, int nsteps,
+                    int snapshots, const double* rho_bar_T, const double* u_bar_T, const double* p_bar_T,
+                    double* rho_bar_0, double* u_bar_0, double* p_bar_0, afl_adjoint_stats* stats) {
+  if (!op || !rho0 || !u0 || !p0 || !rho_bar_T || !u_bar_T || !p_bar_T || !rho_bar_0 || !u_bar_0 || !p_bar_0)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (nsteps < 1 || snapshots < 0 || snapshots > 4096)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "adjoint_run: nsteps >= 1 and 0 <= snapshots <= 4096");
+  const int64_t N = op->N(), NU = N * op->dim();
+  const int s = std::min(snapshots, nsteps - 1);
+  AdjRun R;
+  R.op = op; R.N = N; R.NU = NU;
+  R.snap.resize((size_t)3 * (s + 1));
+  int rc;
+  for (int k = 0; k <= s; ++k)
+    if ((rc = R.snap[3 * k].alloc(N)) || (rc = R.snap[3 * k + 1].alloc(NU)) || (rc = R.snap[3 * k + 2].alloc(N)))
+      return rc;
+  if ((rc = R.wr.alloc(N)) || (rc = R.wu.alloc(NU)) || (rc = R.wp.alloc(N))) return rc;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = R.lam[i][0].alloc(N)) || (rc = R.lam[i][1].alloc(NU)) || (rc = R.lam[i][2].alloc(N))) return rc;
+  for (int k = s; k >= 1; --k) R.free_slots.push_back(k);
+  R.used = R.max_used = 1;  // slot 0: the initial state
+  hipStream_t st = op->o.stream;
+  vec_copy(st, N, rho0, R.sr(0));
+  vec_copy(st, NU, u0, R.su(0));
+  vec_copy(st, N, p0, R.sp(0));
+  vec_copy(st, N, rho_bar_T, R.lam[0][0].p);
+  vec_copy(st, NU, u_bar_T, R.lam[0][1].p);
+  vec_copy(st, N, p_bar_T, R.lam[0][2].p);
+  R.sch.build(nsteps, s);
+  rc = R.reverse(0, nsteps, 0, s);
+  if (!rc) {
+    vec_copy(st, N, R.lam[R.cur][0].p, rho_bar_0);
+    vec_copy(st, NU, R.lam[R.cur][1].p, u_bar_0);
+    vec_copy(st, N, R.lam[R.cur][2].p, p_bar_0);
+    rc = check_device("adjoint_run");
+  }
+  hipStreamSynchronize(st);
+  R.st.snapshots_used = R.max_used;
+  if (stats) *stats = R.st;
+  for (DevBuf& b : R.snap) b.release();
+  for (DevBuf* b : {&R.wr, &R.wu, &R.wp}) b->release();
+  for (auto& l : R.lam)
+    for (DevBuf& b : l) b.release();
+  return rc;
+}
+
 int afl_init_hill_state(afl_operator* op, const afl_hill_state* s, double* rho, double* u, double* p) {
   if (!op || !s || !rho || !u || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   if (!(s->brunt > 0.0) || !(s->theta0 > 0.0) || !(s->p0 > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "bad background");

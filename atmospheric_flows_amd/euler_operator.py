@@ -14,7 +14,7 @@ Vectors are torch.float64 CUDA tensors in the layout documented in include/afl.h
 import ctypes as C
 import os
 
-from ._abi import MeshDesc, SolverParams, StepInfo, HillState, Sponge, Diagnostics
+from ._abi import MeshDesc, SolverParams, StepInfo, HillState, Sponge, Diagnostics, AdjointStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafl_hip.so")
@@ -68,6 +68,7 @@ def load_library():
         getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
     L.afl_vmult_pressure_transpose.argtypes = [vp, dp, dp]
     L.afl_vjp_rhs_pressure_u_fixed.argtypes = [vp, dp, dp, dp, dp]
+    L.afl_adjoint_run.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(AdjointStats)]
     L.afl_imex_step_vjp.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.POINTER(StepInfo)]
     for n in ("afl_vmult_enthalpy_coefficient_transpose", "afl_vjp_rho_for_fixed",
               "afl_vmult_enthalpy_tangent", "afl_vmult_enthalpy_tangent_transpose", "afl_vmult_schur_tangent",
@@ -195,6 +196,18 @@ class EulerOperator:
         self._chk(self.L.afl_imex_step_vjp(self.h, _ptr(rho), _ptr(u), _ptr(p), _ptr(rho_bar_out), _ptr(u_bar_out),
                                            _ptr(p_bar_out), _ptr(rb), _ptr(ub), _ptr(pb), C.byref(info)))
         return rb, ub, pb, info
+
+    def adjoint_run(self, rho0, u0, p0, nsteps, snapshots, rho_bar_T, u_bar_T, p_bar_T):
+        """Gradient of <xbar_T, x_T> with respect to the initial state over nsteps steps, binomial
+        checkpointing with `snapshots` snapshots. Returns (rho_bar_0, u_bar_0, p_bar_0, stats dict)."""
+        import torch
+        rb, ub, pb = torch.empty_like(rho0), torch.empty_like(u0), torch.empty_like(p0)
+        st = AdjointStats()
+        self._chk(self.L.afl_adjoint_run(self.h, _ptr(rho0), _ptr(u0), _ptr(p0), int(nsteps), int(snapshots),
+                                         _ptr(rho_bar_T), _ptr(u_bar_T), _ptr(p_bar_T), _ptr(rb), _ptr(ub), _ptr(pb),
+                                         C.byref(st)))
+        return rb, ub, pb, {"forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
+                            "snapshots_used": st.snapshots_used}
 
     def vmult_pressure_transpose(self, dst, src):
         self._chk(self.L.afl_vmult_pressure_transpose(self.h, _ptr(dst), _ptr(src)))

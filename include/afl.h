@@ -146,6 +146,16 @@ int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, cons
                       const double* u_bar_out, const double* p_bar_out, double* rho_bar, double* u_bar, double* p_bar,
                       afl_step_info* info);
 
+/* Gradient of a final-time functional with respect to the initial state over nsteps IMEX steps:
+ * (rho_bar_0, ...) = (d x_T / d x_0)^T (rho_bar_T, ...), x_T = nsteps steps of afl_imex_step from
+ * (rho0, u0, p0). States are kept in at most `snapshots` device snapshots besides the initial one
+ * (each 2+dim fields) and recomputed from them with binomial checkpointing (Griewank), with the
+ * split points that minimise the forward steps for that memory. stats may be NULL. No reference
+ * counterpart (the reference has no adjoint). */
+int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, const double* p0, int nsteps,
+                    int snapshots, const double* rho_bar_T, const double* u_bar_T, const double* p_bar_T,
+                    double* rho_bar_0, double* u_bar_0, double* p_bar_0, afl_adjoint_stats* stats);
+
 /* Benchmark instrumentation, no counterpart in the reference (deal.II's TimerOutput times whole
  * phases, not kernels): HIP events recorded on the operator's stream around every launch of one
  * multigrid level kernel (enum afl_timed_kernel) until afl_kernel_timer_stop, at most max_launches

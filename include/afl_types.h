@@ -71,6 +71,14 @@ typedef struct afl_step_info {
   double schur_residual;    /* last GMRES residual estimate */
 } afl_step_info;
 
+/* Counters of an adjoint run with binomial checkpointing (afl_adjoint_run). */
+typedef struct afl_adjoint_stats {
+  int32_t forward_steps;   /* IMEX steps taken to reach checkpoints and re-reach states */
+  int32_t adjoint_steps;   /* afl_imex_step_vjp calls (each also recomputes its forward step) */
+  int32_t snapshots_used;  /* most snapshots held at once (including the initial state) */
+  int32_t pad_;
+} afl_adjoint_stats;
+
 /* Hydrostatic constant-N background plus seeded theta noise. */
 typedef struct afl_hill_state {
   double theta0;      /* surface potential temperature [K] (280) */

@@ -166,3 +166,52 @@ def test_step_vjp_at_size():
         errs.append(abs(fdv - gv) / abs(fdv))
         say(f"  density direction {t}: FD {fdv:.10e} adjoint {gv:.10e} rel {errs[-1]:.2e}")
     assert np.median(errs) <= 1e-4 and max(errs) <= 1e-2
+
+
+def _binomial_cost(t, s, memo={}):
+    """Forward steps of the optimal checkpointed reversal of t steps with s free snapshots."""
+    if t == 1:
+        return 0
+    if s == 0:
+        return t * (t - 1) // 2
+    key = (t, s)
+    if key not in memo:
+        memo[key] = min(m + _binomial_cost(t - m, s - 1) + _binomial_cost(m, s) for m in range(1, t))
+    return memo[key]
+
+
+@pytest.mark.parametrize("snapshots", [0, 1, 2, 7])
+def test_adjoint_run_checkpointing_matches_stored_states(snapshots):
+    """Seven steps reversed with binomial checkpointing give the gradient of the chain of step
+    adjoints over stored states, with the optimal number of recomputed forward steps."""
+    import torch
+    say(f"[gpu adjoint run] 7 steps, {snapshots} snapshots, hill 32x12 k=2")
+    mesh = make_mesh(dim=2, degree=2, n_cells=(32, 1, 12), x_range=(-36e3, 36e3), z_top=12e3,
+                     hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
+    op = EulerOperator(mesh, mach=1.0, froude=FR, dt=2.0)
+    op.set_solver_params(make_solver_params(gmres_eps=1e-12, fp_tol=1e-12))
+    rho, u, p = op.init_hill_state(make_hill_state(theta_sigma=0.01, seed=7))
+    N = op.N
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    rnd = lambda n, s: s * torch.randn(n, dtype=torch.float64, device="cuda", generator=gen)
+    u = u + rnd(2 * N, 0.5)
+    c = (rnd(N, 1.0), rnd(2 * N, 1.0), rnd(N, 1e-5))
+    nsteps = 7
+    xs = [(rho.clone(), u.clone(), p.clone())]
+    for _ in range(nsteps):
+        r, uu, pp = (t.clone() for t in xs[-1])
+        op.imex_step(r, uu, pp)
+        xs.append((r, uu, pp))
+    lam = c
+    for n in reversed(range(nsteps)):
+        lam = op.imex_step_vjp(*xs[n], *lam)[:3]
+    rb, ub, pb, stats = op.adjoint_run(rho, u, p, nsteps, snapshots, *c)
+    say(f"  stats {stats}")
+    for a, b in zip((rb, ub, pb), lam):
+        e = float((a - b).norm() / b.norm())
+        say(f"  checkpointed vs stored: rel {e:.2e}")
+        assert e <= 1e-13
+    s = min(snapshots, nsteps - 1)
+    assert stats["adjoint_steps"] == nsteps
+    assert stats["forward_steps"] == _binomial_cost(nsteps, s)
+    assert stats["snapshots_used"] <= s + 1
