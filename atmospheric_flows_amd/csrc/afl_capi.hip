@@ -440,7 +440,11 @@ int ensure_mg(afl_operator* op) {
 
 // Flexible GMRES (Saad 1993) with the p-multigrid V-cycle as variable right preconditioner:
 // z_j = MG(v_j) is stored, x += Z y. Outer arithmetic, Arnoldi basis and residual in FP64.
-int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+// The operator is a callable apply(in, out); the preconditioner is the multigrid V-cycle of the
+// current solver parameters.
+template <class Apply>
+int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double tol, int max_it, int m, int* iters,
+              double* res_out, const char* what) {
   int rc;
   const int64_t N = op->N();
   hipStream_t st = op->o.stream;
@@ -459,7 +463,7 @@ int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it,
   int it = 0;
   double res = 0.0;
   while (true) {
-    schur_apply(op, x, w);
+    apply(x, w);
     vec_copy(st, N, b, V);
     vec_axpby(st, N, -1.0, w, 1.0, V);
     vec_norm2(st, N, V, op->red.p, sc + kScNrm);
@@ -469,7 +473,7 @@ int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it,
     if (!(beta > tol)) break;
     if (it >= max_it) {
       *iters = it; *res_out = res;
-      return fail(AFL_ERR_NOT_CONVERGED, "Schur FGMRES did not converge");
+      return fail(AFL_ERR_NOT_CONVERGED, what);
     }
     vec_scale_by_inv(st, N, sc + kScNrm, V);
     std::fill(g.begin(), g.end(), 0.0);
@@ -483,7 +487,7 @@ int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it,
       double* vn = V + (int64_t)(j + 1) * N;
       if (f32) op->mg32.apply(op->o, vj, zj);
       else op->mg64.apply(op->o, vj, zj);
-      schur_apply(op, zj, vn);
+      apply(zj, vn);
       vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH1);
       vec_maxpy(st, N, V, N, j + 1, sc + kScH1, -1.0, vn);
       vec_mdot(st, N, V, N, j + 1, vn, op->red.p, sc + kScH2);
@@ -523,12 +527,18 @@ int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it,
     if (conv) break;
     if (it >= max_it) {
       *iters = it; *res_out = res;
-      return fail(AFL_ERR_NOT_CONVERGED, "Schur FGMRES did not converge");
+      return fail(AFL_ERR_NOT_CONVERGED, what);
     }
   }
   *iters = it;
   *res_out = res;
   return check_device("fgmres");
+}
+
+int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+  return fgmres_op(
+      op, [op](const double* in, double* out) { schur_apply(op, in, out); }, x, b, tol, max_it, m, iters, res_out,
+      "Schur FGMRES did not converge");
 }
 
 bool valid_op(afl_operator* op) { return op != nullptr; }
@@ -1352,10 +1362,17 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
   vec_fill(st, N, 0.0, lamp);
   int it = 0;
   double res = 0.0;
-  if (op->host_sc[kScA] > 0.0 &&
-      (rc = gmres_op(op, Z, lamp, r, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res,
-                     "adjoint stage GMRES did not converge")))
-    return rc;
+  // Z differs from the Schur operator's transpose by coefficient terms, and the Schur operator is
+  // nearly symmetric (C = -kappa B^T up to the Rusanov term), so the Schur preconditioner serves Z
+  if (op->host_sc[kScA] > 0.0) {
+    if (op->sp.schur_precond == AFL_SCHUR_JACOBI)
+      rc = gmres_op(op, Z, lamp, r, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res,
+                    "adjoint stage GMRES did not converge");
+    else
+      rc = fgmres_op(op, Z, lamp, r, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res,
+                     "adjoint stage FGMRES did not converge");
+    if (rc) return rc;
+  }
   // lam_u = A^{-1} (ubar + E3_u^T lam_p)
   launch_scaled_velocity_mass(o, lamp, rho_new, us, -Ma2, -cgz, t1);
   vec_axpby(st, NU, 1.0, ubar, 1.0, t1);

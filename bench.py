@@ -200,6 +200,24 @@ def main():
         # algorithmic bytes: each distinct field once (FP32), plus one 4-byte scale per cell
         level[name] = {"launches_per_step": n / args.steps, "avg_us": 1e3 * ms / max(n, 1),
                        "ms_per_step": ms / args.steps, "bytes_per_launch": fields * N * 4 + ncells * 4}
+    # gradient of a final-time functional <c, x_T> with respect to x_0 over the same number of
+    # steps (binomial checkpointing with two snapshots; FP32-preconditioned solves, also in the
+    # adjoint stage solves), against the forward run of those steps
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    cbar = (torch.randn(N, dtype=torch.float64, device="cuda", generator=gen),
+            torch.randn(dim * N, dtype=torch.float64, device="cuda", generator=gen),
+            1e-5 * torch.randn(N, dtype=torch.float64, device="cuda", generator=gen))
+    x0 = tuple(t.clone() for t in s32)
+    adjoint = []
+    for snaps in sorted({min(2, args.steps - 1), args.steps - 1}):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        *_, adj_stats = o32.adjoint_run(*x0, args.steps, snaps, *cbar)
+        torch.cuda.synchronize()
+        t_adj = time.perf_counter() - t0
+        adjoint.append({"steps": args.steps, "snapshots": snaps, "ms": t_adj * 1e3,
+                        "forward_run_ms": step_s * args.steps * 1e3,
+                        "cost_vs_forward_run": t_adj / (step_s * args.steps), "schedule": adj_stats})
     dom = max(level, key=lambda k: level[k]["ms_per_step"])
     bytes_dom = level[dom]["bytes_per_launch"]
     ach = bytes_dom / (level[dom]["avg_us"] * 1e-6) / 1e9
@@ -236,6 +254,7 @@ def main():
                                  "schur_vmult": 9 * field_bytes / t_schur / 1e9,
                                  "schur_velocity_kernel": (2 + dim) * field_bytes / t_sw / 1e9},
         "level_kernels_in_step": level,
+        "adjoint_gradient": adjoint,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": ach, "bytes_per_launch": bytes_dom,
                      "avg_launch_us": level[dom]["avg_us"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
