@@ -193,6 +193,14 @@ struct afl_operator {
   DevBuf adjW0, adjT1, adjT2, adjC;
   static constexpr int kAdjBufs = 34;
   DevBuf aj[kAdjBufs];
+  // observation points (device copies of their location) and point scratch
+  ObsDev obs;
+  DevBuf obs_f;
+  void release_obs() {
+    hipFree(obs.cell); hipFree(obs.ref); hipFree(obs.z); hipFree(obs.ccell); hipFree(obs.cstart); hipFree(obs.order);
+    obs = ObsDev{};
+    obs_f.release();
+  }
   double* host_sc = nullptr;  // pinned scalars
   Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
   KernelTimer timer;          // per-kernel HIP-event timing (afl_kernel_timer_*)
@@ -202,6 +210,7 @@ struct afl_operator {
                      &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix, &adjW0, &adjT1, &adjT2, &adjC};
     for (DevBuf* b : all) b->release();
     for (DevBuf& b : aj) b.release();
+    release_obs();
     if (host_sc) hipHostFree(host_sc);
     if (dtab) hipFree(dtab);
     for (hipEvent_t e : timer.ev) hipEventDestroy(e);
@@ -1655,6 +1664,64 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
   for (auto& l : R.lam)
     for (DevBuf& b : l) b.release();
   return rc;
+}
+
+// ---- observation operator ----------------------------------------------------------------------
+int afl_set_observation_points(afl_operator* op, int npts, const double* xyz) {
+  if (!op || npts < 0 || (npts > 0 && !xyz)) return fail(AFL_ERR_INVALID_ARGUMENT, "bad observation points");
+  std::vector<int64_t> cell;
+  std::vector<double> ref, zpt;
+  const int bad = locate_points(op->o.geo, op->o.dim, npts, xyz, cell, ref, zpt);
+  if (bad >= 0) return fail(AFL_ERR_INVALID_ARGUMENT, "observation point outside the domain");
+  // group the points by cell (stable), for the deterministic transpose
+  std::vector<int> order(npts);
+  for (int k = 0; k < npts; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell[a] < cell[b]; });
+  std::vector<int64_t> ccell;
+  std::vector<int> cstart;
+  for (int q = 0; q < npts; ++q)
+    if (q == 0 || cell[order[q]] != cell[order[q - 1]]) { ccell.push_back(cell[order[q]]); cstart.push_back(q); }
+  cstart.push_back(npts);
+  op->release_obs();
+  ObsDev& ob = op->obs;
+  ob.npts = npts;
+  ob.ncp = (int)ccell.size();
+  auto up = [&](void** dst, const void* src, size_t bytes) -> bool {
+    if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
+    return !bytes || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up((void**)&ob.cell, cell.data(), cell.size() * sizeof(int64_t)) ||
+      !up((void**)&ob.ref, ref.data(), ref.size() * sizeof(double)) ||
+      !up((void**)&ob.z, zpt.data(), zpt.size() * sizeof(double)) ||
+      !up((void**)&ob.ccell, ccell.data(), ccell.size() * sizeof(int64_t)) ||
+      !up((void**)&ob.cstart, cstart.data(), cstart.size() * sizeof(int)) ||
+      !up((void**)&ob.order, order.data(), order.size() * sizeof(int)))
+    return fail(AFL_ERR_DEVICE, "observation points: device copy failed");
+  int rc;
+  if ((rc = op->obs_f.alloc(2 * (int64_t)npts))) return rc;
+  return AFL_OK;
+}
+
+int afl_observe_theta(afl_operator* op, const afl_hill_state* bg, const double* rho, const double* p,
+                      double* theta_prime) {
+  if (!op || !bg || !rho || !p || !theta_prime) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (op->obs.npts == 0) return AFL_OK;
+  launch_observe(op->o, op->obs, rho, p, bg->theta0, bg->brunt * bg->brunt, bg->p0, theta_prime, nullptr, nullptr,
+                 nullptr);
+  return check_device("observe_theta");
+}
+
+int afl_observe_theta_transpose(afl_operator* op, const afl_hill_state* bg, const double* rho, const double* p,
+                                const double* wbar, double* rho_bar, double* p_bar) {
+  if (!op || !bg || !rho || !p || !wbar || !rho_bar || !p_bar) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  vec_fill(op->o.stream, op->N(), 0.0, rho_bar);
+  vec_fill(op->o.stream, op->N(), 0.0, p_bar);
+  if (op->obs.npts == 0) return check_device("observe_theta_transpose");
+  double* fr = op->obs_f.p;
+  double* fp = op->obs_f.p + op->obs.npts;
+  launch_observe(op->o, op->obs, rho, p, bg->theta0, bg->brunt * bg->brunt, bg->p0, nullptr, wbar, fr, fp);
+  launch_observe_t(op->o, op->obs, fr, fp, rho_bar, p_bar);
+  return check_device("observe_theta_transpose");
 }
 
 int afl_init_hill_state(afl_operator* op, const afl_hill_state* s, double* rho, double* u, double* p) {

@@ -58,6 +58,23 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst);
 void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst);
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x);
 void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* w);
+// observation points (afl_observe.hip): located on the host, device copies here
+struct ObsDev {
+  int npts = 0, ncp = 0;            // points; cells that hold points
+  int64_t* cell = nullptr;          // cell of point k
+  double* ref = nullptr;            // reference coordinates, 3 per point
+  double* z = nullptr;              // physical height of point k
+  int64_t* ccell = nullptr;         // cells with points (ncp)
+  int* cstart = nullptr;            // CSR offsets into order (ncp + 1)
+  int* order = nullptr;             // point indices grouped by cell
+};
+int locate_points(const Geo& g, int dim, int npts, const double* xyz, std::vector<int64_t>& cell,
+                  std::vector<double>& ref, std::vector<double>& zpt);
+void launch_observe(const OpCtx& o, const ObsDev& ob, const double* rho, const double* p, double theta0, double n2,
+                    double p0, double* out, const double* wbar, double* fr, double* fp);
+void launch_observe_t(const OpCtx& o, const ObsDev& ob, const double* fr, const double* fp, double* rbar,
+                      double* pbar);
+
 // discrete-adjoint building blocks (afl_adjoint.hip)
 void launch_vjp_rhs_rho(const OpCtx& o, int stage, const double* rbar, const double* const* src, double* const* out);
 void launch_vjp_rhs_u(const OpCtx& o, int stage, const double* rbar, const double* const* src, double* const* out);

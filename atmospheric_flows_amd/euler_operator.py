@@ -68,6 +68,9 @@ def load_library():
         getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
     L.afl_vmult_pressure_transpose.argtypes = [vp, dp, dp]
     L.afl_vjp_rhs_pressure_u_fixed.argtypes = [vp, dp, dp, dp, dp]
+    L.afl_set_observation_points.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.afl_observe_theta.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
+    L.afl_observe_theta_transpose.argtypes = [vp, C.POINTER(HillState), dp, dp, dp, dp, dp]
     L.afl_adjoint_run.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(AdjointStats)]
     L.afl_imex_step_vjp.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.POINTER(StepInfo)]
     for n in ("afl_vmult_enthalpy_coefficient_transpose", "afl_vjp_rho_for_fixed",
@@ -208,6 +211,27 @@ class EulerOperator:
                                          C.byref(st)))
         return rb, ub, pb, {"forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
                             "snapshots_used": st.snapshots_used}
+
+    # ---- observation operator -------------------------------------------------------------------
+    def set_observation_points(self, xyz):
+        """Points as a host array of shape (npts, dim): x[, y], z."""
+        import numpy as np
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.npts = xyz.size // self.dim
+        self._chk(self.L.afl_set_observation_points(self.h, self.npts, xyz.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def observe_theta(self, st, rho, p):
+        import torch
+        out = torch.empty(self.npts, dtype=torch.float64, device="cuda")
+        self._chk(self.L.afl_observe_theta(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(out)))
+        return out
+
+    def observe_theta_transpose(self, st, rho, p, wbar):
+        import torch
+        rb, pb = torch.empty_like(rho), torch.empty_like(p)
+        self._chk(self.L.afl_observe_theta_transpose(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(wbar), _ptr(rb),
+                                                     _ptr(pb)))
+        return rb, pb
 
     def vmult_pressure_transpose(self, dst, src):
         self._chk(self.L.afl_vmult_pressure_transpose(self.h, _ptr(dst), _ptr(src)))

@@ -156,6 +156,19 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
                     int snapshots, const double* rho_bar_T, const double* u_bar_T, const double* p_bar_T,
                     double* rho_bar_0, double* u_bar_0, double* p_bar_0, afl_adjoint_stats* stats);
 
+/* Observation operator (no reference counterpart; the misfit of the gradient configurations):
+ * afl_set_observation_points copies npts points (host array, dim coordinates each: x[, y], z) and
+ * locates them on the mapped mesh (error if one lies outside). afl_observe_theta writes, per point,
+ * theta' = theta(rho, p) - theta0 exp(N^2 z / g) with theta = p / (rho R (p / p0)^(R/cp)), rho and p
+ * evaluated at the point with the cell's Lagrange basis, background from bg (theta0, brunt, p0).
+ * afl_observe_theta_transpose overwrites (rho_bar, p_bar) with the transpose of its linearisation
+ * applied to wbar (device, npts). Deterministic: points are grouped by cell on the host. */
+int afl_set_observation_points(afl_operator* op, int npts, const double* xyz);
+int afl_observe_theta(afl_operator* op, const afl_hill_state* bg, const double* rho, const double* p,
+                      double* theta_prime);
+int afl_observe_theta_transpose(afl_operator* op, const afl_hill_state* bg, const double* rho, const double* p,
+                                const double* wbar, double* rho_bar, double* p_bar);
+
 /* Benchmark instrumentation, no counterpart in the reference (deal.II's TimerOutput times whole
  * phases, not kernels): HIP events recorded on the operator's stream around every launch of one
  * multigrid level kernel (enum afl_timed_kernel) until afl_kernel_timer_stop, at most max_launches

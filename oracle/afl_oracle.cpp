@@ -238,6 +238,8 @@ struct orc_operator {
   afl_solver_params sp{1e-8, 30000, 28, 1e-10, 100, 0};
   afl_sponge sponge{15000.0, 0.0, {0, 0, 0}, 0, 0};
   std::vector<double> full_phi;  // npc x nqc basis table for dense cell matrices
+  std::vector<int> obs_cell;     // observation points: cell, reference coordinates, height
+  std::vector<double> obs_ref, obs_z;
 
   // ---- geometry -------------------------------------------------------------------
   void cell_geometry(int c, Scratch& s) const {
@@ -1781,6 +1783,76 @@ int orc_imex_step(orc_operator* op, double* rho, double* u, double* p, afl_step_
     }
   }
   if (info) *info = inf;
+  return 0;
+}
+
+// Observation operator (test restatement; the reference has no point evaluation): locate a point
+// by inverting z = zeta + h(x, y)(1 - zeta/H) in its column, evaluate rho and p there with the
+// cell's Lagrange basis, theta' = p / (rho R (p/p0)^(R/cp)) - theta0 exp(N^2 z / g).
+int orc_set_observation_points(orc_operator* op, int npts, const double* xyz) {
+  const Mesh& m = op->mesh;
+  const int dim = op->dim, v = dim - 1;
+  op->obs_cell.assign(npts, 0);
+  op->obs_ref.assign((size_t)3 * npts, 0.0);
+  op->obs_z.assign(npts, 0.0);
+  for (int k = 0; k < npts; ++k) {
+    const double* X = xyz + (size_t)dim * k;
+    int ic[3] = {0, 0, 0};
+    double r2 = 0.0;
+    for (int d = 0; d < v; ++d) {
+      const double s = (X[d] - m.lo[d]) / m.hc[d];
+      if (!(s >= 0.0 && s <= m.nc[d])) { g_err = "observation point outside the domain"; return 1; }
+      ic[d] = std::min((int)std::floor(s), m.nc[d] - 1);
+      op->obs_ref[(size_t)3 * k + d] = s - ic[d];
+      r2 += X[d] * X[d];
+    }
+    const double h = m.h0 / (1.0 + r2 / (m.a * m.a));
+    const double s = (X[v] - h) / (1.0 - h / m.H) / m.hc[v];
+    if (!(s >= 0.0 && s <= m.nc[v])) { g_err = "observation point outside the domain"; return 1; }
+    ic[v] = std::min((int)std::floor(s), m.nc[v] - 1);
+    op->obs_ref[(size_t)3 * k + v] = s - ic[v];
+    op->obs_cell[k] = m.cell_of(ic);
+    op->obs_z[k] = X[v];
+  }
+  return 0;
+}
+
+namespace {
+// basis weight of node i of a cell at reference point r
+double point_weight(const orc_operator* op, int i, const double* r) {
+  const int n1 = op->n1;
+  int ii[3] = {i % n1, (i / n1) % n1, i / (n1 * n1)};
+  double w = 1.0;
+  for (int d = 0; d < op->dim; ++d) w *= lagrange(op->B.nodes, ii[d], r[d]);
+  return w;
+}
+}  // namespace
+
+int orc_observe_theta(orc_operator* op, const afl_hill_state* bg, const double* rho, const double* p, double* out,
+                      const double* wbar, double* rho_bar, double* p_bar) {
+  const int npts = (int)op->obs_cell.size();
+  const double kap = (kGamma - 1.0) / kGamma, n2 = bg->brunt * bg->brunt;
+  if (wbar) { std::fill(rho_bar, rho_bar + op->N, 0.0); std::fill(p_bar, p_bar + op->N, 0.0); }
+  for (int k = 0; k < npts; ++k) {
+    const int64_t base = (int64_t)op->obs_cell[k] * op->npc;
+    const double* r = &op->obs_ref[(size_t)3 * k];
+    double rv = 0.0, pv = 0.0;
+    for (int i = 0; i < op->npc; ++i) {
+      const double w = point_weight(op, i, r);
+      rv += w * rho[base + i];
+      pv += w * p[base + i];
+    }
+    const double th = pv / (rv * kR * std::pow(pv / bg->p0, kap));
+    if (out) out[k] = th - bg->theta0 * std::exp(n2 * op->obs_z[k] / kG);
+    if (wbar) {
+      const double fr = -wbar[k] * th / rv, fp = wbar[k] * th * (1.0 - kap) / pv;
+      for (int i = 0; i < op->npc; ++i) {
+        const double w = point_weight(op, i, r);
+        rho_bar[base + i] += w * fr;
+        p_bar[base + i] += w * fp;
+      }
+    }
+  }
   return 0;
 }
 

@@ -75,6 +75,11 @@ int orc_diagnostics(orc_operator* op, const double* rho, const double* u, const 
 int orc_l2_difference(orc_operator* op, int ncomp, const double* a, const double* b, double* err, double* ref);
 int orc_basis_tables(const orc_operator* op, double* nodes, double* qp, double* qw, double* S, double* D);
 int orc_node_coords(const orc_operator* op, double* X); /* dim blocks of N */
+/* observation operator: points (dim coords each), theta' at the points (out may be NULL) and,
+ * when wbar != NULL, the transpose of its linearisation into rho_bar, p_bar (overwritten) */
+int orc_set_observation_points(orc_operator* op, int npts, const double* xyz);
+int orc_observe_theta(orc_operator* op, const afl_hill_state* bg, const double* rho, const double* p, double* out,
+                      const double* wbar, double* rho_bar, double* p_bar);
 
 #ifdef __cplusplus
 }

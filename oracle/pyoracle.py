@@ -59,6 +59,8 @@ def lib():
         L.orc_diagnostics.argtypes = [C.c_void_p, _P, _P, _P, _P]
         L.orc_l2_difference.argtypes = [C.c_void_p, C.c_int, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_vjp_rhs_rho_update.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
+        L.orc_set_observation_points.argtypes = [C.c_void_p, C.c_int, _P]
+        L.orc_observe_theta.argtypes = [C.c_void_p, C.POINTER(HillState), _P, _P, _P, _P, _P, _P]
         L.orc_vjp_rhs_velocity_fixed.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
         L.orc_vjp_rhs_pressure.argtypes = [C.c_void_p, _P, C.POINTER(_P), C.POINTER(_P), C.c_int]
         L.orc_vmult_pressure_transpose.argtypes = [C.c_void_p, _P, _P]
@@ -221,6 +223,24 @@ class OracleOperator:
     def vmult_schur_tangent(self, dp, p0): return self._lin2(self.L.orc_vmult_schur_tangent, self.N, dp, p0)
     def vmult_schur_tangent_transpose(self, y, p0):
         return self._lin2(self.L.orc_vmult_schur_tangent_transpose, self.N, y, p0)
+
+    def set_observation_points(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.npts = xyz.size // self.dim
+        self._chk(self.L.orc_set_observation_points(self.h, self.npts, _ptr(xyz)))
+
+    def observe_theta(self, st, rho, p):
+        out = np.zeros(self.npts)
+        self._chk(self.L.orc_observe_theta(self.h, C.byref(st), _ptr(np.ascontiguousarray(rho)),
+                                           _ptr(np.ascontiguousarray(p)), _ptr(out), None, None, None))
+        return out
+
+    def observe_theta_transpose(self, st, rho, p, wbar):
+        rb, pb = np.zeros(self.N), np.zeros(self.N)
+        self._chk(self.L.orc_observe_theta(self.h, C.byref(st), _ptr(np.ascontiguousarray(rho)),
+                                           _ptr(np.ascontiguousarray(p)), None,
+                                           _ptr(np.ascontiguousarray(wbar, dtype=np.float64)), _ptr(rb), _ptr(pb)))
+        return rb, pb
 
     def node_coords(self):
         X = np.zeros(self.dim * self.N)
