@@ -3,6 +3,21 @@
 // Weak forms follow euler_operator.h; every kernel cites the reference assembler it replaces.
 #include "afl_internal.h"
 
+// Tuning of the FP32 multigrid level kernels (2D): minimum waves per SIMD the compiler must allow
+// (register budget) and whether the C apply evaluates its fields in one batched pass (fewer
+// wavefront syncs) or one at a time (half the LDS per cell). At 512x100, k = 4 the defaults (no
+// register cap, batched) are within 2 % of every combination of 6 or 8 waves and unbatched
+// evaluation: these kernels are not occupancy-bound.
+#ifndef AFL_WLVL_WAVES
+#define AFL_WLVL_WAVES 1
+#endif
+#ifndef AFL_CHEB_WAVES
+#define AFL_CHEB_WAVES 1
+#endif
+#ifndef AFL_C_BATCH_F32
+#define AFL_C_BATCH_F32 1
+#endif
+
 namespace afl {
 
 struct Srcs { const double* p[12]; };
@@ -532,7 +547,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
                                     const R* __restrict__ pD, R scaleD, R (&out)[K + 1]) {
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ;
-  constexpr bool BATCH = sizeof(R) == 4;
+  constexpr bool BATCH = sizeof(R) == 4 && AFL_C_BATCH_F32;
   using LC = LdsC<DIM, K, WITH_D, BATCH>;
   R* t1 = sh;
   R* i1 = t1;
@@ -626,7 +641,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __rest
                                                      const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
                                                      R* __restrict__ dst) {
   const auto& T = *Tp;
-  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4>;
+  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4 && AFL_C_BATCH_F32>;
   __shared__ R shw[Cfg<DIM, K>::CPB * LC::WF];
   Slot<DIM, K> s(g);
   const int cbs = s.cb < Cfg<DIM, K>::CPB ? s.cb : 0;
@@ -642,12 +657,12 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __rest
 // The block preconditioner is cell-local (LDS transposes of the cell's rows); pD, d and x are
 // read only inside the thread's own cell, so d and x (and pD == d or x) update in place.
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) void k_apply_C_cheb(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 && sizeof(R) == 4 ? AFL_CHEB_WAVES : 1))) void k_apply_C_cheb(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
                                                           const R* __restrict__ w, const R* pD, R scaleD,
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
   const auto& T = *Tp;
-  using LC = LdsC<DIM, K, true, sizeof(R) == 4>;
+  using LC = LdsC<DIM, K, true, sizeof(R) == 4 && AFL_C_BATCH_F32>;
   __shared__ R shw[Cfg<DIM, K>::CPB * LC::WF];
   using C = Cfg<DIM, K>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
@@ -1208,7 +1223,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
 // JxW of cell c, refreshed with the stage coefficients). P^{-1} A has its spectrum within the
 // cell's relative variation of rho J about the mean, a few percent.
 template <int DIM, int K, typename R>
-__global__ __launch_bounds__(AFL_NT) void k_schur_w_lvl(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ sinv, R f, int64_t N,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 && sizeof(R) == 4 ? AFL_WLVL_WAVES : 1))) void k_schur_w_lvl(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ sinv, R f, int64_t N,
                                                          const R* __restrict__ p, R* __restrict__ w) {
   const auto& T = *Tp;
   using C = Cfg<DIM, K>;
