@@ -88,7 +88,7 @@ def make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01,
 
 
 def make_solver_params(gmres_eps=1e-10, gmres_max_iter=30000, gmres_restart=28, fp_tol=1e-10, fp_max_iter=100,
-                       schur_precond=0, mg_smoother_degree=3, mg_coarse_degree=16, mg_levels=1,
+                       schur_precond=0, mg_smoother_degree=3, mg_coarse_degree=0, mg_levels=1,
                        mg_smoothing_range=15.0, mg_cell_tol=0.05):
     p = SolverParams()
     p.gmres_eps, p.gmres_max_iter, p.gmres_restart = gmres_eps, gmres_max_iter, gmres_restart

@@ -173,7 +173,7 @@ struct DevBuf {
 struct afl_operator {
   OpCtx o;
   unsigned ns = 1;
-  afl_solver_params sp{1e-8, 30000, 28, 1e-10, 100, 0, AFL_SCHUR_JACOBI, 3, 16, 1, 15.0, 0.05};
+  afl_solver_params sp{1e-8, 30000, 28, 1e-10, 100, 0, AFL_SCHUR_JACOBI, 3, 0, 1, 15.0, 0.05};
   // p-multigrid preconditioner state
   MgT<float> mg32;
   MgT<double> mg64;
@@ -396,10 +396,12 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
 }
 
 // p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
-MgConfig mg_config(const afl_solver_params& sp) {
+// (coarsest-level Chebyshev degree: 16 in 2D, 5 in 3D unless set; on the 3D hill at 48x48x20,
+// k = 3 the solves take one FGMRES iteration from degree 5 up, so a higher degree only costs)
+MgConfig mg_config(const afl_solver_params& sp, int dim) {
   MgConfig c;
   if (sp.mg_smoother_degree > 0) c.smoother_degree = sp.mg_smoother_degree;
-  if (sp.mg_coarse_degree > 0) c.coarse_degree = sp.mg_coarse_degree;
+  c.coarse_degree = sp.mg_coarse_degree > 0 ? sp.mg_coarse_degree : (dim == 2 ? 16 : 5);
   if (sp.mg_levels > 0) c.max_levels = sp.mg_levels;
   if (sp.mg_smoothing_range > 1.0) c.smoothing_range = sp.mg_smoothing_range;
   if (sp.mg_cell_tol > 0.0) c.cell_tol = sp.mg_cell_tol;
@@ -410,7 +412,7 @@ int ensure_mg(afl_operator* op) {
   const int want = op->sp.schur_precond;
   int rc;
   if (op->mg_built != want) {
-    const MgConfig cfg = mg_config(op->sp);
+    const MgConfig cfg = mg_config(op->sp, op->o.dim);
     rc = want == AFL_SCHUR_MG_FP32 ? op->mg32.setup_levels(op->o, cfg, op->tabs_by_k)
                                    : op->mg64.setup_levels(op->o, cfg, op->tabs_by_k);
     if (rc) return fail(AFL_ERR_DEVICE, "multigrid set-up: hipMalloc failed");
@@ -419,7 +421,7 @@ int ensure_mg(afl_operator* op) {
     op->mg_coef_stale = true;
   }
   {  // smoothing degrees and range take effect without a rebuild
-    const MgConfig cfg = mg_config(op->sp);
+    const MgConfig cfg = mg_config(op->sp, op->o.dim);
     auto upd = [&](auto& mg) {
       mg.smoother_degree = cfg.smoother_degree;
       mg.coarse_degree = cfg.coarse_degree;
@@ -437,7 +439,7 @@ int ensure_mg(afl_operator* op) {
     op->mg_coef_stale = false;
   }
   if (op->mg_eig_stale) {
-    const int it = mg_config(op->sp).eig_iters;
+    const int it = mg_config(op->sp, op->o.dim).eig_iters;
     rc = want == AFL_SCHUR_MG_FP32
              ? op->mg32.estimate_eigenvalues(op->o, it, op->mgw.p, op->dsc.p + kScA + 4, op->host_sc + kScA + 4)
              : op->mg64.estimate_eigenvalues(op->o, it, op->mgw.p, op->dsc.p + kScA + 4, op->host_sc + kScA + 4);
