@@ -193,6 +193,10 @@ struct afl_operator {
   DevBuf adjW0, adjT1, adjT2, adjC;
   static constexpr int kAdjBufs = 34;
   DevBuf aj[kAdjBufs];
+  // last pressure multiplier of each implicit stage: the initial guess of the next adjoint stage
+  // solve (adjoint variables change little from one step to the previous)
+  DevBuf adj_lam_prev[2];
+  bool adj_lam_valid[2] = {false, false};
   // observation points (device copies of their location) and point scratch
   ObsDev obs;
   DevBuf obs_f;
@@ -210,6 +214,7 @@ struct afl_operator {
                      &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix, &adjW0, &adjT1, &adjT2, &adjC};
     for (DevBuf* b : all) b->release();
     for (DevBuf& b : aj) b.release();
+    for (DevBuf& b : adj_lam_prev) b.release();
     release_obs();
     if (host_sc) hipHostFree(host_sc);
     if (dtab) hipFree(dtab);
@@ -1370,7 +1375,9 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
   vec_norm2(st, N, r, op->red.p, op->dsc.p + kScA);
   if ((rc = fetch(op, kScA, 1))) return rc;
   const double tol = op->sp.gmres_eps * op->host_sc[kScA];
-  vec_fill(st, N, 0.0, lamp);
+  if ((rc = op->adj_lam_prev[stage - 1].alloc(N))) return rc;
+  if (op->adj_lam_valid[stage - 1] && op->host_sc[kScA] > 0.0) vec_copy(st, N, op->adj_lam_prev[stage - 1].p, lamp);
+  else vec_fill(st, N, 0.0, lamp);
   int it = 0;
   double res = 0.0;
   // Z differs from the Schur operator's transpose by coefficient terms, and the Schur operator is
@@ -1384,6 +1391,8 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
                      "adjoint stage FGMRES did not converge");
     if (rc) return rc;
   }
+  vec_copy(st, N, lamp, op->adj_lam_prev[stage - 1].p);
+  op->adj_lam_valid[stage - 1] = true;
   // lam_u = A^{-1} (ubar + E3_u^T lam_p)
   launch_scaled_velocity_mass(o, lamp, rho_new, us, -Ma2, -cgz, t1);
   vec_axpby(st, NU, 1.0, ubar, 1.0, t1);
@@ -1417,11 +1426,11 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
 }  // namespace
 }  // extern "C++"
 
-int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, const double* p, const double* rho_bar_out,
-                      const double* u_bar_out, const double* p_bar_out, double* rho_bar, double* u_bar, double* p_bar,
-                      afl_step_info* info) {
-  if (!op || !rho || !u || !p || !rho_bar_out || !u_bar_out || !p_bar_out || !rho_bar || !u_bar || !p_bar)
-    return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+// Reverse part of the step adjoint, given the step's input state and its stage vectors
+// stg = {rho2, u2, p2, rho3, u3, p3, rho_c, u_c (before the sponge)}.
+int step_vjp_reverse(afl_operator* op, const double* rho, const double* u, const double* p, const double* const* stg,
+                     const double* rho_bar_out, const double* u_bar_out, const double* p_bar_out, double* rho_bar,
+                     double* u_bar, double* p_bar) {
   OpCtx& o = op->o;
   const int64_t N = o.N, NU = N * o.dim;
   hipStream_t st = o.stream;
@@ -1431,13 +1440,8 @@ int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, cons
   StepAdj S{op, N, NU, {}, st};
   for (int i = 0; i < afl_operator::kAdjBufs; ++i) S.b[i] = op->aj[i].p;
   double* const* B = S.b;
-  // forward recompute from copies of the input state: restores the stage vectors
-  vec_copy(st, N, rho, B[31]);
-  vec_copy(st, NU, u, B[32]);
-  vec_copy(st, N, p, B[33]);
-  if ((rc = afl_imex_step(op, B[31], B[32], B[33], info))) return rc;
-  const double *r2 = op->rho2.p, *u2 = op->u2.p, *p2 = op->p2.p, *r3 = op->rho3.p, *u3 = op->u3.p, *p3 = op->p3.p;
-  const double *rc_ = op->rhoc.p, *uc = op->uc.p;
+  const double *r2 = stg[0], *u2 = stg[1], *p2 = stg[2], *r3 = stg[3], *u3 = stg[4], *p3 = stg[5];
+  const double *rc_ = stg[6], *uc = stg[7];
   double *rb2 = B[0], *ub2 = B[1], *pb2 = B[2], *rb3 = B[3], *ub3 = B[4], *pb3 = B[5], *rbc = B[6], *ubc = B[7],
          *pbc = B[8], *e = B[20], *eU = B[21], *t3 = B[28];
   double* outs[11];
@@ -1513,6 +1517,27 @@ int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, cons
     acc(rho_bar, outs[0], N);
     acc(u_bar, outs[1], NU);
   }
+  return AFL_OK;
+}
+
+int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, const double* p, const double* rho_bar_out,
+                      const double* u_bar_out, const double* p_bar_out, double* rho_bar, double* u_bar, double* p_bar,
+                      afl_step_info* info) {
+  if (!op || !rho || !u || !p || !rho_bar_out || !u_bar_out || !p_bar_out || !rho_bar || !u_bar || !p_bar)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  OpCtx& o = op->o;
+  const int64_t N = o.N, NU = N * o.dim;
+  hipStream_t st = o.stream;
+  int rc;
+  for (DevBuf& b : op->aj)
+    if ((rc = b.alloc(NU))) return rc;
+  // forward recompute from copies of the input state: restores the stage vectors
+  vec_copy(st, N, rho, op->aj[31].p);
+  vec_copy(st, NU, u, op->aj[32].p);
+  vec_copy(st, N, p, op->aj[33].p);
+  if ((rc = afl_imex_step(op, op->aj[31].p, op->aj[32].p, op->aj[33].p, info))) return rc;
+  const double* stg[8] = {op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p, op->rhoc.p, op->uc.p};
+  if ((rc = step_vjp_reverse(op, rho, u, p, stg, rho_bar_out, u_bar_out, p_bar_out, rho_bar, u_bar, p_bar))) return rc;
   return check_device("imex_step_vjp");
 }
 
@@ -1627,9 +1652,76 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
                     double* rho_bar_0, double* u_bar_0, double* p_bar_0, afl_adjoint_stats* stats) {
   if (!op || !rho0 || !u0 || !p0 || !rho_bar_T || !u_bar_T || !p_bar_T || !rho_bar_0 || !u_bar_0 || !p_bar_0)
     return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
-  if (nsteps < 1 || snapshots < 0 || snapshots > 4096)
-    return fail(AFL_ERR_INVALID_ARGUMENT, "adjoint_run: nsteps >= 1 and 0 <= snapshots <= 4096");
+  if (nsteps < 1 || snapshots < AFL_ADJOINT_STORE_ALL || snapshots > 4096)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "adjoint_run: nsteps >= 1 and AFL_ADJOINT_STORE_ALL <= snapshots <= 4096");
   const int64_t N = op->N(), NU = N * op->dim();
+  if (snapshots == AFL_ADJOINT_STORE_ALL) {
+    // one forward sweep keeping every step's input state and stage vectors, then the reverse sweep
+    // with no recomputation
+    constexpr int kF = 11;  // rho, u, p, rho2, u2, p2, rho3, u3, p3, rho_c, u_c
+    const bool vec[kF] = {false, true, false, false, true, false, false, true, false, false, true};
+    std::vector<DevBuf> store((size_t)kF * nsteps);
+    DevBuf x[3], lam[2][3];
+    int rc;
+    auto release = [&]() {
+      for (DevBuf& b : store) b.release();
+      for (DevBuf& b : x) b.release();
+      for (auto& l : lam)
+        for (DevBuf& b : l) b.release();
+    };
+    for (int k = 0; k < nsteps; ++k)
+      for (int f = 0; f < kF; ++f)
+        if ((rc = store[(size_t)kF * k + f].alloc(vec[f] ? NU : N))) {
+          release();
+          return fail(AFL_ERR_DEVICE, "adjoint_run: not enough device memory to store every step; use snapshots");
+        }
+    for (int f = 0; f < 3; ++f)
+      if ((rc = x[f].alloc(f == 1 ? NU : N)) || (rc = lam[0][f].alloc(f == 1 ? NU : N)) ||
+          (rc = lam[1][f].alloc(f == 1 ? NU : N))) {
+        release();
+        return rc;
+      }
+    hipStream_t st = op->o.stream;
+    afl_adjoint_stats as{};
+    vec_copy(st, N, rho0, x[0].p);
+    vec_copy(st, NU, u0, x[1].p);
+    vec_copy(st, N, p0, x[2].p);
+    for (int k = 0; k < nsteps && !rc; ++k) {
+      DevBuf* S = &store[(size_t)kF * k];
+      for (int f = 0; f < 3; ++f) vec_copy(st, f == 1 ? NU : N, x[f].p, S[f].p);
+      if ((rc = afl_imex_step(op, x[0].p, x[1].p, x[2].p, nullptr))) break;
+      const double* sv[8] = {op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p, op->rhoc.p, op->uc.p};
+      for (int f = 0; f < 8; ++f) vec_copy(st, vec[3 + f] ? NU : N, sv[f], S[3 + f].p);
+      ++as.forward_steps;
+    }
+    int cur = 0;
+    if (!rc) {
+      vec_copy(st, N, rho_bar_T, lam[0][0].p);
+      vec_copy(st, NU, u_bar_T, lam[0][1].p);
+      vec_copy(st, N, p_bar_T, lam[0][2].p);
+    }
+    for (int k = nsteps - 1; k >= 0 && !rc; --k) {
+      DevBuf* S = &store[(size_t)kF * k];
+      const double* stg[8];
+      for (int f = 0; f < 8; ++f) stg[f] = S[3 + f].p;
+      const int nx = 1 - cur;
+      rc = step_vjp_reverse(op, S[0].p, S[1].p, S[2].p, stg, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p,
+                            lam[nx][0].p, lam[nx][1].p, lam[nx][2].p);
+      cur = nx;
+      ++as.adjoint_steps;
+    }
+    if (!rc) {
+      vec_copy(st, N, lam[cur][0].p, rho_bar_0);
+      vec_copy(st, NU, lam[cur][1].p, u_bar_0);
+      vec_copy(st, N, lam[cur][2].p, p_bar_0);
+      rc = check_device("adjoint_run");
+    }
+    hipStreamSynchronize(st);
+    as.snapshots_used = nsteps;
+    if (stats) *stats = as;
+    release();
+    return rc;
+  }
   const int s = std::min(snapshots, nsteps - 1);
   AdjRun R;
   R.op = op; R.N = N; R.NU = NU;

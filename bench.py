@@ -209,7 +209,9 @@ def main():
             1e-5 * torch.randn(N, dtype=torch.float64, device="cuda", generator=gen))
     x0 = tuple(t.clone() for t in s32)
     adjoint = []
-    for snaps in sorted({min(2, args.steps - 1), args.steps - 1}):
+    # two snapshots, a snapshot per step (each reverse step recomputes its own step), and every
+    # step's stage vectors kept from one forward sweep (-1: no recomputation)
+    for snaps in sorted({min(2, args.steps - 1), args.steps - 1}) + [-1]:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         *_, adj_stats = o32.adjoint_run(*x0, args.steps, snaps, *cbar)

@@ -71,6 +71,10 @@ typedef struct afl_step_info {
   double schur_residual;    /* last GMRES residual estimate */
 } afl_step_info;
 
+/* afl_adjoint_run's snapshots argument for keeping every step's state and stage vectors
+ * (15 scalar fields per step in 2D, 20 in 3D): no recomputation at all. */
+#define AFL_ADJOINT_STORE_ALL (-1)
+
 /* Counters of an adjoint run with binomial checkpointing (afl_adjoint_run). */
 typedef struct afl_adjoint_stats {
   int32_t forward_steps;   /* IMEX steps taken to reach checkpoints and re-reach states */

@@ -129,8 +129,9 @@ def test_multistep_gradient_matches_oracle_differences(oracle_mod):
 
 
 def test_step_vjp_at_size():
-    """64x25 k=2 hill: the step adjoint is linear in the output adjoint (no state carried between
-    calls) and agrees with GPU difference quotients along density directions."""
+    """64x25 k=2 hill: the step adjoint is linear in the output adjoint (calls share only the
+    initial guess of the stage solves) and agrees with GPU difference quotients along density
+    directions."""
     import torch
     say("[gpu step adjoint] hill 64x25 k=2")
     mesh = make_mesh(dim=2, degree=2, n_cells=(64, 1, 25), x_range=(-72e3, 72e3), z_top=25e3,
@@ -180,10 +181,11 @@ def _binomial_cost(t, s, memo={}):
     return memo[key]
 
 
-@pytest.mark.parametrize("snapshots", [0, 1, 2, 7])
+@pytest.mark.parametrize("snapshots", [0, 1, 2, 7, -1])
 def test_adjoint_run_checkpointing_matches_stored_states(snapshots):
-    """Seven steps reversed with binomial checkpointing give the gradient of the chain of step
-    adjoints over stored states, with the optimal number of recomputed forward steps."""
+    """Seven steps reversed with binomial checkpointing (or with every step's stages kept, -1) give
+    the gradient of the chain of step adjoints over stored states (to the solver tolerance), with
+    the optimal number of recomputed forward steps."""
     import torch
     say(f"[gpu adjoint run] 7 steps, {snapshots} snapshots, hill 32x12 k=2")
     mesh = make_mesh(dim=2, degree=2, n_cells=(32, 1, 12), x_range=(-36e3, 36e3), z_top=12e3,
@@ -207,14 +209,19 @@ def test_adjoint_run_checkpointing_matches_stored_states(snapshots):
         lam = op.imex_step_vjp(*xs[n], *lam)[:3]
     rb, ub, pb, stats = op.adjoint_run(rho, u, p, nsteps, snapshots, *c)
     say(f"  stats {stats}")
+    # the adjoint stage solves start from the previous call's multiplier, so the two runs agree to
+    # the solver tolerance rather than bitwise
     for a, b in zip((rb, ub, pb), lam):
         e = float((a - b).norm() / b.norm())
         say(f"  checkpointed vs stored: rel {e:.2e}")
-        assert e <= 1e-13
-    s = min(snapshots, nsteps - 1)
+        assert e <= 1e-10
     assert stats["adjoint_steps"] == nsteps
-    assert stats["forward_steps"] == _binomial_cost(nsteps, s)
-    assert stats["snapshots_used"] <= s + 1
+    if snapshots < 0:  # every step's stages kept: one forward sweep, no recomputation
+        assert stats["forward_steps"] == nsteps and stats["snapshots_used"] == nsteps
+    else:
+        s = min(snapshots, nsteps - 1)
+        assert stats["forward_steps"] == _binomial_cost(nsteps, s)
+        assert stats["snapshots_used"] <= s + 1
 
 
 def test_step_vjp_with_fp32_multigrid_solves(oracle_mod):
