@@ -599,6 +599,72 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
   }
 }
 
+// Traces only: the face-point values of eval_line_faces (tr[f][0] own side, tr[f][1] neighbour side,
+// own trace at walls) without the cell interpolation. Row owners stage their row's face nodes and
+// the neighbours' matching nodes in fn; one barrier; every thread interpolates at its face point.
+// Ends with a barrier (fn is reused by the next call). fn: FaceLds::FN values per cell.
+template <int DIM, int K, typename R = double>
+__device__ __attribute__((always_inline)) inline void trace_line_faces(const R* __restrict__ f, const Slot<DIM, K>& s,
+                                                                       const Nbrs<DIM>& nb, R* fn, const TabT<R>& T,
+                                                                       R (&tr)[2 * DIM][2]) {
+  using C = Cfg<DIM, K>;
+  constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN;
+  const R* u = f + (s.act ? s.cell : 0) * NPC;
+  auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
+  if (s.act && s.l < C::ROWS) {
+    R d[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+    FN(0, 0, s.l) = d[0];
+    FN(1, 0, s.l) = d[K];
+    FN(0, 1, s.l) = nb.c[0] >= 0 ? f[nb.c[0] * NPC + s.l * N1 + K] : d[0];
+    FN(1, 1, s.l) = nb.c[1] >= 0 ? f[nb.c[1] * NPC + s.l * N1] : d[K];
+    if (DIM == 2) {
+      if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
+      if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
+      if (nb.c[2] >= 0) FN(2, 1, s.l) = f[nb.c[2] * NPC + K * N1 + s.l];
+      if (nb.c[3] >= 0) FN(3, 1, s.l) = f[nb.c[3] * NPC + s.l];
+    } else {
+      const int iy = s.l % N1, iz = s.l / N1;
+      if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
+      if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
+      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = f[nb.c[2] * NPC + (iz * N1 + K) * N1 + iy];
+      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = f[nb.c[3] * NPC + (iz * N1) * N1 + iy];
+      if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
+      if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
+      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = f[nb.c[4] * NPC + (K * N1 + iz) * N1 + iy];
+      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
+    }
+  }
+  cell_sync<DIM, K>();
+  if (s.act) {
+#pragma unroll
+    for (int fc = 0; fc < 2 * DIM; ++fc)
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd) {
+        R a = 0.0;
+        if (DIM == 2) {
+#pragma unroll
+          for (int t = 0; t < N1; ++t) a += pick_row<NQ>(T.S, s.l, t) * FN(fc, sd, t);
+        } else {
+          const int qa = s.l % NQ, qb = s.l / NQ;
+#pragma unroll
+          for (int tb = 0; tb < N1; ++tb) {
+            R r = 0.0;
+#pragma unroll
+            for (int ta = 0; ta < N1; ++ta) r += pick_row<NQ>(T.S, qa, ta) * FN(fc, sd, ta + N1 * tb);
+            a += pick_row<NQ>(T.S, qb, tb) * r;
+          }
+        }
+        tr[fc][sd] = a;
+      }
+#pragma unroll
+    for (int fc = 0; fc < 2 * DIM; ++fc)
+      if (nb.c[fc] < 0) tr[fc][1] = tr[fc][0];
+  }
+  cell_sync<DIM, K>();
+}
+
 // Batched eval_line_faces: NF scalar fields evaluated together (one barrier per sweep for all of
 // them), the first NFF of them also at the face points. Per cell: t1 NF*EV1, t2 NF*EV2 and
 // fn NFF*FaceLds::FN values. Row owners issue all NF row loads before the sweeps.

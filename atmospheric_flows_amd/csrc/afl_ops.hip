@@ -17,6 +17,9 @@
 #ifndef AFL_C_BATCH_F32
 #define AFL_C_BATCH_F32 1
 #endif
+#ifndef AFL_E2_WAVES
+#define AFL_E2_WAVES 2
+#endif
 
 namespace afl {
 
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 struct E2Coef { double w[3], wp[3], gw[3]; };
 
 template <int DIM, int K, int NE, int NG>
-__global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__ Tp, Srcs src, E2Coef cf, double dt, double iMa2,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? AFL_E2_WAVES : 1))) void k_rhs_u(Geo g, const Tab* __restrict__ Tp, Srcs src, E2Coef cf, double dt, double iMa2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
@@ -168,23 +171,16 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f)
     for (int a = 0; a < DIM; ++a) F[f][a] = 0.0;
-#pragma unroll
+  // faces first, from the traces alone, then the cells from the values alone: the two sets of
+  // a state's fields are never live together (2D: state loops kept rolled)
+#pragma unroll(DIM == 2 ? 1 : NE)
   for (int j = 0; j < NE; ++j) {
     const double wj = pick<3>(cf.w, j), wpj = pick<3>(cf.wp, j);
-    double r[NQ], u[DIM][NQ], pj[NQ];
     double trr[2 * DIM][2], tru[DIM][2 * DIM][2], trp[2 * DIM][2];
-    eval_line_faces<DIM, K>(src_at(src, 3 * j), s, nb, t1, t2, fn, T, r, trr);
+    trace_line_faces<DIM, K>(src_at(src, 3 * j), s, nb, fn, T, trr);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
-    eval_line_faces<DIM, K>(src_at(src, 3 * j + 2), s, nb, t1, t2, fn, T, pj, trp);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (j == 0) for (int d = 0; d < DIM; ++d) V[d][q] = r[q] * u[d][q];
-      const double pw = wpj * dt * iMa2 * pj[q];
-      int m = 0;
-      for (int a = 0; a < DIM; ++a)
-        for (int b = a; b < DIM; ++b) Sym[m++][q] += wj * dt * r[q] * u[a][q] * u[b][q] + (a == b ? pw : 0.0);
-    }
+    for (int d = 0; d < DIM; ++d) trace_line_faces<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, nb, fn, T, tru[d]);
+    trace_line_faces<DIM, K>(src_at(src, 3 * j + 2), s, nb, fn, T, trp);
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
       double nfv[3], jfv;  // face geometry recomputed here: fewer live registers
@@ -208,6 +204,23 @@ __global__ __launch_bounds__(AFL_NT) void k_rhs_u(Geo g, const Tab* __restrict__
 #pragma unroll
         for (int a = 0; a < DIM; ++a) F[f][a] += wpj * dt * iMa2 * (0.5 * (pp + pp)) * nfv[a];
       }
+    }
+  }
+#pragma unroll(DIM == 2 ? 1 : NE)
+  for (int j = 0; j < NE; ++j) {
+    const double wj = pick<3>(cf.w, j), wpj = pick<3>(cf.wp, j);
+    double r[NQ], u[DIM][NQ], pj[NQ];
+    eval_line<DIM, K>(src_at(src, 3 * j), s, t1, t2, T, r);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) eval_line<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, t1, t2, T, u[d]);
+    eval_line<DIM, K>(src_at(src, 3 * j + 2), s, t1, t2, T, pj);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (j == 0) for (int d = 0; d < DIM; ++d) V[d][q] = r[q] * u[d][q];
+      const double pw = wpj * dt * iMa2 * pj[q];
+      int m = 0;
+      for (int a = 0; a < DIM; ++a)
+        for (int b = a; b < DIM; ++b) Sym[m++][q] += wj * dt * r[q] * u[a][q] * u[b][q] + (a == b ? pw : 0.0);
     }
   }
 #pragma unroll
@@ -272,32 +285,21 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   for (int q = 0; q < NQ; ++q) { V[q] = 0.0; for (int d = 0; d < DIM; ++d) G[d][q] = 0.0; }
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f) F[f] = 0.0;
-  // 2D: one explicit state per iteration, not unrolled, so the states' fields are never live
-  // together (3D unrolls: its register allocation spills when the loop is kept)
+  // faces first, from the traces alone, then the cells from the values alone: the two sets of
+  // a state's fields are never live together (2D: state loops kept rolled)
 #pragma unroll(DIM == 2 ? 1 : NE)
   for (int j = 0; j < NE; ++j) {
     const double wj = pick<3>(cf.w, j), wpj = pick<3>(cf.wt, j);
-    double r[NQ], u[DIM][NQ], p[NQ];
     double trr[2 * DIM][2], tru[DIM][2 * DIM][2], trp[2 * DIM][2];
-    eval_line_faces<DIM, K>(src_at(src, 3 * j), s, nb, t1, t2, fn, T, r, trr);
+    trace_line_faces<DIM, K>(src_at(src, 3 * j), s, nb, fn, T, trr);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, nb, t1, t2, fn, T, u[d], tru[d]);
-    eval_line_faces<DIM, K>(src_at(src, 3 * j + 2), s, nb, t1, t2, fn, T, p, trp);
+    for (int d = 0; d < DIM; ++d) trace_line_faces<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, nb, fn, T, tru[d]);
+    trace_line_faces<DIM, K>(src_at(src, 3 * j + 2), s, nb, fn, T, trp);
     if (STAGE == 2 && j == 0)
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
 #pragma unroll
         for (int f = 0; f < 2 * DIM; ++f) { tu0[d][f][0] = tru[d][f][0]; tu0[d][f][1] = tru[d][f][1]; }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      double uu = 0.0;
-      for (int d = 0; d < DIM; ++d) uu += u[d][q] * u[d][q];
-      const double E = 1.0 / gm1 * (p[q] / r[q]) + 0.5 * Ma2 * uu;
-      if (j == 0) V[q] += r[q] * E;
-      const double kin = 0.5 * wj * dt * Ma2 * uu * r[q];
-      const double ent = wpj * dt * (r[q] * (E - 0.5 * Ma2 * uu) + p[q]);
-      for (int d = 0; d < DIM; ++d) G[d][q] += (kin + ent) * u[d][q];
-    }
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
       if (nb.c[f] < 0) continue;
@@ -336,6 +338,25 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
         const double je = rp * (Ep - 0.5 * Ma2 * uup) - rm * (Em - 0.5 * Ma2 * uum);
         F[f] += wj * dt * Ma2 * (avg_kin + 0.5 * lam * jk) + wpj * dt * (avg_ent + 0.5 * lam * je);
       }
+    }
+  }
+#pragma unroll(DIM == 2 ? 1 : NE)
+  for (int j = 0; j < NE; ++j) {
+    const double wj = pick<3>(cf.w, j), wpj = pick<3>(cf.wt, j);
+    double r[NQ], u[DIM][NQ], p[NQ];
+    eval_line<DIM, K>(src_at(src, 3 * j), s, t1, t2, T, r);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) eval_line<DIM, K>(src_at(src, 3 * j + 1) + d * N, s, t1, t2, T, u[d]);
+    eval_line<DIM, K>(src_at(src, 3 * j + 2), s, t1, t2, T, p);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double uu = 0.0;
+      for (int d = 0; d < DIM; ++d) uu += u[d][q] * u[d][q];
+      const double E = 1.0 / gm1 * (p[q] / r[q]) + 0.5 * Ma2 * uu;
+      if (j == 0) V[q] += r[q] * E;
+      const double kin = 0.5 * wj * dt * Ma2 * uu * r[q];
+      const double ent = wpj * dt * (r[q] * (E - 0.5 * Ma2 * uu) + p[q]);
+      for (int d = 0; d < DIM; ++d) G[d][q] += (kin + ent) * u[d][q];
     }
   }
   {
