@@ -19,7 +19,7 @@ class MeshDesc(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [
         ("gmres_eps", C.c_double), ("gmres_max_iter", C.c_int32), ("gmres_restart", C.c_int32),
-        ("fp_tol", C.c_double), ("fp_max_iter", C.c_int32), ("pad_", C.c_int32),
+        ("fp_tol", C.c_double), ("fp_max_iter", C.c_int32), ("mg_quad_points", C.c_int32),
         ("schur_precond", C.c_int32), ("mg_smoother_degree", C.c_int32), ("mg_coarse_degree", C.c_int32),
         ("mg_levels", C.c_int32), ("mg_smoothing_range", C.c_double), ("mg_cell_tol", C.c_double),
     ]
@@ -89,12 +89,13 @@ def make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01,
 
 def make_solver_params(gmres_eps=1e-10, gmres_max_iter=30000, gmres_restart=28, fp_tol=1e-10, fp_max_iter=100,
                        schur_precond=0, mg_smoother_degree=3, mg_coarse_degree=0, mg_levels=1,
-                       mg_smoothing_range=15.0, mg_cell_tol=0.05):
+                       mg_smoothing_range=15.0, mg_cell_tol=0.05, mg_quad_points=0):
     p = SolverParams()
     p.gmres_eps, p.gmres_max_iter, p.gmres_restart = gmres_eps, gmres_max_iter, gmres_restart
     p.fp_tol, p.fp_max_iter = fp_tol, fp_max_iter
     p.schur_precond, p.mg_smoother_degree, p.mg_coarse_degree = schur_precond, mg_smoother_degree, mg_coarse_degree
     p.mg_levels, p.mg_smoothing_range, p.mg_cell_tol = mg_levels, mg_smoothing_range, mg_cell_tol
+    p.mg_quad_points = mg_quad_points
     return p
 
 

@@ -102,9 +102,12 @@ double lag_d(const double* xn, int n1, int i, double x) {
   return s;
 }
 
-void build_tab(int k, int node_type, Tab& T) {
+// 1D tables of degree k on nq Gauss points (default 2k + 1, the operator's rule; the reference
+// mass matrix is exact for nq >= k + 1)
+void build_tab(int k, int node_type, Tab& T, int nq = 0) {
   std::memset(&T, 0, sizeof(T));
-  const int n1 = k + 1, nq = 2 * k + 1;
+  const int n1 = k + 1;
+  if (nq <= 0) nq = 2 * k + 1;
   nodes01(k, node_type, T.ND);
   gauss01(nq, T.QP, T.W);
   for (int q = 0; q < nq; ++q)
@@ -181,6 +184,7 @@ struct afl_operator {
   bool mg_eig_stale = true;  // rho_for_fixed changed since the last eigenvalue estimate
   bool mg_coef_stale = true; // rho_for_fixed / pres_fixed changed since the levels copied them
   Tab tabs_by_k[5]{};        // 1D tables per degree (same node family)
+  Tab tabs_rq_by_k[5]{};     // the same on k + 1 Gauss points (multigrid level quadrature)
   DevBuf gZ, mgw, diagw;
   double min_diam = 0.0;  // GridTools::minimal_cell_diameter of the mapped mesh
   afl_sponge sponge{15000.0, 0.0, {0.0, 0.0, 0.0}, 0, 0};
@@ -403,13 +407,15 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
 // p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
 // (coarsest-level Chebyshev degree: 16 in 2D, 5 in 3D unless set; on the 3D hill at 48x48x20,
 // k = 3 the solves take one FGMRES iteration from degree 5 up, so a higher degree only costs)
-MgConfig mg_config(const afl_solver_params& sp, int dim) {
+MgConfig mg_config(const afl_solver_params& sp, int dim, int k) {
   MgConfig c;
   if (sp.mg_smoother_degree > 0) c.smoother_degree = sp.mg_smoother_degree;
   c.coarse_degree = sp.mg_coarse_degree > 0 ? sp.mg_coarse_degree : (dim == 2 ? 16 : 5);
   if (sp.mg_levels > 0) c.max_levels = sp.mg_levels;
   if (sp.mg_smoothing_range > 1.0) c.smoothing_range = sp.mg_smoothing_range;
   if (sp.mg_cell_tol > 0.0) c.cell_tol = sp.mg_cell_tol;
+  // k + 1 points per direction on every level unless the operator's own 2k + 1 are asked for
+  c.reduced_quadrature = sp.mg_quad_points != 2 * k + 1;
   return c;
 }
 
@@ -417,16 +423,16 @@ int ensure_mg(afl_operator* op) {
   const int want = op->sp.schur_precond;
   int rc;
   if (op->mg_built != want) {
-    const MgConfig cfg = mg_config(op->sp, op->o.dim);
-    rc = want == AFL_SCHUR_MG_FP32 ? op->mg32.setup_levels(op->o, cfg, op->tabs_by_k)
-                                   : op->mg64.setup_levels(op->o, cfg, op->tabs_by_k);
+    const MgConfig cfg = mg_config(op->sp, op->o.dim, op->o.k);
+    rc = want == AFL_SCHUR_MG_FP32 ? op->mg32.setup_levels(op->o, cfg, op->tabs_by_k, op->tabs_rq_by_k)
+                                   : op->mg64.setup_levels(op->o, cfg, op->tabs_by_k, op->tabs_rq_by_k);
     if (rc) return fail(AFL_ERR_DEVICE, "multigrid set-up: hipMalloc failed");
     op->mg_built = want;
     op->mg_eig_stale = true;
     op->mg_coef_stale = true;
   }
   {  // smoothing degrees and range take effect without a rebuild
-    const MgConfig cfg = mg_config(op->sp, op->o.dim);
+    const MgConfig cfg = mg_config(op->sp, op->o.dim, op->o.k);
     auto upd = [&](auto& mg) {
       mg.smoother_degree = cfg.smoother_degree;
       mg.coarse_degree = cfg.coarse_degree;
@@ -444,7 +450,7 @@ int ensure_mg(afl_operator* op) {
     op->mg_coef_stale = false;
   }
   if (op->mg_eig_stale) {
-    const int it = mg_config(op->sp, op->o.dim).eig_iters;
+    const int it = mg_config(op->sp, op->o.dim, op->o.k).eig_iters;
     rc = want == AFL_SCHUR_MG_FP32
              ? op->mg32.estimate_eigenvalues(op->o, it, op->mgw.p, op->dsc.p + kScA + 4, op->host_sc + kScA + 4)
              : op->mg64.estimate_eigenvalues(op->o, it, op->mgw.p, op->dsc.p + kScA + 4, op->host_sc + kScA + 4);
@@ -588,7 +594,10 @@ int afl_create(const afl_mesh_desc* md, double mach, double froude, double dt, a
   o.dim = md->dim;
   o.k = md->degree;
   build_tab(o.k, md->node_type, o.T);
-  for (int kk = 1; kk <= 4; ++kk) build_tab(kk, md->node_type, op->tabs_by_k[kk]);
+  for (int kk = 1; kk <= 4; ++kk) {
+    build_tab(kk, md->node_type, op->tabs_by_k[kk]);
+    build_tab(kk, md->node_type, op->tabs_rq_by_k[kk], kk + 1);
+  }
   if (hipMalloc((void**)&op->dtab, sizeof(Tab)) != hipSuccess ||
       hipMemcpy(op->dtab, &o.T, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess) {
     delete op;
@@ -705,8 +714,10 @@ int afl_set_solver_params(afl_operator* op, const afl_solver_params* sp) {
     return fail(AFL_ERR_INVALID_ARGUMENT, "unknown schur_precond");
   if (sp->mg_smoother_degree > 12 || sp->mg_coarse_degree > 64 || sp->mg_levels > 4)
     return fail(AFL_ERR_INVALID_ARGUMENT, "multigrid parameters out of range");
+  if (sp->mg_quad_points != 0 && sp->mg_quad_points != op->o.k + 1 && sp->mg_quad_points != 2 * op->o.k + 1)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "mg_quad_points must be 0, k + 1 or 2k + 1");
   const bool rebuild = sp->schur_precond != op->sp.schur_precond || sp->mg_levels != op->sp.mg_levels ||
-                       sp->mg_cell_tol != op->sp.mg_cell_tol;
+                       sp->mg_cell_tol != op->sp.mg_cell_tol || sp->mg_quad_points != op->sp.mg_quad_points;
   op->sp = *sp;
   if (rebuild) op->mg_built = 0;
   return AFL_OK;
@@ -914,6 +925,18 @@ int afl_mg_vmult(afl_operator* op, double* dst, const double* src) {
   if (pc == AFL_SCHUR_MG_FP32) op->mg32.apply(op->o, src, dst);
   else op->mg64.apply(op->o, src, dst);
   return check_device("mg_vmult");
+}
+
+int afl_mg_level_vmult(afl_operator* op, double* dst, const double* src) {
+  if (!op || !dst || !src) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  const int pc = op->sp.schur_precond;
+  if (pc != AFL_SCHUR_MG_FP32 && pc != AFL_SCHUR_MG_FP64)
+    return fail(AFL_ERR_STATE, "mg_level_vmult needs solver_params.schur_precond = AFL_SCHUR_MG_FP32 or AFL_SCHUR_MG_FP64");
+  int rc;
+  if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
+  if (pc == AFL_SCHUR_MG_FP32) op->mg32.level_vmult(op->o, src, dst);
+  else op->mg64.level_vmult(op->o, src, dst);
+  return check_device("mg_level_vmult");
 }
 
 int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {

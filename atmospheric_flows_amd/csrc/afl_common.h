@@ -74,10 +74,10 @@ __device__ __attribute__((always_inline)) inline R pick_col(const R (&S)[9][5], 
   return v;
 }
 
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 struct Cfg {
   static constexpr int N1 = K + 1;
-  static constexpr int NQ = 2 * K + 1;
+  static constexpr int NQ = Q;  // quadrature points per direction (2k+1: the operator's rule)
   static constexpr int LT = (DIM == 2) ? NQ : NQ * NQ;        // threads per cell
   // Cells that fit in one 64-wide wavefront are packed whole into waves (CPW per wave, the few
   // spare lanes idle): all exchanges inside a cell are then wave-local and need no block barrier.
@@ -98,7 +98,7 @@ struct Cfg {
 };
 
 // Thread position inside the block.
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 struct Slot {
   int cb;        // cell slot in block
   int l;         // line index in cell
@@ -106,7 +106,7 @@ struct Slot {
   bool act;      // slot maps to an existing cell
   int ic[3];     // cell coordinates
   __device__ Slot(const Geo& g) {
-    using C = Cfg<DIM, K>;
+    using C = Cfg<DIM, K, Q>;
     if (C::WAVE_LOCAL) {
       const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, lc = lane / C::LT;
       l = lane % C::LT;
@@ -126,9 +126,9 @@ struct Slot {
 
 // Barrier between the phases of a cell's computation (LDS written by some of the cell's threads,
 // read by others). Wave-local cells need only ordering inside the wavefront; otherwise the block.
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 __device__ __attribute__((always_inline)) inline void cell_sync() {
-  if constexpr (Cfg<DIM, K>::WAVE_LOCAL) {
+  if constexpr (Cfg<DIM, K, Q>::WAVE_LOCAL) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -152,14 +152,14 @@ __device__ __attribute__((always_inline)) inline void hill(const Geo& g, R x, R 
 }
 
 // Per-line geometry: everything that is constant along the thread's vertical line.
-template <int DIM, int K, typename R = double>
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
 struct LineGeo {
   R c;      // hz (1 - h/H) = dz/dzeta
   R dhx, dhy;
   R wxy;    // horizontal quadrature weight product
   R hx, hy, hz, iH, zeta0;
   __device__ void init(const Geo& g, const TabT<R>& T, const int* ic, int l) {
-    using C = Cfg<DIM, K>;
+    using C = Cfg<DIM, K, Q>;
     hx = R(g.hc[0]);
     hy = DIM == 3 ? R(g.hc[1]) : R(1);
     hz = R(g.hc[DIM - 1]);
@@ -219,9 +219,9 @@ struct LineGeo {
 
 // Geometry of face f = 2*dir + side of cell ic at face point fq (2D: fq = q; 3D: fq = qa + NQ qb,
 // qa/qb along the two tangential axes in increasing order). Returns outward normal and JxW.
-template <int DIM, int K, typename R = double>
+template <int DIM, int K, int Q = 2 * K + 1, typename R = double>
 __device__ __attribute__((always_inline)) inline void face_geo(const Geo& g, const TabT<R>& T, const int* ic, int f, int fq, R* n, R& jxw) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   const int dir = f >> 1, side = f & 1;
   const R sg = side ? R(1) : R(-1);
   const R iH = R(1.0 / g.H);
@@ -306,10 +306,10 @@ __device__ __attribute__((always_inline)) inline int face_node(int dir, int side
 // ---------------------------------------------------------------------------------------
 // Evaluate a scalar field (global dofs) at the quadrature points of the thread's line.
 // All threads of the block must call it (it contains barriers).
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void eval_line(const R* __restrict__ f, const Slot<DIM, K>& s, R* t1, R* t2,
-                                 const TabT<R>& T, R (&v)[2 * K + 1]) {
-  using C = Cfg<DIM, K>;
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_line(const R* __restrict__ f, const Slot<DIM, K, Q>& s, R* t1, R* t2,
+                                 const TabT<R>& T, R (&v)[Q]) {
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ;
   const R* u = f + (s.act ? s.cell : 0) * C::NPC;
   if (DIM == 2) {
@@ -325,7 +325,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         t1[s.l * NQ + q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
       R c[N1];
 #pragma unroll
@@ -338,7 +338,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         v[q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     if (s.act && s.l < N1 * N1) {
       R d[N1];
@@ -352,7 +352,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         t1[s.l * NQ + q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
       R c[N1];
@@ -366,7 +366,7 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         t2[(iz * NQ + q) * NQ + qx] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
       R c[N1];
 #pragma unroll
@@ -379,17 +379,17 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
         v[q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
 // Reference-coordinate gradient of a scalar field at the quadrature points of the thread's line:
 // gr[b][q] = d/dxi_b (D along axis b, S along the others). Work area per cell: 2D 2*N1*NQ,
 // 3D 2*N1*N1*NQ + 3*N1*NQ*NQ values from t1 on (no t2). Contains barriers.
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __restrict__ f, const Slot<DIM, K>& s, R* t1,
-                                      const TabT<R>& T, R (&gr)[DIM][2 * K + 1]) {
-  using C = Cfg<DIM, K>;
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __restrict__ f, const Slot<DIM, K, Q>& s, R* t1,
+                                      const TabT<R>& T, R (&gr)[DIM][Q]) {
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ;
   const R* u = f + (s.act ? s.cell : 0) * C::NPC;
   constexpr int X1 = (DIM == 2) ? N1 * NQ : N1 * N1 * NQ;  // one x-sweep array
@@ -408,7 +408,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __
       tD[s.l * NQ + q] = b;
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   if (DIM == 2) {
     if (s.act) {
       R cs[N1], cd[N1];
@@ -423,7 +423,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __
         gr[DIM - 1][q] = b;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     constexpr int Y1 = N1 * NQ * NQ;
     R* uSS = t1 + 2 * X1;  // S_x S_y: for the z derivative
@@ -443,7 +443,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __
         uSS[o] = a; uSD[o] = b; uDS[o] = c;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
       R a0[N1], a1[N1], a2[N1];
 #pragma unroll
@@ -456,7 +456,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_grad(const R* __
         gr[0][q] = x; gr[1][q] = y; gr[DIM - 1][q] = z;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
@@ -483,11 +483,11 @@ struct FaceLds {
 // Row owners copy their own face nodes and fetch the neighbour's matching face nodes (2 scalar
 // loads per face direction) into fn during the x sweep; face-point threads interpolate after the
 // first barrier, so no barrier is added. fn: FaceLds::FN doubles per cell.
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
-                                       R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[2 * K + 1],
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_line_faces(const R* __restrict__ f, const Slot<DIM, K, Q>& s, const Nbrs<DIM>& nb,
+                                       R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[Q],
                                        R (&tr)[2 * DIM][2]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN;
   const R* u = f + (s.act ? s.cell : 0) * NPC;
   auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
@@ -527,7 +527,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
       if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   // traces at face point l (every thread of an active cell owns one face point per face)
   if (s.act) {
 #pragma unroll
@@ -567,7 +567,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
         v[q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
@@ -582,7 +582,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
         t2[(iz * NQ + q) * NQ + qx] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
       R c[N1];
 #pragma unroll
@@ -595,7 +595,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
         v[q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
@@ -603,11 +603,11 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
 // own trace at walls) without the cell interpolation. Row owners stage their row's face nodes and
 // the neighbours' matching nodes in fn; one barrier; every thread interpolates at its face point.
 // Ends with a barrier (fn is reused by the next call). fn: FaceLds::FN values per cell.
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void trace_line_faces(const R* __restrict__ f, const Slot<DIM, K>& s,
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void trace_line_faces(const R* __restrict__ f, const Slot<DIM, K, Q>& s,
                                                                        const Nbrs<DIM>& nb, R* fn, const TabT<R>& T,
                                                                        R (&tr)[2 * DIM][2]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN;
   const R* u = f + (s.act ? s.cell : 0) * NPC;
   auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
@@ -636,7 +636,7 @@ __device__ __attribute__((always_inline)) inline void trace_line_faces(const R* 
       if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   if (s.act) {
 #pragma unroll
     for (int fc = 0; fc < 2 * DIM; ++fc)
@@ -662,17 +662,17 @@ __device__ __attribute__((always_inline)) inline void trace_line_faces(const R* 
     for (int fc = 0; fc < 2 * DIM; ++fc)
       if (nb.c[fc] < 0) tr[fc][1] = tr[fc][0];
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
 }
 
 // Batched eval_line_faces: NF scalar fields evaluated together (one barrier per sweep for all of
 // them), the first NFF of them also at the face points. Per cell: t1 NF*EV1, t2 NF*EV2 and
 // fn NFF*FaceLds::FN values. Row owners issue all NF row loads before the sweeps.
-template <int DIM, int K, int NF, int NFF, typename R = double>
-__device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* const (&f)[NF], const Slot<DIM, K>& s, const Nbrs<DIM>& nb,
-                                        R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[NF][2 * K + 1],
+template <int DIM, int K, int NF, int NFF, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* const (&f)[NF], const Slot<DIM, K, Q>& s, const Nbrs<DIM>& nb,
+                                        R* t1, R* t2, R* fn, const TabT<R>& T, R (&v)[NF][Q],
                                         R (&tr)[NFF > 0 ? NFF : 1][2 * DIM][2]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, NFN = FaceLds<DIM, K>::NFN, FNS = FaceLds<DIM, K>::FN;
   constexpr int E1 = C::EV1, E2 = C::EV2;
   const int64_t cell = s.act ? s.cell : 0;
@@ -717,7 +717,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
       }
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   if (s.act) {
     // the thread's rows of the interpolation table (face point l), read once
     const int qa = DIM == 2 ? s.l : s.l % NQ, qb = DIM == 2 ? 0 : s.l / NQ;
@@ -766,7 +766,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
@@ -784,7 +784,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
@@ -800,15 +800,15 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
 // Same as eval_line but the dofs come from an LDS copy of the cell (u = lds + cb*NPC).
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2,
-                                     const TabT<R>& T, R (&v)[2 * K + 1]) {
-  using C = Cfg<DIM, K>;
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, const Slot<DIM, K, Q>& s, R* t1, R* t2,
+                                     const TabT<R>& T, R (&v)[Q]) {
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
@@ -820,7 +820,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         t1[s.l * NQ + q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -830,7 +830,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         v[q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
@@ -841,7 +841,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         t1[s.l * NQ + q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -852,7 +852,7 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         t2[(iz * NQ + q) * NQ + qx] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -862,16 +862,16 @@ __device__ __attribute__((always_inline)) inline void eval_line_lds(const R* u, 
         v[q] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
 // Batched evaluation of NF scalar fields whose cell dofs sit in LDS (field f at u + f*NPC).
 // t1 needs NF*EV1 doubles, t2 NF*EV2 doubles per cell. Contains barriers.
-template <int DIM, int K, int NF, typename R = double>
-__device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u, const Slot<DIM, K>& s, R* t1, R* t2, const TabT<R>& T,
-                                      R (&v)[NF][2 * K + 1]) {
-  using C = Cfg<DIM, K>;
+template <int DIM, int K, int NF, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u, const Slot<DIM, K, Q>& s, R* t1, R* t2, const TabT<R>& T,
+                                      R (&v)[NF][Q]) {
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ, NPC = C::NPC, E1 = C::EV1, E2 = C::EV2;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
@@ -885,7 +885,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
           t1[f * E1 + s.l * NQ + q] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -901,7 +901,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
@@ -914,7 +914,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
           t1[f * E1 + s.l * NQ + q] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -931,7 +931,7 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -947,25 +947,25 @@ __device__ __attribute__((always_inline)) inline void eval_lines_lds(const R* u,
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
 // Batched value-only integration of NF line fields; row l of field f ends in out[f].
 // i1 needs NF*I1V doubles, i2 NF*I2V doubles per cell (2D: I1V = N1*NQ, I2V = 0;
 // 3D: I1V = N1*NQ*NQ, I2V = N1*N1*NQ). Contains barriers.
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 struct IntV {
-  static constexpr int N1 = K + 1, NQ = 2 * K + 1;
+  static constexpr int N1 = K + 1, NQ = Q;
   static constexpr int I1 = DIM == 2 ? N1 * NQ : N1 * NQ * NQ;
   static constexpr int I2 = DIM == 2 ? 0 : N1 * N1 * NQ;
 };
 
-template <int DIM, int K, int NF, typename R = double>
-__device__ __attribute__((always_inline)) inline void integrate_lines_v(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
-                                         const R (&V)[NF][2 * K + 1], R (&out)[NF][K + 1]) {
-  using C = Cfg<DIM, K>;
-  constexpr int N1 = C::N1, NQ = C::NQ, A1 = IntV<DIM, K>::I1, A2 = IntV<DIM, K>::I2;
+template <int DIM, int K, int NF, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void integrate_lines_v(const Slot<DIM, K, Q>& s, R* i1, R* i2, const TabT<R>& T,
+                                         const R (&V)[NF][Q], R (&out)[NF][K + 1]) {
+  using C = Cfg<DIM, K, Q>;
+  constexpr int N1 = C::N1, NQ = C::NQ, A1 = IntV<DIM, K, Q>::I1, A2 = IntV<DIM, K, Q>::I2;
   if (DIM == 2) {
     if (s.act) {
 #pragma unroll
@@ -978,7 +978,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           i1[f * A1 + i * NQ + s.l] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < N1) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
@@ -990,7 +990,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           out[f][i] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     if (s.act) {
 #pragma unroll
@@ -1003,7 +1003,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           i1[f * A1 + i * NQ * NQ + s.l] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -1016,7 +1016,7 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           i2[f * A2 + (iz * N1 + i) * NQ + qx] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
@@ -1028,18 +1028,18 @@ __device__ __attribute__((always_inline)) inline void integrate_lines_v(const Sl
           out[f][i] = a;
         }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
 // Integrate line data (values V and reference-gradient contributions G[d], all already
 // multiplied by JxW) against the test functions. Row `l` (l < ROWS) of the result ends up
 // in out[N1]. HAS_V / HAS_G select the terms. Contains barriers.
-template <int DIM, int K, bool HAS_V, bool HAS_G, typename R = double>
-__device__ __attribute__((always_inline)) inline void integrate_line(const Slot<DIM, K>& s, R* i1, R* i2, const TabT<R>& T,
-                                      const R (&V)[2 * K + 1], const R (&G)[DIM][2 * K + 1],
+template <int DIM, int K, bool HAS_V, bool HAS_G, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void integrate_line(const Slot<DIM, K, Q>& s, R* i1, R* i2, const TabT<R>& T,
+                                      const R (&V)[Q], const R (&G)[DIM][Q],
                                       R (&out)[K + 1]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ;
   if (DIM == 2) {
     R* ta = i1;
@@ -1057,7 +1057,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         if (HAS_G) tb[i * NQ + s.l] = b;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < N1) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
@@ -1070,7 +1070,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         out[i] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   } else {
     R* ta = i1;
     R* tx = i1 + N1 * NQ * NQ;
@@ -1090,7 +1090,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         if (HAS_G) { tx[i * NQ * NQ + s.l] = bx; ty[i * NQ * NQ + s.l] = by; }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < NQ * N1) {
       const int qx = s.l % NQ, iz = s.l / NQ;
 #pragma unroll
@@ -1106,7 +1106,7 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         if (HAS_G) ux[(iz * N1 + i) * NQ + qx] = bx;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < N1 * N1) {
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
@@ -1119,16 +1119,16 @@ __device__ __attribute__((always_inline)) inline void integrate_line(const Slot<
         out[i] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   }
 }
 
 // Add face-flux integrals to node row l. Fq points to the LDS block [2*DIM][NFQ] of this cell
 // (values already multiplied by JxW_f and oriented for this cell). Only rows on faces change.
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void add_faces(const Slot<DIM, K>& s, const R* Fq, const TabT<R>& T, R (&out)[K + 1],
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void add_faces(const Slot<DIM, K, Q>& s, const R* Fq, const TabT<R>& T, R (&out)[K + 1],
                                  unsigned face_mask) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ, NFQ = C::NFQ;
   if (!(s.act && s.l < C::ROWS)) return;
   if (DIM == 2) {
@@ -1199,9 +1199,9 @@ __device__ __attribute__((always_inline)) inline void add_faces(const Slot<DIM, 
 }
 
 // Write node row l of a cell.
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void store_row(R* __restrict__ dst, const Slot<DIM, K>& s, const R (&out)[K + 1]) {
-  using C = Cfg<DIM, K>;
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void store_row(R* __restrict__ dst, const Slot<DIM, K, Q>& s, const R (&out)[K + 1]) {
+  using C = Cfg<DIM, K, Q>;
   if (s.act && s.l < C::ROWS) {
     R* o = dst + s.cell * C::NPC + s.l * C::N1;
 #pragma unroll

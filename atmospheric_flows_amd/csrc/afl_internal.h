@@ -90,24 +90,21 @@ void launch_apply_C_tangent(const OpCtx& o, const double* pf, const double* dw, 
 void launch_apply_C_tangent_t(const OpCtx& o, const double* pf, const double* cb, const double* w0, double* dst);
 // Multigrid level operator in precision R. sinv != nullptr selects the mean-density block
 // solve (w = P^{-1} B p, sinv per cell from launch_cell_rho_mass_inv); otherwise the cell-local
-// A solve runs to tol with rho.
+// A solve runs to tol with rho. rq selects the level quadrature: k + 1 Gauss points per direction
+// (rq) or the operator's 2k + 1.
 template <typename R>
-void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
+void launch_schur_pair(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
+                       const R* sinv, const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y);
 template <typename R>
-void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                      R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev = nullptr);
+void launch_schur_w_r(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
+                      const R* sinv, R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev = nullptr);
 template <typename R>
-void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
-                       const R* rin, R* rout, R c1, R c2, R* d, R* x);
+void launch_apply_C_cheb(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* pfix,
+                         R cdt, int64_t N, const R* w, const R* p, const R* jbar, const R* rin, R* rout, R c1, R c2,
+                         R* d, R* x, hipEvent_t* ev = nullptr);
 template <typename R>
-void launch_apply_C_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* pfix, R cdt,
-                         int64_t N, const R* w, const R* p, const R* jbar, const R* rin, R* rout, R c1, R c2, R* d,
-                         R* x, hipEvent_t* ev = nullptr);
-template <typename R>
-void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
-                              R* sinv);
+void launch_cell_rho_mass_inv(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st,
+                              const R* rho, R* sinv);
 
 // diagnostics (afl_diag.hip): out4 = {-min rho, max |rho|, max celerity, max speed};
 // out2 = {sum JxW (a-b)^2, sum JxW b^2}; work holds diag_work_size(o) doubles

@@ -15,6 +15,7 @@ struct MgConfig {
   double cell_tol = 0.05;       // cell-local A solve tolerance on the levels (preconditioner accuracy)
   int cell_maxit = 40;
   int eig_iters = 12;           // power iterations for lambda_max
+  bool reduced_quadrature = true;  // level operators on k + 1 Gauss points per direction (else 2k + 1)
 };
 
 template <typename R>
@@ -30,6 +31,7 @@ struct MgLevelT {
   double lam_max = 1.0;
   double cell_tol = 0.05;
   int cell_maxit = 40;
+  bool rq = true;  // quadrature of the level operator: k + 1 Gauss points per direction (else 2k + 1)
   hipStream_t stream = nullptr;
   R *rho = nullptr, *pfix = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *r2 = nullptr, *z = nullptr,
     *d = nullptr, *t = nullptr, *w = nullptr, *jbar = nullptr, *ev = nullptr, *sinv = nullptr;
@@ -48,12 +50,16 @@ struct MgT {
   int warm_eig_iters = 4;  // power iterations when restarting from the previous eigenvector
   KernelTimer* timer = nullptr;  // optional launch timing of the level kernels
   double smoothing_range = 15.0;
-  int setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k);
+  // tabs_by_k: 1D tables per degree with the operator's 2k + 1 Gauss points; tabs_rq_by_k: the same
+  // with k + 1 points (used when cfg.reduced_quadrature)
+  int setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k, const Tab* tabs_rq_by_k);
   void release();
   // nodal interpolation of pres_fixed (and of rho_for_fixed when rho_changed) to every level
   void set_coefficients(const double* rho_fine, const double* pfix_fine, bool rho_changed);
   int estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, double* scal_d, double* scal_h);
   void apply(const OpCtx& fine, const double* v, double* z);
+  // z = S_0 v with the finest level's operator (its precision, quadrature and cell solves)
+  void level_vmult(const OpCtx& fine, const double* v, double* z);
   // fine-level w = A^{-1} B p in precision R on the level's coefficient copies
   void schur_velocity(const OpCtx& fine, const R* p, R* w);
   ~MgT() { release(); }

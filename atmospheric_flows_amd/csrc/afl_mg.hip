@@ -296,7 +296,7 @@ static void level_schur(const MgLevelT<R>& L, const OpCtx& fine, const R* p, R* 
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
   const R tol = (R)L.cell_tol;
-  launch_schur_pair<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt, tol,
+  launch_schur_pair<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, L.pfix, f, cdt, tol,
                        L.cell_maxit, L.N, p, L.w, y);
 }
 
@@ -336,9 +336,9 @@ static void level_schur_cheb(const MgLevelT<R>& L, const OpCtx& fine, const R* p
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
-  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, f, (R)L.cell_tol,
+  launch_schur_w_r<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, f, (R)L.cell_tol,
                       L.cell_maxit, L.N, p, L.w, tm && L.lean() ? tm->next(kTimeLevelW) : nullptr);
-  launch_apply_C_cheb<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.pfix, cdt, L.N, L.w, p, L.jbar, rin, rout, (R)c1, (R)c2,
+  launch_apply_C_cheb<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.pfix, cdt, L.N, L.w, p, L.jbar, rin, rout, (R)c1, (R)c2,
                          d, x, tm ? tm->next(kTimeLevelCCheb) : nullptr);
 }
 
@@ -403,7 +403,7 @@ static void vcycle(MgT<R>& mg, const OpCtx& fine, int l) {
 static void interp_matrix(const Tab& to, int kto, const Tab& from, int kfrom, double (&E)[5][5]);
 
 template <typename R>
-int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k) {
+int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs_by_k, const Tab* tabs_rq_by_k) {
   smoother_degree = cfg.smoother_degree;
   coarse_degree = cfg.coarse_degree;
   smoothing_range = cfg.smoothing_range;
@@ -427,11 +427,13 @@ int MgT<R>::setup_levels(const OpCtx& fine, const MgConfig& cfg, const Tab* tabs
     L.stream = fine.stream;
     L.cell_tol = cfg.cell_tol;
     L.cell_maxit = cfg.cell_maxit;
+    L.rq = cfg.reduced_quadrature;
     const Tab& tb = tabs_by_k[L.k];
+    const Tab& tq = L.rq ? tabs_rq_by_k[L.k] : tb;  // quadrature of the level operator
     for (int q = 0; q < 9; ++q) {
-      for (int i = 0; i < 5; ++i) { L.T.S[q][i] = (R)tb.S[q][i]; L.T.D[q][i] = (R)tb.D[q][i]; }
-      L.T.W[q] = (R)tb.W[q];
-      L.T.QP[q] = (R)tb.QP[q];
+      for (int i = 0; i < 5; ++i) { L.T.S[q][i] = (R)tq.S[q][i]; L.T.D[q][i] = (R)tq.D[q][i]; }
+      L.T.W[q] = (R)tq.W[q];
+      L.T.QP[q] = (R)tq.QP[q];
     }
     for (int i = 0; i < 5; ++i) {
       L.T.ND[i] = (R)tb.ND[i];
@@ -499,7 +501,7 @@ void MgT<R>::set_coefficients(const double* rho_fine, const double* pfix_fine, b
                                                                                             pfix_fine, L.pfix);
       });
     }
-    if (rho_changed && L.lean()) launch_cell_rho_mass_inv<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.sinv);
+    if (rho_changed && L.lean()) launch_cell_rho_mass_inv<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.rho, L.sinv);
   }
 }
 
@@ -547,10 +549,18 @@ void MgT<R>::apply(const OpCtx& fine, const double* v, double* z) {
 }
 
 template <typename R>
+void MgT<R>::level_vmult(const OpCtx& fine, const double* v, double* z) {
+  MgLevelT<R>& L = levels[0];
+  k_convert<double, R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, v, L.b);
+  level_schur(L, fine, L.b, L.t);
+  k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.t, z);
+}
+
+template <typename R>
 void MgT<R>::schur_velocity(const OpCtx& fine, const R* p, R* w) {
   const MgLevelT<R>& L = levels[0];
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
-  launch_schur_w_r<R>(L.dim, L.k, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr,
+  launch_schur_w_r<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr,
                       (R)(coeff * fine.dt / (fine.Ma * fine.Ma)),
                       (R)L.cell_tol, L.cell_maxit, L.N, p, w);
 }

@@ -33,9 +33,9 @@ __device__ __attribute__((always_inline)) inline const double* src_at(const Srcs
   return v;
 }
 
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 struct Lds {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   static constexpr int EV = C::EV1 + C::EV2;
   static constexpr int IN = C::IN1 + C::IN2;
   static constexpr int W = EV > IN ? EV : IN;   // eval / integrate work area per cell
@@ -58,15 +58,15 @@ struct Lds {
   (void)t2; (void)i2; (void)fn;
 
 // Face geometry of the thread's face point on all 2*dim faces (outward normals, JxW).
-template <int DIM, int K, typename R = double>
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
 struct FacePts {
   R n[2 * DIM][DIM];
   R jxw[2 * DIM];
-  __device__ FacePts(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s) {
+  __device__ FacePts(const Geo& g, const TabT<R>& T, const Slot<DIM, K, Q>& s) {
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) {
       R nn[3];
-      face_geo<DIM, K, R>(g, T, s.ic, f, s.l, nn, jxw[f]);
+      face_geo<DIM, K, Q>(g, T, s.ic, f, s.l, nn, jxw[f]);
 #pragma unroll
       for (int d = 0; d < DIM; ++d) n[f][d] = nn[d];
     }
@@ -491,11 +491,11 @@ __global__ __launch_bounds__(AFL_NT) void k_inv_diag(Geo g, const Tab* __restric
 // ---------------------------------------------------------------------------------------
 // B: vmult_pressure (euler_operator.h:1510-1617). Volume -f p div v, faces +f {p} n . v
 // (own side), walls +f p n . v, with f = coeff dt / Ma^2. fn: face-node LDS of this cell.
-template <int DIM, int K, typename R = double>
-__device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
-                                    const Nbrs<DIM>& nb, const FacePts<DIM, K, R>& fp, const R* __restrict__ p,
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K, Q>& s, const LineGeo<DIM, K, R, Q>& lg,
+                                    const Nbrs<DIM>& nb, const FacePts<DIM, K, R, Q>& fp, const R* __restrict__ p,
                                     R f, R* t1, R* t2, R* i1, R* i2, R* fn, R* Fq, R (&out)[DIM][K + 1]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ;
   R pq[NQ], trp[2 * DIM][2];
   eval_line_faces<DIM, K, R>(p, s, nb, t1, t2, fn, T, pq, trp);
@@ -507,7 +507,7 @@ __device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g,
       for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + fc) * C::NFQ + s.l] = f * avg * fp.n[fc][a] * fp.jxw[fc];
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     R V[NQ], G[DIM][NQ];
@@ -544,9 +544,9 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, const Tab* __restrict
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // LDS per cell of the C apply. FP32 (BATCH): all fields (p_fix, w, and p for the D term) are
 // evaluated in one batched pass, so the work area holds every field's sweep and face nodes.
-template <int DIM, int K, bool WITH_D, bool BATCH>
+template <int DIM, int K, bool WITH_D, bool BATCH, int Q = 2 * K + 1>
 struct LdsC {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   static constexpr int NF = 1 + DIM;                     // batched: p_fix, w_0..w_{d-1} (p apart)
   static constexpr int NFF = 1 + DIM;                    // fields needed on the faces
   static constexpr int EV = BATCH ? NF * (C::EV1 + C::EV2) : C::EV1 + C::EV2;
@@ -562,23 +562,23 @@ struct LdsC {
 
 // Result rows of C (optionally fused with D p) for the thread's node row: out = (WITH_D ?
 // scaleD M pD - C w : C w). sh: the cell's work area (LdsC::WF), Fq: its face flux block (Lds::F).
-template <int DIM, int K, bool WITH_D, typename R>
-__device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K>& s, R* sh, R* Fq, R cdt,
+template <int DIM, int K, bool WITH_D, typename R, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K, Q>& s, R* sh, R* Fq, R cdt,
                                     const R* __restrict__ pfix, const R* __restrict__ w, int64_t N,
                                     const R* __restrict__ pD, R scaleD, R (&out)[K + 1]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ;
   constexpr bool BATCH = sizeof(R) == 4 && AFL_C_BATCH_F32;
-  using LC = LdsC<DIM, K, WITH_D, BATCH>;
+  using LC = LdsC<DIM, K, WITH_D, BATCH, Q>;
   R* t1 = sh;
   R* i1 = t1;
   R* i2 = t1 + C::IN1;
   const R gr = R(kGamma / (kGamma - 1.0)), igm1 = R(1.0 / (kGamma - 1.0)), half = R(0.5);
   const R sgn = WITH_D ? R(-1) : R(1);
-  LineGeo<DIM, K, R> lg;
+  LineGeo<DIM, K, R, Q> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K, R> fp(g, T, s);
+  const FacePts<DIM, K, R, Q> fp(g, T, s);
   R pq[NQ], G[DIM][NQ], V[NQ];
   R trp[2 * DIM][2], trw[DIM][2 * DIM][2];
   if constexpr (BATCH) {
@@ -650,22 +650,22 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       Fq[f * C::NFQ + s.l] = F;
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   integrate_line<DIM, K, WITH_D, true, R>(s, i1, i2, T, V, G, out);
   add_faces<DIM, K, R>(s, Fq, T, out, 0xffu);
 }
 
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // Optionally fused with D p (Schur second half): dst = scaleD * M p - C w.
-template <int DIM, int K, bool WITH_D, typename R = double>
+template <int DIM, int K, bool WITH_D, typename R = double, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
                                                      const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
                                                      R* __restrict__ dst) {
   const auto& T = *Tp;
-  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4 && AFL_C_BATCH_F32>;
-  __shared__ R shw[Cfg<DIM, K>::CPB * LC::WF];
-  Slot<DIM, K> s(g);
-  const int cbs = s.cb < Cfg<DIM, K>::CPB ? s.cb : 0;
+  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4 && AFL_C_BATCH_F32, Q>;
+  __shared__ R shw[Cfg<DIM, K, Q>::CPB * LC::WF];
+  Slot<DIM, K, Q> s(g);
+  const int cbs = s.cb < Cfg<DIM, K, Q>::CPB ? s.cb : 0;
   R out[K + 1];
   R* sh = shw + cbs * LC::WF;
   apply_C_rows<DIM, K, WITH_D, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, out);
@@ -677,23 +677,23 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __rest
 //   d = c1 d + c2 z;  x += d.
 // The block preconditioner is cell-local (LDS transposes of the cell's rows); pD, d and x are
 // read only inside the thread's own cell, so d and x (and pD == d or x) update in place.
-template <int DIM, int K, typename R>
+template <int DIM, int K, typename R, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 && sizeof(R) == 4 ? AFL_CHEB_WAVES : 1))) void k_apply_C_cheb(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
                                                           const R* __restrict__ w, const R* pD, R scaleD,
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
   const auto& T = *Tp;
-  using LC = LdsC<DIM, K, true, sizeof(R) == 4 && AFL_C_BATCH_F32>;
-  __shared__ R shw[Cfg<DIM, K>::CPB * LC::WF];
-  using C = Cfg<DIM, K>;
+  using LC = LdsC<DIM, K, true, sizeof(R) == 4 && AFL_C_BATCH_F32, Q>;
+  __shared__ R shw[Cfg<DIM, K, Q>::CPB * LC::WF];
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
-  Slot<DIM, K> s(g);
+  Slot<DIM, K, Q> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
   R* sh = shw + cbs * LC::WF;
   static_assert(2 * C::NPC <= LC::FOFF || LC::FOFF >= LC::W, "epilogue scratch must not overlap the face fluxes");
   R t[N1];
   apply_C_rows<DIM, K, true, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, t);
-  cell_sync<DIM, K>();  // the face fluxes and work area are read until here; the epilogue reuses them
+  cell_sync<DIM, K, Q>();  // the face fluxes and work area are read until here; the epilogue reuses them
   const bool ro = s.act && s.l < ROWS;
   const int64_t base = (s.act ? s.cell : 0) * NPC + s.l * N1;
   R* tmp = sh;
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
       tmp[s.l * N1 + i] = a;
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   const R* zr;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
@@ -728,7 +728,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
         tmp2[i * N1 + s.l] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     zr = tmp2;
   } else {
     if (s.act && s.l < N1 * N1) {
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
         tmp2[(iz * N1 + i) * N1 + ix] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < N1 * N1) {
       const int ix = s.l % N1, iy = s.l / N1;
       R v[N1];
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
         tmp[(i * N1 + iy) * N1 + ix] = a;
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     zr = tmp;
   }
   if (ro) {
@@ -783,11 +783,11 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 // up front, with no inner products and no convergence flags. On the terrain-following mesh the
 // Jacobian factorises exactly, so mass and D solves finish in one step and density-weighted
 // solves in a few. All NC components share every barrier. b rows in, x rows out.
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 struct SolvLds {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   static constexpr int EVF = C::EV1 + C::EV2;
-  static constexpr int INF = IntV<DIM, K>::I1 + IntV<DIM, K>::I2;
+  static constexpr int INF = IntV<DIM, K, Q>::I1 + IntV<DIM, K, Q>::I2;
   static constexpr int WF = EVF > INF ? EVF : INF;  // per component work area
   static constexpr int PER = WF + C::NPC;            // + the component's dofs
   // set-up scratch: co per line, line sums (3D), marginals, per-line ratio bounds
@@ -815,20 +815,20 @@ __device__ __attribute__((always_inline)) inline void inv_spd(R (&A)[N][N]) {
   }
 }
 
-template <int DIM, int K, int NC, typename R = double, bool SEP = true>
-__device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K>& s, const LineGeo<DIM, K, R>& lg,
-                                 const R (&co)[2 * K + 1], R* sl /* SolvLds::size(NC) per cell */,
+template <int DIM, int K, int NC, typename R = double, bool SEP = true, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T, const Slot<DIM, K, Q>& s, const LineGeo<DIM, K, R, Q>& lg,
+                                 const R (&co)[Q], R* sl /* SolvLds::size(NC) per cell */,
                                  R tol, int maxit, const R (&b)[NC][K + 1], R (&x)[NC][K + 1]) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ, N1 = C::N1, ROWS = C::ROWS, NPC = C::NPC, LT = C::LT;
-  constexpr int WF = SolvLds<DIM, K>::WF;
+  constexpr int WF = SolvLds<DIM, K, Q>::WF;
   R* dl = sl;                 // NC*NPC dofs
   R* wk = sl + NC * NPC;      // NC*WF work
   R* t1 = wk;
   R* t2 = wk + NC * C::EV1;
   R* i1 = wk;
-  R* i2 = wk + NC * IntV<DIM, K>::I1;
-  R* Mi = sl + SolvLds<DIM, K>::sep_off(NC);  // [DIM][N1][N1]: x, (y,) z inverse 1D matrices
+  R* i2 = wk + NC * IntV<DIM, K, Q>::I1;
+  R* Mi = sl + SolvLds<DIM, K, Q>::sep_off(NC);  // [DIM][N1][N1]: x, (y,) z inverse 1D matrices
   const bool row_owner = s.act && s.l < ROWS;
   __shared__ int nit_blk;
   // ---- set-up: marginals of co, separable weights, ratio bounds, 1D inverses ----
@@ -845,7 +845,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
     for (int q = 0; q < NQ; ++q) sco[s.l * NQ + q] = co[q];
     if (DIM == 3) sL[s.l] = lsum;
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   if (s.act && s.l < NQ) {
     // vertical marginal: sum over the lines of co at qz = l (and its reciprocal)
     R c = R(0);
@@ -862,7 +862,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
       smg[NQ + s.l] = by;  // sum over qx of the lines (qx, qy = l)
     }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   R total = R(0);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) total += smg[q];
@@ -904,7 +904,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
 #pragma unroll
       for (int j = 0; j < N1; ++j) Mi[(s.l * N1 + i) * N1 + j] = Am[i][j];
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   R cmin = R(1), cmax = R(1);
   if (s.act) {
     R mn = R(1e30), mx = R(0);
@@ -967,7 +967,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (DIM == 2) {
       if (s.act && s.l < N1) {
         R v[NC][N1];
@@ -989,7 +989,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
           }
         }
       }
-      cell_sync<DIM, K>();
+      cell_sync<DIM, K, Q>();
       if (row_owner)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
@@ -1012,7 +1012,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
           }
         }
       }
-      cell_sync<DIM, K>();
+      cell_sync<DIM, K, Q>();
       if (s.act && s.l < N1 * N1) {
         const int ix = s.l % N1, iy = s.l / N1;
 #pragma unroll
@@ -1029,14 +1029,14 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
           }
         }
       }
-      cell_sync<DIM, K>();
+      cell_sync<DIM, K, Q>();
       if (row_owner)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
           for (int i = 0; i < N1; ++i) z[c][i] = zsc * dl[c * NPC + s.l * N1 + i];
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
   };
   // y = A v for all components
   auto apply = [&](const R (&v)[NC][N1], R (&y)[NC][N1]) {
@@ -1045,7 +1045,7 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int i = 0; i < N1; ++i) dl[c * NPC + s.l * N1 + i] = v[c][i];
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     R vq[NC][NQ];
     eval_lines_lds<DIM, K, NC, R>(dl, s, t1, t2, T, vq);
 #pragma unroll
@@ -1119,29 +1119,29 @@ __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, const Tab* __restr
 }
 
 // Schur first half: w = A^{-1} B p (fused B apply + cell-local A solve).
-template <int DIM, int K, typename R = double>
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ rho, R f, int64_t N,
                                                      const R* __restrict__ p, R* __restrict__ w, R tol, int maxit) {
   const auto& T = *Tp;
-  constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;  // B phase: work + face nodes + face fluxes
-  constexpr int PS = SolvLds<DIM, K>::size(DIM);             // solver phase
+  constexpr int PB = Lds<DIM, K, Q>::WF + Lds<DIM, K, Q>::F * DIM;  // B phase: work + face nodes + face fluxes
+  constexpr int PS = SolvLds<DIM, K, Q>::size(DIM);             // solver phase
   constexpr int PC = PB > PS ? PB : PS;
-  __shared__ R shw[Cfg<DIM, K>::CPB * PC];
-  using C = Cfg<DIM, K>;
+  __shared__ R shw[Cfg<DIM, K, Q>::CPB * PC];
+  using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ, N1 = C::N1;
-  Slot<DIM, K> s(g);
+  Slot<DIM, K, Q> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
   R* base = shw + cbs * PC;
   R* t1 = base;
   R* t2 = base + C::EV1;
   R* i1 = base;
   R* i2 = base + C::IN1;
-  R* fn = base + Lds<DIM, K>::W;
-  R* Fq = base + Lds<DIM, K>::WF;
-  LineGeo<DIM, K, R> lg;
+  R* fn = base + Lds<DIM, K, Q>::W;
+  R* Fq = base + Lds<DIM, K, Q>::WF;
+  LineGeo<DIM, K, R, Q> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K, R> fp(g, T, s);
+  const FacePts<DIM, K, R, Q> fp(g, T, s);
   R bp[DIM][N1];
   apply_B_rows<DIM, K, R>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, Fq, bp);
   R co[NQ];
@@ -1158,9 +1158,9 @@ __global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, const TabT<R>* __rest
 
 // rows (held by the row owners) <- (MI x MI [x MI]) rows for NC components: the inverse of the
 // reference-cell mass matrix, one LDS transpose per further axis. tmp: 2*NC*NPC values per cell.
-template <int DIM, int K, int NC, typename R>
-__device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<DIM, K>& s, const TabT<R>& T, R* tmp, R (&rows)[NC][K + 1]) {
-  using C = Cfg<DIM, K>;
+template <int DIM, int K, int NC, typename R, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<DIM, K, Q>& s, const TabT<R>& T, R* tmp, R (&rows)[NC][K + 1]) {
+  using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
   R* t2 = tmp + NC * NPC;
   const bool ro = s.act && s.l < ROWS;
@@ -1175,7 +1175,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         tmp[c * NPC + s.l * N1 + i] = a;
       }
   }
-  cell_sync<DIM, K>();
+  cell_sync<DIM, K, Q>();
   const R* res;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
@@ -1193,7 +1193,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     res = t2;
   } else {
     if (s.act && s.l < N1 * N1) {
@@ -1212,7 +1212,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     if (s.act && s.l < N1 * N1) {
       const int ix = s.l % N1, iy = s.l / N1;
 #pragma unroll
@@ -1229,7 +1229,7 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
         }
       }
     }
-    cell_sync<DIM, K>();
+    cell_sync<DIM, K, Q>();
     res = tmp;
   }
   if (ro)
@@ -1243,26 +1243,26 @@ __device__ __attribute__((always_inline)) inline void cell_mi_apply(const Slot<D
 // mean-density block P = (sum_q rho JxW) (M1 x M1 [x M1]) in place of A (sinv[c] = 1 / sum_q rho
 // JxW of cell c, refreshed with the stage coefficients). P^{-1} A has its spectrum within the
 // cell's relative variation of rho J about the mean, a few percent.
-template <int DIM, int K, typename R>
+template <int DIM, int K, typename R, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 && sizeof(R) == 4 ? AFL_WLVL_WAVES : 1))) void k_schur_w_lvl(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ sinv, R f, int64_t N,
                                                          const R* __restrict__ p, R* __restrict__ w) {
   const auto& T = *Tp;
-  using C = Cfg<DIM, K>;
-  constexpr int PB = Lds<DIM, K>::WF + Lds<DIM, K>::F * DIM;
+  using C = Cfg<DIM, K, Q>;
+  constexpr int PB = Lds<DIM, K, Q>::WF + Lds<DIM, K, Q>::F * DIM;
   constexpr int PM = 2 * DIM * C::NPC;
   constexpr int PC = PB > PM ? PB : PM;
   __shared__ R shw[C::CPB * PC];
   constexpr int N1 = C::N1;
-  Slot<DIM, K> s(g);
+  Slot<DIM, K, Q> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
   R* base = shw + cbs * PC;
-  LineGeo<DIM, K, R> lg;
+  LineGeo<DIM, K, R, Q> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
-  const FacePts<DIM, K, R> fp(g, T, s);
+  const FacePts<DIM, K, R, Q> fp(g, T, s);
   R bp[DIM][N1];
   apply_B_rows<DIM, K, R>(g, T, s, lg, nb, fp, p, f, base, base + C::EV1, base, base + C::IN1,
-                          base + Lds<DIM, K>::W, base + Lds<DIM, K>::WF, bp);
+                          base + Lds<DIM, K, Q>::W, base + Lds<DIM, K, Q>::WF, bp);
   // the mass-inverse transposes use [0, PM), disjoint from the face fluxes add_faces may still read
   cell_mi_apply<DIM, K, DIM, R>(s, T, base, bp);
   const R sc = s.act ? sinv[s.cell] : R(0);
@@ -1275,11 +1275,11 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 }
 
 // sinv[c] = 1 / sum_q rho(q) JxW(q) of cell c (one thread per cell)
-template <int DIM, int K, typename R>
+template <int DIM, int K, typename R, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ rho,
                                                                R* __restrict__ sinv) {
   const auto& T = *Tp;
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ, N1 = C::N1, NPC = C::NPC;
   for (int64_t c = blockIdx.x * (int64_t)AFL_NT + threadIdx.x; c < g.ncells; c += (int64_t)gridDim.x * AFL_NT) {
     int ic[3];
@@ -1289,7 +1289,7 @@ __global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, const TabT<
     const R* r = rho + c * NPC;
     double acc = 0.0;
     for (int l = 0; l < C::LT; ++l) {
-      LineGeo<DIM, K, R> lg;
+      LineGeo<DIM, K, R, Q> lg;
       lg.init(g, T, ic, l);
       const int qx = l % NQ, qy = DIM == 3 ? l / NQ : 0;
       for (int qz = 0; qz < NQ; ++qz) {
@@ -1309,9 +1309,9 @@ __global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, const TabT<
 
 // ---------------------------------------------------------------------------------------
 // Launchers (dispatch on dim and degree).
-template <int DIM, int K>
+template <int DIM, int K, int Q = 2 * K + 1>
 static dim3 grid_for(int ncells) {
-  using C = Cfg<DIM, K>;
+  using C = Cfg<DIM, K, Q>;
   return dim3((ncells + C::CPB - 1) / C::CPB);
 }
 
@@ -1458,91 +1458,74 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   });
 }
 
+// Multigrid level operators: besides dim and degree, the quadrature rule of the level, k + 1
+// (rq) or 2k + 1 Gauss points per direction.
+#define AFL_DISPATCH_LQ(DIMV, KV, RQ, ...)                          \
+  AFL_DISPATCH(DIMV, KV, {                                           \
+    if (RQ) { constexpr int Q_ = K_ + 1; __VA_ARGS__; }              \
+    else { constexpr int Q_ = 2 * K_ + 1; __VA_ARGS__; }             \
+  })
+
 // Level Schur apply y = D p - C A^{-1} B p in precision R (multigrid levels).
 template <typename R>
-void launch_schur_pair(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
-  AFL_DISPATCH(dim, k, {
-    const dim3 gr = grid_for<D_, K_>(geo.ncells);
-    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, sinv, f, N, p, w);
-    else k_schur_w<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
-    k_apply_C<D_, K_, true, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
+void launch_schur_pair(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
+                       const R* sinv, const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, R* y) {
+  AFL_DISPATCH_LQ(dim, k, rq, {
+    const dim3 gr = grid_for<D_, K_, Q_>(geo.ncells);
+    const dim3 bl(Cfg<D_, K_, Q_>::NT);
+    if (sinv) k_schur_w_lvl<D_, K_, R, Q_><<<gr, bl, 0, st>>>(geo, T, sinv, f, N, p, w);
+    else k_schur_w<D_, K_, R, Q_><<<gr, bl, 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+    k_apply_C<D_, K_, true, R, Q_><<<gr, bl, 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), y);
   });
 }
 // w = A^{-1} B p alone in precision R (the first half of the level operator)
 template <typename R>
-void launch_schur_w_r(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                      R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev) {
-  AFL_DISPATCH(dim, k, {
+void launch_schur_w_r(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
+                      const R* sinv, R f, R tol, int maxit, int64_t N, const R* p, R* w, hipEvent_t* ev) {
+  AFL_DISPATCH_LQ(dim, k, rq, {
+    const dim3 gr = grid_for<D_, K_, Q_>(geo.ncells);
+    const dim3 bl(Cfg<D_, K_, Q_>::NT);
     if (sinv && ev)
-      hipExtLaunchKernelGGL(k_schur_w_lvl<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st, ev[0], ev[1],
-                            0, geo, T, sinv, f, N, p, w);
+      hipExtLaunchKernelGGL(k_schur_w_lvl<D_, K_, R, Q_>, gr, bl, 0, st, ev[0], ev[1], 0, geo, T, sinv, f, N, p, w);
     else if (sinv)
-      k_schur_w_lvl<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, sinv, f, N, p, w);
+      k_schur_w_lvl<D_, K_, R, Q_><<<gr, bl, 0, st>>>(geo, T, sinv, f, N, p, w);
     else
-      k_schur_w<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
+      k_schur_w<D_, K_, R, Q_><<<gr, bl, 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
   });
 }
 template <typename R>
-void launch_cell_rho_mass_inv(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho,
-                              R* sinv) {
+void launch_cell_rho_mass_inv(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st,
+                              const R* rho, R* sinv) {
   const int nb = (int)((geo.ncells + AFL_NT - 1) / AFL_NT);
-  AFL_DISPATCH(dim, k, { k_cell_rho_mass_inv<D_, K_, R><<<dim3(nb), dim3(AFL_NT), 0, st>>>(geo, T, rho, sinv); });
+  AFL_DISPATCH_LQ(dim, k, rq, { k_cell_rho_mass_inv<D_, K_, R, Q_><<<dim3(nb), dim3(AFL_NT), 0, st>>>(geo, T, rho, sinv); });
 }
-template void launch_cell_rho_mass_inv<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
-                                              float*);
-template void launch_cell_rho_mass_inv<double>(int, int, const Geo&, const TabT<double>*, hipStream_t,
-                                               const double*, double*);
-template void launch_schur_w_r<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
-                                      const float*, float, float, int, int64_t, const float*, float*, hipEvent_t*);
-template void launch_schur_w_r<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
-                                       const double*, double, double, int, int64_t, const double*, double*,
-                                       hipEvent_t*);
-
-// w = A^{-1} B p followed by the fused C/D apply and Chebyshev step (see k_apply_C_cheb)
+// S p (second half) fused with one Chebyshev step (see k_apply_C_cheb)
 template <typename R>
-void launch_schur_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* rho, const R* sinv,
-                       const R* pfix, R f, R cdt, R tol, int maxit, int64_t N, const R* p, R* w, const R* jbar,
-                       const R* rin, R* rout, R c1, R c2, R* d, R* x) {
-  AFL_DISPATCH(dim, k, {
-    const dim3 gr = grid_for<D_, K_>(geo.ncells);
-    if (sinv) k_schur_w_lvl<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, sinv, f, N, p, w);
-    else k_schur_w<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, rho, f, N, p, w, tol, maxit);
-    k_apply_C_cheb<D_, K_, R><<<gr, dim3(Cfg<D_, K_>::NT), 0, st>>>(geo, T, cdt, N, pfix, w, p, (R)(1.0 / (kGamma - 1.0)), jbar,
-                                                           rin, rout, c1, c2, d, x);
-  });
-}
-template <typename R>
-void launch_apply_C_cheb(int dim, int k, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* pfix, R cdt,
-                         int64_t N, const R* w, const R* p, const R* jbar, const R* rin, R* rout, R c1, R c2, R* d,
-                         R* x, hipEvent_t* ev) {
+void launch_apply_C_cheb(int dim, int k, bool rq, const Geo& geo, const TabT<R>* T, hipStream_t st, const R* pfix,
+                         R cdt, int64_t N, const R* w, const R* p, const R* jbar, const R* rin, R* rout, R c1, R c2,
+                         R* d, R* x, hipEvent_t* ev) {
   const R sD = (R)(1.0 / (kGamma - 1.0));
-  AFL_DISPATCH(dim, k, {
+  AFL_DISPATCH_LQ(dim, k, rq, {
+    const dim3 gr = grid_for<D_, K_, Q_>(geo.ncells);
+    const dim3 bl(Cfg<D_, K_, Q_>::NT);
     if (ev)
-      hipExtLaunchKernelGGL(k_apply_C_cheb<D_, K_, R>, grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st, ev[0], ev[1],
-                            0, geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
+      hipExtLaunchKernelGGL(k_apply_C_cheb<D_, K_, R, Q_>, gr, bl, 0, st, ev[0], ev[1], 0, geo, T, cdt, N, pfix, w, p,
+                            sD, jbar, rin, rout, c1, c2, d, x);
     else
-      k_apply_C_cheb<D_, K_, R><<<grid_for<D_, K_>(geo.ncells), dim3(Cfg<D_, K_>::NT), 0, st>>>(
-          geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
+      k_apply_C_cheb<D_, K_, R, Q_><<<gr, bl, 0, st>>>(geo, T, cdt, N, pfix, w, p, sD, jbar, rin, rout, c1, c2, d, x);
   });
 }
-template void launch_apply_C_cheb<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*, float,
-                                         int64_t, const float*, const float*, const float*, const float*, float*,
-                                         float, float, float*, float*, hipEvent_t*);
-template void launch_apply_C_cheb<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
-                                          double, int64_t, const double*, const double*, const double*,
-                                          const double*, double*, double, double, double*, double*, hipEvent_t*);
-template void launch_schur_cheb<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
-                                       const float*, const float*, float, float, float, int, int64_t, const float*, float*,
-                                       const float*, const float*, float*, float, float, float*, float*);
-template void launch_schur_cheb<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
-                                        const double*, const double*, double, double, double, int, int64_t, const double*, double*,
-                                        const double*, const double*, double*, double, double, double*, double*);
-template void launch_schur_pair<float>(int, int, const Geo&, const TabT<float>*, hipStream_t, const float*,
-                                       const float*, const float*, float, float, float, int, int64_t, const float*, float*,
-                                       float*);
-template void launch_schur_pair<double>(int, int, const Geo&, const TabT<double>*, hipStream_t, const double*,
-                                        const double*, const double*, double, double, double, int, int64_t,
-                                        const double*, double*, double*);
+
+#define AFL_LEVEL_INSTANCES(R)                                                                                       \
+  template void launch_schur_pair<R>(int, int, bool, const Geo&, const TabT<R>*, hipStream_t, const R*, const R*,    \
+                                     const R*, R, R, R, int, int64_t, const R*, R*, R*);                               \
+  template void launch_schur_w_r<R>(int, int, bool, const Geo&, const TabT<R>*, hipStream_t, const R*, const R*, R, \
+                                    R, int, int64_t, const R*, R*, hipEvent_t*);                                       \
+  template void launch_cell_rho_mass_inv<R>(int, int, bool, const Geo&, const TabT<R>*, hipStream_t, const R*, R*); \
+  template void launch_apply_C_cheb<R>(int, int, bool, const Geo&, const TabT<R>*, hipStream_t, const R*, R, int64_t, \
+                                       const R*, const R*, const R*, const R*, R*, R, R, R*, R*, hipEvent_t*);
+AFL_LEVEL_INSTANCES(float)
+AFL_LEVEL_INSTANCES(double)
+#undef AFL_LEVEL_INSTANCES
 
 }  // namespace afl

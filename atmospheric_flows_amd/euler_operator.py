@@ -51,7 +51,7 @@ def load_library():
     for n in ("afl_vmult_rhs_rho_update", "afl_vmult_rhs_velocity_fixed", "afl_vmult_rhs_pressure"):
         getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.c_int]
     for n in ("afl_vmult", "afl_vmult_pressure", "afl_vmult_enthalpy", "afl_solve_mass", "afl_vmult_schur_velocity",
-              "afl_mg_vmult", "afl_vmult_schur_velocity_f32"):
+              "afl_mg_vmult", "afl_mg_level_vmult", "afl_vmult_schur_velocity_f32"):
         getattr(L, n).argtypes = [vp, dp, dp]
     L.afl_compute_diagonal.argtypes = [vp]
     L.afl_get_matrix_diagonal_inverse.argtypes = [vp, dp]
@@ -268,6 +268,11 @@ class EulerOperator:
     def mg_vmult(self, dst, src):
         """Schur preconditioner apply (p-multigrid V-cycle of the selected schur_precond)."""
         self._chk(self.L.afl_mg_vmult(self.h, _ptr(dst), _ptr(src)))
+        return dst
+
+    def mg_level_vmult(self, dst, src):
+        """The finest multigrid level's Schur operator (level precision and quadrature) applied to src."""
+        self._chk(self.L.afl_mg_level_vmult(self.h, _ptr(dst), _ptr(src)))
         return dst
 
     def vmult_schur_velocity_f32(self, w, p):

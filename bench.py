@@ -260,7 +260,7 @@ def main():
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": ach, "bytes_per_launch": bytes_dom,
                      "avg_launch_us": level[dom]["avg_us"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(dom.replace("<float>", f"<{dim}, {args.degree}, float>"), args)},
+                     "traffic": profiled_traffic(dom.replace("<float>", f"<{dim}, {args.degree}, float, {args.degree + 1}>"), args)},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

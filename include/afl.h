@@ -90,6 +90,12 @@ int afl_vmult_schur_velocity(afl_operator* op, double* w, const double* p);
  * the current stage coefficients). Requires solver_params.schur_precond = AFL_SCHUR_MG_FP32 or
  * AFL_SCHUR_MG_FP64; FP64 vectors in and out, level arithmetic in the selected precision. */
 int afl_mg_vmult(afl_operator* op, double* dst, const double* src);
+/* The operator that V-cycle smooths on its finest level, dst = S_0 src: the Schur operator of
+ * the current stage coefficients rediscretised in the level precision on mg_quad_points Gauss
+ * points per direction, with the level cell solves (mean-density block when mg_cell_tol >= 0.05).
+ * No counterpart in the reference (whose preconditioners are deal.II's); it exposes the
+ * preconditioner's operator for testing. Requires schur_precond = AFL_SCHUR_MG_FP32 or _FP64. */
+int afl_mg_level_vmult(afl_operator* op, double* dst, const double* src);
 /* The FP32 fine-level half of that V-cycle's operator, w = A^{-1} B p in single precision on the
  * FP32 copies of rho_for_fixed (requires schur_precond = AFL_SCHUR_MG_FP32). */
 int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p);

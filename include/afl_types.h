@@ -46,7 +46,10 @@ typedef struct afl_solver_params {
   int32_t gmres_restart;   /* Krylov basis size before restart [deal.II default 28] */
   double fp_tol;           /* fixed-point tolerance on the relative inf-norm change of p [1e-10] */
   int32_t fp_max_iter;     /* [100] */
-  int32_t pad_;
+  int32_t mg_quad_points;  /* Gauss points per direction of the multigrid level operators: 0 or k + 1
+                            * [default; the level operator of the preconditioner only], or 2k + 1 (the
+                            * operator's own rule). Results are unaffected: the outer FGMRES applies
+                            * the exact Schur operator. */
   /* Schur preconditioner (zero-initialised fields select the reference's plain path):
    * AFL_SCHUR_JACOBI: GMRES + point Jacobi on D, all FP64;
    * AFL_SCHUR_MG_FP32 / AFL_SCHUR_MG_FP64: flexible GMRES (FP64) + p-multigrid V-cycle with

@@ -119,8 +119,10 @@ double lagrange_deriv(const std::vector<double>& xn, int i, double x) {
 struct Basis {
   int k = 0, n1 = 0, nq = 0;
   std::vector<double> nodes, qp, qw, S, D, e0, e1;  // S[q*n1+i] = phi_i(qp[q])
-  void init(int k_, int type) {
-    k = k_; n1 = k + 1; nq = 2 * k + 1;
+  int type = 0;
+  // degree k_ on nq_ Gauss points per direction (default 2k + 1)
+  void init(int k_, int type_, int nq_ = 0) {
+    k = k_; n1 = k + 1; nq = nq_ > 0 ? nq_ : 2 * k + 1; type = type_;
     support_points(k, type, nodes);
     gauss_points(nq, qp, qw);
     S.resize(nq * n1); D.resize(nq * n1); e0.resize(n1); e1.resize(n1);
@@ -1547,6 +1549,7 @@ double linf(const double* a, int64_t n) {
   return m;
 }
 
+void setup_quadrature(orc_operator& o, int nq);
 }  // namespace
 
 // =======================================================================================
@@ -1560,10 +1563,6 @@ orc_operator* orc_create(const afl_mesh_desc* md, double mach, double froude, do
   op->dim = md->dim;
   op->mesh.dim = md->dim; op->mesh.k = md->degree;
   op->B.init(md->degree, md->node_type);
-  op->n1 = op->B.n1; op->nq = op->B.nq;
-  op->npc = op->nqc = 1;
-  for (int d = 0; d < op->dim; ++d) { op->npc *= op->n1; op->nqc *= op->nq; }
-  op->nfq = op->dim == 2 ? op->nq : op->nq * op->nq;
   Mesh& m = op->mesh;
   m.nc[0] = md->n_cells[0];
   if (op->dim == 2) { m.nc[1] = md->n_cells[2]; m.nc[2] = 1; }
@@ -1574,11 +1573,36 @@ orc_operator* orc_create(const afl_mesh_desc* md, double mach, double froude, do
   m.lo[op->dim - 1] = 0.0; m.hc[op->dim - 1] = md->z_top / m.nc[op->dim - 1];
   m.H = md->z_top; m.h0 = md->hill_height; m.a = md->hill_width;
   m.periodic[0] = md->periodic[0]; m.periodic[1] = op->dim == 3 ? md->periodic[1] : 0; m.periodic[2] = 0;
+  setup_quadrature(*op, 0);
   op->N = (int64_t)m.ncells * op->npc;
   op->Ma = mach; op->Fr = froude; op->dt = dt;
   op->rho_fix.assign((size_t)op->N, 1.0);
   op->p_fix.assign((size_t)op->N, 0.0);
   op->u_fix.assign((size_t)op->N * op->dim, 0.0);
+  return op;
+}
+
+void orc_destroy(orc_operator* op) { delete op; }
+
+// Gauss points per direction of every integral (volume and face): 0 restores the operator's
+// 2k + 1; k + 1 gives the level operator of the GPU multigrid preconditioner.
+int orc_set_quadrature(orc_operator* op, int nq) {
+  if (!op || nq < 0 || nq > 9 || (nq > 0 && nq < op->B.k + 1)) { g_err = "bad quadrature"; return 1; }
+  setup_quadrature(*op, nq);
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+void setup_quadrature(orc_operator& o, int nq) {
+  orc_operator* op = &o;
+  op->B.init(op->B.k, op->B.type, nq);
+  op->n1 = op->B.n1; op->nq = op->B.nq;
+  op->npc = op->nqc = 1;
+  for (int d = 0; d < op->dim; ++d) { op->npc *= op->n1; op->nqc *= op->nq; }
+  op->nfq = op->dim == 2 ? op->nq : op->nq * op->nq;
+  op->N = (int64_t)op->mesh.ncells * op->npc;
   // full tensor-product basis table phi_i(q)
   op->full_phi.resize((size_t)op->npc * op->nqc);
   for (int i = 0; i < op->npc; ++i)
@@ -1590,10 +1614,11 @@ orc_operator* orc_create(const afl_mesh_desc* md, double mach, double froude, do
       op->full_phi[(size_t)i * op->nqc + q] = v;
     }
   factor_cells(*op, op->chol_mass, 1.0, nullptr);
-  return op;
+  op->chol_A_valid = false;
 }
+}  // namespace
 
-void orc_destroy(orc_operator* op) { delete op; }
+extern "C" {
 
 int orc_set_dt(orc_operator* op, double dt) { op->dt = dt; return 0; }
 int orc_set_mach(orc_operator* op, double v) { op->Ma = v; return 0; }

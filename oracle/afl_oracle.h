@@ -24,6 +24,10 @@ typedef struct orc_operator orc_operator;
 orc_operator* orc_create(const afl_mesh_desc* mesh, double mach, double froude, double dt);
 void orc_destroy(orc_operator* op);
 const char* orc_last_error(void);
+/* Gauss points per direction of every volume and face integral: 0 restores 2k + 1 (the
+ * reference's QGauss(2k+1)); k + 1 gives the level operator of the GPU multigrid
+ * preconditioner. Resets the fixed fields' factorisations. */
+int orc_set_quadrature(orc_operator* op, int nq);
 
 int orc_set_dt(orc_operator* op, double dt);
 int orc_set_mach(orc_operator* op, double mach);

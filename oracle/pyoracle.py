@@ -33,6 +33,7 @@ def lib():
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.POINTER(MeshDesc), C.c_double, C.c_double, C.c_double]
         L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_quadrature.argtypes = [C.c_void_p, C.c_int]
         L.orc_last_error.restype = C.c_char_p
         for name in ("orc_set_dt", "orc_set_mach", "orc_set_froude"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_double]
@@ -103,6 +104,7 @@ class OracleOperator:
     def set_dt(self, v): self._chk(self.L.orc_set_dt(self.h, v))
     def set_mach(self, v): self._chk(self.L.orc_set_mach(self.h, v))
     def set_froude(self, v): self._chk(self.L.orc_set_froude(self.h, v))
+    def set_quadrature(self, nq): self._chk(self.L.orc_set_quadrature(self.h, nq))
     def set_hyperbolic_stage(self, s): self._chk(self.L.orc_set_hyperbolic_stage(self.h, s))
     def set_ns_stage(self, s): self._chk(self.L.orc_set_ns_stage(self.h, s))
     def set_rho_for_fixed(self, a): self._chk(self.L.orc_set_rho_for_fixed(self.h, _ptr(np.ascontiguousarray(a))))
