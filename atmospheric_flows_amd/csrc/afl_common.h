@@ -304,19 +304,30 @@ __device__ __attribute__((always_inline)) inline int face_node(int dir, int side
 }
 
 // ---------------------------------------------------------------------------------------
-// Evaluate a scalar field (global dofs) at the quadrature points of the thread's line.
-// All threads of the block must call it (it contains barriers).
+// The thread's node row of a scalar field (global dofs): row l of its cell for the row owners
+// (l < ROWS), zeros elsewhere. Loading rows ahead of the barriers of a kernel lets their memory
+// latency overlap earlier work.
 template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
-__device__ __attribute__((always_inline)) inline void eval_line(const R* __restrict__ f, const Slot<DIM, K, Q>& s, R* t1, R* t2,
-                                 const TabT<R>& T, R (&v)[Q]) {
+__device__ __attribute__((always_inline)) inline void load_row(const R* __restrict__ f, const Slot<DIM, K, Q>& s,
+                                                                R (&d)[K + 1]) {
+  using C = Cfg<DIM, K, Q>;
+  const R* u = f + (s.act ? s.cell : 0) * C::NPC;
+#pragma unroll
+  for (int i = 0; i < C::N1; ++i) d[i] = R(0);
+  if (s.act && s.l < C::ROWS)
+#pragma unroll
+    for (int i = 0; i < C::N1; ++i) d[i] = u[s.l * C::N1 + i];
+}
+
+// Evaluate a scalar field at the quadrature points of the thread's line from the node rows that
+// load_row gave the row owners. All threads of the block must call it (it contains barriers).
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_line_row(const R (&d)[K + 1], const Slot<DIM, K, Q>& s, R* t1,
+                                                                     R* t2, const TabT<R>& T, R (&v)[Q]) {
   using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NQ = C::NQ;
-  const R* u = f + (s.act ? s.cell : 0) * C::NPC;
   if (DIM == 2) {
     if (s.act && s.l < N1) {
-      R d[N1];
-#pragma unroll
-      for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         R a = 0.0;
@@ -341,9 +352,6 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
     cell_sync<DIM, K, Q>();
   } else {
     if (s.act && s.l < N1 * N1) {
-      R d[N1];
-#pragma unroll
-      for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         R a = 0.0;
@@ -381,6 +389,20 @@ __device__ __attribute__((always_inline)) inline void eval_line(const R* __restr
     }
     cell_sync<DIM, K, Q>();
   }
+}
+
+// Evaluate a scalar field (global dofs) at the quadrature points of the thread's line.
+// All threads of the block must call it (it contains barriers).
+template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
+__device__ __attribute__((always_inline)) inline void eval_line(const R* __restrict__ f, const Slot<DIM, K, Q>& s, R* t1, R* t2,
+                                 const TabT<R>& T, R (&v)[Q]) {
+  using C = Cfg<DIM, K, Q>;
+  const R* u = f + (s.act ? s.cell : 0) * C::NPC;
+  R d[K + 1];  // read by the row owners only
+  if (s.act && s.l < C::ROWS)
+#pragma unroll
+    for (int i = 0; i < C::N1; ++i) d[i] = u[s.l * C::N1 + i];
+  eval_line_row<DIM, K, R>(d, s, t1, t2, T, v);
 }
 
 // Reference-coordinate gradient of a scalar field at the quadrature points of the thread's line:

@@ -581,6 +581,8 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
   const FacePts<DIM, K, R, Q> fp(g, T, s);
   R pq[NQ], G[DIM][NQ], V[NQ];
   R trp[2 * DIM][2], trw[DIM][2 * DIM][2];
+  R prow[K + 1];  // pD's node row, loaded ahead of the evaluations' barriers
+  if (WITH_D) load_row<DIM, K, R>(pD, s, prow);
   if constexpr (BATCH) {
     const R* fl[LC::NF];
     fl[0] = pfix;
@@ -595,7 +597,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       for (int d = 0; d < DIM; ++d) G[d][q] = sgn * (-cdt * gr * v[0][q] * v[1 + d][q]);
     }
     if (WITH_D) {
-      eval_line<DIM, K, R>(pD, s, t1, t1 + C::EV1, T, V);
+      eval_line_row<DIM, K, R>(prow, s, t1, t1 + C::EV1, T, V);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
     }
@@ -619,7 +621,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       for (int q = 0; q < NQ; ++q) G[d][q] = sgn * (-cdt * gr * pq[q] * wq[q]);
     }
     if (WITH_D) {
-      eval_line<DIM, K, R>(pD, s, t1, t2, T, V);
+      eval_line_row<DIM, K, R>(prow, s, t1, t2, T, V);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
     }
@@ -691,18 +693,25 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   const int cbs = s.cb < C::CPB ? s.cb : 0;
   R* sh = shw + cbs * LC::WF;
   static_assert(2 * C::NPC <= LC::FOFF || LC::FOFF >= LC::W, "epilogue scratch must not overlap the face fluxes");
+  const bool ro = s.act && s.l < ROWS;
+  const int64_t base = (s.act ? s.cell : 0) * NPC + s.l * N1;
+  // the epilogue's inputs are loaded first, so their latency overlaps the C apply (d and x are
+  // read and written only by their row's thread, after every read of pD == d or x)
+  R rr[N1], dr[N1], xr[N1];
+  load_row<DIM, K, R>(rin, s, rr);
+  load_row<DIM, K, R>(d, s, dr);
+  load_row<DIM, K, R>(x, s, xr);
+  const R zs = s.act ? R(kGamma - 1.0) / jbar[s.cell] : R(0);
   R t[N1];
   apply_C_rows<DIM, K, true, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, t);
   cell_sync<DIM, K, Q>();  // the face fluxes and work area are read until here; the epilogue reuses them
-  const bool ro = s.act && s.l < ROWS;
-  const int64_t base = (s.act ? s.cell : 0) * NPC + s.l * N1;
   R* tmp = sh;
   R* tmp2 = sh + NPC;
   if (ro) {
     R r[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
-      r[i] = rin[base + i] - t[i];
+      r[i] = rr[i] - t[i];
       rout[base + i] = r[i];
     }
 #pragma unroll
@@ -762,12 +771,11 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
     zr = tmp;
   }
   if (ro) {
-    const R zs = R(kGamma - 1.0) / jbar[s.cell];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
-      const R dn = (c1 != R(0) ? c1 * d[base + i] : R(0)) + c2 * zs * zr[s.l * N1 + i];
+      const R dn = (c1 != R(0) ? c1 * dr[i] : R(0)) + c2 * zs * zr[s.l * N1 + i];
       d[base + i] = dn;
-      x[base + i] += dn;
+      x[base + i] = xr[i] + dn;
     }
   }
 }
