@@ -492,6 +492,33 @@ struct Nbrs {
   }
 };
 
+// Neighbour-side face nodes of node row l for every face of the cell: nbv[fc] is the field at the
+// neighbour's node that matches the row's own face node on face fc. All faces are loaded before
+// any is used (a wall reads the own cell and the value is discarded), so the loads are in flight
+// together instead of each waiting before its LDS store.
+template <int DIM, int K, typename R>
+__device__ __attribute__((always_inline)) inline void nbr_face_nodes(const R* __restrict__ f, const Nbrs<DIM>& nb,
+                                                                      int64_t cell, int l, R (&nbv)[2 * DIM]) {
+  constexpr int N1 = K + 1, NPC = DIM == 2 ? N1 * N1 : N1 * N1 * N1;
+  int64_t c[2 * DIM];
+#pragma unroll
+  for (int fc = 0; fc < 2 * DIM; ++fc) c[fc] = (nb.c[fc] >= 0 ? nb.c[fc] : cell) * NPC;
+  // x faces: the neighbour's opposite node of the same row
+  nbv[0] = f[c[0] + l * N1 + K];
+  nbv[1] = f[c[1] + l * N1];
+  if (DIM == 2) {
+    // z faces: the neighbour's row K (below) / 0 (above)
+    nbv[2] = f[c[2] + K * N1 + l];
+    nbv[3] = f[c[3] + l];
+  } else {
+    const int iy = l % N1, iz = l / N1;
+    nbv[2] = f[c[2] + (iz * N1 + K) * N1 + iy];
+    nbv[3] = f[c[3] + (iz * N1) * N1 + iy];
+    nbv[4] = f[c[4] + (K * N1 + iz) * N1 + iy];
+    nbv[5] = f[c[5] + iz * N1 + iy];
+  }
+}
+
 // LDS doubles per cell for one field's face nodes: [face][own|neighbour][face node]
 template <int DIM, int K>
 struct FaceLds {
@@ -517,6 +544,8 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
     R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+    R nbv[2 * DIM];
+    nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       R a = 0.0;
@@ -527,26 +556,26 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
     // x faces: node (0|K, row l); neighbour's opposite node of the same row
     FN(0, 0, s.l) = d[0];
     FN(1, 0, s.l) = d[K];
-    FN(0, 1, s.l) = nb.c[0] >= 0 ? f[nb.c[0] * NPC + s.l * N1 + K] : d[0];
-    FN(1, 1, s.l) = nb.c[1] >= 0 ? f[nb.c[1] * NPC + s.l * N1] : d[K];
+    FN(0, 1, s.l) = nb.c[0] >= 0 ? nbv[0] : d[0];
+    FN(1, 1, s.l) = nb.c[1] >= 0 ? nbv[1] : d[K];
     if (DIM == 2) {
       // z faces: own rows 0 / K; neighbour rows K (below) / 0 (above), one node per row owner
       if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
       if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, s.l) = f[nb.c[2] * NPC + K * N1 + s.l];
-      if (nb.c[3] >= 0) FN(3, 1, s.l) = f[nb.c[3] * NPC + s.l];
+      if (nb.c[2] >= 0) FN(2, 1, s.l) = nbv[2];
+      if (nb.c[3] >= 0) FN(3, 1, s.l) = nbv[3];
     } else {
       const int iy = s.l % N1, iz = s.l / N1;
       // y faces: own nodes (ix, 0|K, iz) -> t = ix + N1 iz; neighbour node (ix = iy, K|0, iz)
       if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
       if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = f[nb.c[2] * NPC + (iz * N1 + K) * N1 + iy];
-      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = f[nb.c[3] * NPC + (iz * N1) * N1 + iy];
+      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = nbv[2];
+      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = nbv[3];
       // z faces: own nodes (ix, iy, 0|K) -> t = ix + N1 iy; neighbour node (ix = iy, iy = iz, K|0)
       if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
       if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
-      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = f[nb.c[4] * NPC + (K * N1 + iz) * N1 + iy];
-      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
+      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = nbv[4];
+      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = nbv[5];
     }
   }
   cell_sync<DIM, K, Q>();
@@ -637,25 +666,27 @@ __device__ __attribute__((always_inline)) inline void trace_line_faces(const R* 
     R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+    R nbv[2 * DIM];
+    nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
     FN(0, 0, s.l) = d[0];
     FN(1, 0, s.l) = d[K];
-    FN(0, 1, s.l) = nb.c[0] >= 0 ? f[nb.c[0] * NPC + s.l * N1 + K] : d[0];
-    FN(1, 1, s.l) = nb.c[1] >= 0 ? f[nb.c[1] * NPC + s.l * N1] : d[K];
+    FN(0, 1, s.l) = nb.c[0] >= 0 ? nbv[0] : d[0];
+    FN(1, 1, s.l) = nb.c[1] >= 0 ? nbv[1] : d[K];
     if (DIM == 2) {
       if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
       if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, s.l) = f[nb.c[2] * NPC + K * N1 + s.l];
-      if (nb.c[3] >= 0) FN(3, 1, s.l) = f[nb.c[3] * NPC + s.l];
+      if (nb.c[2] >= 0) FN(2, 1, s.l) = nbv[2];
+      if (nb.c[3] >= 0) FN(3, 1, s.l) = nbv[3];
     } else {
       const int iy = s.l % N1, iz = s.l / N1;
       if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
       if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = f[nb.c[2] * NPC + (iz * N1 + K) * N1 + iy];
-      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = f[nb.c[3] * NPC + (iz * N1) * N1 + iy];
+      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = nbv[2];
+      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = nbv[3];
       if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
       if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
-      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = f[nb.c[4] * NPC + (K * N1 + iz) * N1 + iy];
-      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = f[nb.c[5] * NPC + iz * N1 + iy];
+      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = nbv[4];
+      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = nbv[5];
     }
   }
   cell_sync<DIM, K, Q>();
@@ -705,6 +736,9 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
     for (int a = 0; a < NF; ++a)
 #pragma unroll
       for (int i = 0; i < N1; ++i) d[a][i] = f[a][cell * NPC + s.l * N1 + i];
+    R nbv[NFF > 0 ? NFF : 1][2 * DIM];
+#pragma unroll
+    for (int a = 0; a < NFF; ++a) nbr_face_nodes<DIM, K, R>(f[a], nb, cell, s.l, nbv[a]);
 #pragma unroll
     for (int a = 0; a < NF; ++a)
 #pragma unroll
@@ -716,26 +750,25 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
       }
 #pragma unroll
     for (int a = 0; a < NFF; ++a) {
-      const R* u = f[a];
       FN(a, 0, 0, s.l) = d[a][0];
       FN(a, 1, 0, s.l) = d[a][K];
-      FN(a, 0, 1, s.l) = nb.c[0] >= 0 ? u[nb.c[0] * NPC + s.l * N1 + K] : d[a][0];
-      FN(a, 1, 1, s.l) = nb.c[1] >= 0 ? u[nb.c[1] * NPC + s.l * N1] : d[a][K];
+      FN(a, 0, 1, s.l) = nb.c[0] >= 0 ? nbv[a][0] : d[a][0];
+      FN(a, 1, 1, s.l) = nb.c[1] >= 0 ? nbv[a][1] : d[a][K];
       if (DIM == 2) {
         if (s.l == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i) = d[a][i];
         if (s.l == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i) = d[a][i];
-        if (nb.c[2] >= 0) FN(a, 2, 1, s.l) = u[nb.c[2] * NPC + K * N1 + s.l];
-        if (nb.c[3] >= 0) FN(a, 3, 1, s.l) = u[nb.c[3] * NPC + s.l];
+        if (nb.c[2] >= 0) FN(a, 2, 1, s.l) = nbv[a][2];
+        if (nb.c[3] >= 0) FN(a, 3, 1, s.l) = nbv[a][3];
       } else {
         const int iy = s.l % N1, iz = s.l / N1;
         if (iy == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i + N1 * iz) = d[a][i];
         if (iy == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i + N1 * iz) = d[a][i];
-        if (nb.c[2] >= 0) FN(a, 2, 1, iy + N1 * iz) = u[nb.c[2] * NPC + (iz * N1 + K) * N1 + iy];
-        if (nb.c[3] >= 0) FN(a, 3, 1, iy + N1 * iz) = u[nb.c[3] * NPC + (iz * N1) * N1 + iy];
+        if (nb.c[2] >= 0) FN(a, 2, 1, iy + N1 * iz) = nbv[a][2];
+        if (nb.c[3] >= 0) FN(a, 3, 1, iy + N1 * iz) = nbv[a][3];
         if (iz == 0) for (int i = 0; i < N1; ++i) FN(a, 4, 0, i + N1 * iy) = d[a][i];
         if (iz == K) for (int i = 0; i < N1; ++i) FN(a, 5, 0, i + N1 * iy) = d[a][i];
-        if (nb.c[4] >= 0) FN(a, 4, 1, iy + N1 * iz) = u[nb.c[4] * NPC + (K * N1 + iz) * N1 + iy];
-        if (nb.c[5] >= 0) FN(a, 5, 1, iy + N1 * iz) = u[nb.c[5] * NPC + iz * N1 + iy];
+        if (nb.c[4] >= 0) FN(a, 4, 1, iy + N1 * iz) = nbv[a][4];
+        if (nb.c[5] >= 0) FN(a, 5, 1, iy + N1 * iz) = nbv[a][5];
       }
     }
   }
