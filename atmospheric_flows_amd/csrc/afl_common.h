@@ -731,14 +731,15 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
   const int64_t cell = s.act ? s.cell : 0;
   auto FN = [&](int a, int face, int side, int t) -> R& { return fn[a * FNS + (face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
+    // the neighbours' face nodes first, then the rows: every load is issued before the first use
+    R nbv[NFF > 0 ? NFF : 1][2 * DIM];
+#pragma unroll
+    for (int a = 0; a < NFF; ++a) nbr_face_nodes<DIM, K, R>(f[a], nb, cell, s.l, nbv[a]);
     R d[NF][N1];
 #pragma unroll
     for (int a = 0; a < NF; ++a)
 #pragma unroll
       for (int i = 0; i < N1; ++i) d[a][i] = f[a][cell * NPC + s.l * N1 + i];
-    R nbv[NFF > 0 ? NFF : 1][2 * DIM];
-#pragma unroll
-    for (int a = 0; a < NFF; ++a) nbr_face_nodes<DIM, K, R>(f[a], nb, cell, s.l, nbv[a]);
 #pragma unroll
     for (int a = 0; a < NF; ++a)
 #pragma unroll

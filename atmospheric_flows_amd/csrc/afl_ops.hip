@@ -565,7 +565,8 @@ struct LdsC {
 template <int DIM, int K, bool WITH_D, typename R, int Q = 2 * K + 1>
 __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g, const TabT<R>& T, const Slot<DIM, K, Q>& s, R* sh, R* Fq, R cdt,
                                     const R* __restrict__ pfix, const R* __restrict__ w, int64_t N,
-                                    const R* __restrict__ pD, R scaleD, R (&out)[K + 1]) {
+                                    const R* __restrict__ pD, R scaleD, R (&out)[K + 1],
+                                    const R* rows_a = nullptr, const R* rows_b = nullptr, int sel = -1) {
   using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ;
   constexpr bool BATCH = sizeof(R) == 4 && AFL_C_BATCH_F32;
@@ -581,8 +582,15 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
   const FacePts<DIM, K, R, Q> fp(g, T, s);
   R pq[NQ], G[DIM][NQ], V[NQ];
   R trp[2 * DIM][2], trw[DIM][2 * DIM][2];
-  R prow[K + 1];  // pD's node row, loaded ahead of the evaluations' barriers
-  if (WITH_D) load_row<DIM, K, R>(pD, s, prow);
+  // pD's node row: loaded ahead of the evaluations' barriers, or (sel = 0 / 1) the caller's
+  // rows_a / rows_b, already loaded, picked where it is used
+  R prow[K + 1];
+  if (WITH_D && sel < 0) load_row<DIM, K, R>(pD, s, prow);
+  auto pick_prow = [&]() {
+    if (sel >= 0)
+#pragma unroll
+      for (int i = 0; i <= K; ++i) prow[i] = sel == 0 ? rows_a[i] : rows_b[i];
+  };
   if constexpr (BATCH) {
     const R* fl[LC::NF];
     fl[0] = pfix;
@@ -597,6 +605,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       for (int d = 0; d < DIM; ++d) G[d][q] = sgn * (-cdt * gr * v[0][q] * v[1 + d][q]);
     }
     if (WITH_D) {
+      pick_prow();
       eval_line_row<DIM, K, R>(prow, s, t1, t1 + C::EV1, T, V);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
@@ -621,6 +630,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
       for (int q = 0; q < NQ; ++q) G[d][q] = sgn * (-cdt * gr * pq[q] * wq[q]);
     }
     if (WITH_D) {
+      pick_prow();
       eval_line_row<DIM, K, R>(prow, s, t1, t2, T, V);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) V[q] *= scaleD * lg.jxw(g, T, q);
@@ -697,13 +707,15 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   const int64_t base = (s.act ? s.cell : 0) * NPC + s.l * N1;
   // the epilogue's inputs are loaded first, so their latency overlaps the C apply (d and x are
   // read and written only by their row's thread, after every read of pD == d or x)
+  const R jb = s.act ? jbar[s.cell] : R(1);
   R rr[N1], dr[N1], xr[N1];
   load_row<DIM, K, R>(rin, s, rr);
   load_row<DIM, K, R>(d, s, dr);
   load_row<DIM, K, R>(x, s, xr);
-  const R zs = s.act ? R(kGamma - 1.0) / jbar[s.cell] : R(0);
+  // pD is d or x in the Chebyshev recursion: its rows are then already loaded
+  const int sel = pD == d ? 0 : (pD == x ? 1 : -1);
   R t[N1];
-  apply_C_rows<DIM, K, true, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, t);
+  apply_C_rows<DIM, K, true, R>(g, T, s, sh, sh + LC::FOFF, cdt, pfix, w, N, pD, scaleD, t, dr, xr, sel);
   cell_sync<DIM, K, Q>();  // the face fluxes and work area are read until here; the epilogue reuses them
   R* tmp = sh;
   R* tmp2 = sh + NPC;
@@ -771,6 +783,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
     zr = tmp;
   }
   if (ro) {
+    const R zs = R(kGamma - 1.0) / jb;
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const R dn = (c1 != R(0) ? c1 * dr[i] : R(0)) + c2 * zs * zr[s.l * N1 + i];
