@@ -52,26 +52,49 @@ struct Geo {
 // compiler copy the whole table into per-lane scratch.
 template <int CNT, int N, typename R>
 __device__ __attribute__((always_inline)) inline R pick(const R (&a)[N], int i) {
-  R v = a[0];
+  if constexpr (sizeof(R) == 4) {
+    // FP32 tables read through a pointer: a blend of the uniform entries, since the compiler
+    // turns a select chain over memory into a per-lane indexed load (a memory round trip)
+    R v = R(0);
 #pragma unroll
-  for (int j = 1; j < CNT; ++j) v = (i == j) ? a[j] : v;
-  return v;
+    for (int j = 0; j < CNT; ++j) v += (i == j ? R(1) : R(0)) * a[j];
+    return v;
+  } else {
+    R v = a[0];
+#pragma unroll
+    for (int j = 1; j < CNT; ++j) v = (i == j) ? a[j] : v;
+    return v;
+  }
 }
 // S[r][col] for a runtime row r < ROWS and a compile-time column
 template <int ROWS, typename R>
 __device__ __attribute__((always_inline)) inline R pick_row(const R (&S)[9][5], int r, int col) {
-  R v = S[0][col];
+  if constexpr (sizeof(R) == 4) {  // blend, as in pick_col
+    R v = R(0);
 #pragma unroll
-  for (int j = 1; j < ROWS; ++j) v = (r == j) ? S[j][col] : v;
-  return v;
+    for (int j = 0; j < ROWS; ++j) v += (r == j ? R(1) : R(0)) * S[j][col];
+    return v;
+  } else {
+    R v = S[0][col];
+#pragma unroll
+    for (int j = 1; j < ROWS; ++j) v = (r == j) ? S[j][col] : v;
+    return v;
+  }
 }
 // S[row][c] for a compile-time row and a runtime column c < COLS
 template <int COLS, typename R>
 __device__ __attribute__((always_inline)) inline R pick_col(const R (&S)[9][5], int row, int c) {
-  R v = S[row][0];
+  if constexpr (sizeof(R) == 4) {  // blend, as in pick
+    R v = R(0);
 #pragma unroll
-  for (int j = 1; j < COLS; ++j) v = (c == j) ? S[row][j] : v;
-  return v;
+    for (int j = 0; j < COLS; ++j) v += (c == j ? R(1) : R(0)) * S[row][j];
+    return v;
+  } else {
+    R v = S[row][0];
+#pragma unroll
+    for (int j = 1; j < COLS; ++j) v = (c == j) ? S[row][j] : v;
+    return v;
+  }
 }
 
 template <int DIM, int K, int Q = 2 * K + 1>
