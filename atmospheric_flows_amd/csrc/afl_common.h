@@ -47,14 +47,19 @@ struct Geo {
   double H, h0, inv_a2;
 };
 
-// Runtime-indexed reads of the basis tables. The tables travel in the kernel argument (uniform
-// values); a select chain keeps them there, where an indexed memory access would make the
-// compiler copy the whole table into per-lane scratch.
+// Runtime-indexed reads of the basis tables (uniform values in device memory, read through the
+// kernel's table pointer). A select chain over them is turned by the compiler into one per-lane
+// indexed load, a memory round trip on the critical path; a blend of the uniform entries costs
+// CNT multiply-adds instead. The FP32 multigrid level kernels blend; the FP64 kernels keep the
+// select chain, since there the blend's extra FP64 work and registers cost more than it saves
+// (AFL_PICK_BLEND_F64=1 at 512x100, k = 4: E1 stage 3 102 -> 97 us, E2 140 -> 162 us,
+// E3 171 -> 197 us).
+#ifndef AFL_PICK_BLEND_F64
+#define AFL_PICK_BLEND_F64 0
+#endif
 template <int CNT, int N, typename R>
 __device__ __attribute__((always_inline)) inline R pick(const R (&a)[N], int i) {
-  if constexpr (sizeof(R) == 4) {
-    // FP32 tables read through a pointer: a blend of the uniform entries, since the compiler
-    // turns a select chain over memory into a per-lane indexed load (a memory round trip)
+  if constexpr (sizeof(R) == 4 || AFL_PICK_BLEND_F64) {
     R v = R(0);
 #pragma unroll
     for (int j = 0; j < CNT; ++j) v += (i == j ? R(1) : R(0)) * a[j];
@@ -69,7 +74,7 @@ __device__ __attribute__((always_inline)) inline R pick(const R (&a)[N], int i) 
 // S[r][col] for a runtime row r < ROWS and a compile-time column
 template <int ROWS, typename R>
 __device__ __attribute__((always_inline)) inline R pick_row(const R (&S)[9][5], int r, int col) {
-  if constexpr (sizeof(R) == 4) {  // blend, as in pick_col
+  if constexpr (sizeof(R) == 4 || AFL_PICK_BLEND_F64) {
     R v = R(0);
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) v += (r == j ? R(1) : R(0)) * S[j][col];
@@ -84,7 +89,7 @@ __device__ __attribute__((always_inline)) inline R pick_row(const R (&S)[9][5], 
 // S[row][c] for a compile-time row and a runtime column c < COLS
 template <int COLS, typename R>
 __device__ __attribute__((always_inline)) inline R pick_col(const R (&S)[9][5], int row, int c) {
-  if constexpr (sizeof(R) == 4) {  // blend, as in pick
+  if constexpr (sizeof(R) == 4 || AFL_PICK_BLEND_F64) {
     R v = R(0);
 #pragma unroll
     for (int j = 0; j < COLS; ++j) v += (c == j ? R(1) : R(0)) * S[row][j];
