@@ -522,8 +522,9 @@ struct Nbrs {
 
 // Neighbour-side face nodes of node row l for every face of the cell: nbv[fc] is the field at the
 // neighbour's node that matches the row's own face node on face fc. All faces are loaded before
-// any is used (a wall reads the own cell and the value is discarded), so the loads are in flight
-// together instead of each waiting before its LDS store.
+// any is used, and stored unconditionally (a wall reads the own cell; its neighbour-side trace is
+// replaced by the own trace after the interpolation), so the loads are in flight together
+// instead of each waiting, inside its own branch, before its LDS store.
 template <int DIM, int K, typename R>
 __device__ __attribute__((always_inline)) inline void nbr_face_nodes(const R* __restrict__ f, const Nbrs<DIM>& nb,
                                                                       int64_t cell, int l, R (&nbv)[2 * DIM]) {
@@ -569,11 +570,11 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
   const R* u = f + (s.act ? s.cell : 0) * NPC;
   auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
+    R nbv[2 * DIM];  // the neighbours' face nodes first, then the row
+    nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
     R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
-    R nbv[2 * DIM];
-    nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       R a = 0.0;
@@ -590,20 +591,20 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
       // z faces: own rows 0 / K; neighbour rows K (below) / 0 (above), one node per row owner
       if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
       if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, s.l) = nbv[2];
-      if (nb.c[3] >= 0) FN(3, 1, s.l) = nbv[3];
+      FN(2, 1, s.l) = nbv[2];
+      FN(3, 1, s.l) = nbv[3];
     } else {
       const int iy = s.l % N1, iz = s.l / N1;
       // y faces: own nodes (ix, 0|K, iz) -> t = ix + N1 iz; neighbour node (ix = iy, K|0, iz)
       if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
       if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = nbv[2];
-      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = nbv[3];
+      FN(2, 1, iy + N1 * iz) = nbv[2];
+      FN(3, 1, iy + N1 * iz) = nbv[3];
       // z faces: own nodes (ix, iy, 0|K) -> t = ix + N1 iy; neighbour node (ix = iy, iy = iz, K|0)
       if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
       if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
-      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = nbv[4];
-      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = nbv[5];
+      FN(4, 1, iy + N1 * iz) = nbv[4];
+      FN(5, 1, iy + N1 * iz) = nbv[5];
     }
   }
   cell_sync<DIM, K, Q>();
@@ -691,11 +692,11 @@ __device__ __attribute__((always_inline)) inline void trace_line_faces(const R* 
   const R* u = f + (s.act ? s.cell : 0) * NPC;
   auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
+    R nbv[2 * DIM];  // the neighbours' face nodes first, then the row
+    nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
     R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
-    R nbv[2 * DIM];
-    nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
     FN(0, 0, s.l) = d[0];
     FN(1, 0, s.l) = d[K];
     FN(0, 1, s.l) = nb.c[0] >= 0 ? nbv[0] : d[0];
@@ -703,18 +704,18 @@ __device__ __attribute__((always_inline)) inline void trace_line_faces(const R* 
     if (DIM == 2) {
       if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
       if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, s.l) = nbv[2];
-      if (nb.c[3] >= 0) FN(3, 1, s.l) = nbv[3];
+      FN(2, 1, s.l) = nbv[2];
+      FN(3, 1, s.l) = nbv[3];
     } else {
       const int iy = s.l % N1, iz = s.l / N1;
       if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
       if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
-      if (nb.c[2] >= 0) FN(2, 1, iy + N1 * iz) = nbv[2];
-      if (nb.c[3] >= 0) FN(3, 1, iy + N1 * iz) = nbv[3];
+      FN(2, 1, iy + N1 * iz) = nbv[2];
+      FN(3, 1, iy + N1 * iz) = nbv[3];
       if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
       if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
-      if (nb.c[4] >= 0) FN(4, 1, iy + N1 * iz) = nbv[4];
-      if (nb.c[5] >= 0) FN(5, 1, iy + N1 * iz) = nbv[5];
+      FN(4, 1, iy + N1 * iz) = nbv[4];
+      FN(5, 1, iy + N1 * iz) = nbv[5];
     }
   }
   cell_sync<DIM, K, Q>();
@@ -786,18 +787,18 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
       if (DIM == 2) {
         if (s.l == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i) = d[a][i];
         if (s.l == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i) = d[a][i];
-        if (nb.c[2] >= 0) FN(a, 2, 1, s.l) = nbv[a][2];
-        if (nb.c[3] >= 0) FN(a, 3, 1, s.l) = nbv[a][3];
+        FN(a, 2, 1, s.l) = nbv[a][2];
+        FN(a, 3, 1, s.l) = nbv[a][3];
       } else {
         const int iy = s.l % N1, iz = s.l / N1;
         if (iy == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i + N1 * iz) = d[a][i];
         if (iy == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i + N1 * iz) = d[a][i];
-        if (nb.c[2] >= 0) FN(a, 2, 1, iy + N1 * iz) = nbv[a][2];
-        if (nb.c[3] >= 0) FN(a, 3, 1, iy + N1 * iz) = nbv[a][3];
+        FN(a, 2, 1, iy + N1 * iz) = nbv[a][2];
+        FN(a, 3, 1, iy + N1 * iz) = nbv[a][3];
         if (iz == 0) for (int i = 0; i < N1; ++i) FN(a, 4, 0, i + N1 * iy) = d[a][i];
         if (iz == K) for (int i = 0; i < N1; ++i) FN(a, 5, 0, i + N1 * iy) = d[a][i];
-        if (nb.c[4] >= 0) FN(a, 4, 1, iy + N1 * iz) = nbv[a][4];
-        if (nb.c[5] >= 0) FN(a, 5, 1, iy + N1 * iz) = nbv[a][5];
+        FN(a, 4, 1, iy + N1 * iz) = nbv[a][4];
+        FN(a, 5, 1, iy + N1 * iz) = nbv[a][5];
       }
     }
   }

@@ -1277,6 +1277,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   Slot<DIM, K, Q> s(g);
   const int cbs = s.cb < C::CPB ? s.cb : 0;
   R* base = shw + cbs * PC;
+  const R sc = s.act ? sinv[s.cell] : R(0);  // loaded first: not a round trip at the end
   LineGeo<DIM, K, R, Q> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
@@ -1286,7 +1287,6 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
                           base + Lds<DIM, K, Q>::W, base + Lds<DIM, K, Q>::WF, bp);
   // the mass-inverse transposes use [0, PM), disjoint from the face fluxes add_faces may still read
   cell_mi_apply<DIM, K, DIM, R>(s, T, base, bp);
-  const R sc = s.act ? sinv[s.cell] : R(0);
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
 #pragma unroll
