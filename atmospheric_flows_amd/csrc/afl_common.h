@@ -570,11 +570,25 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
   const R* u = f + (s.act ? s.cell : 0) * NPC;
   auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
-    R nbv[2 * DIM];  // the neighbours' face nodes first, then the row
+    R nbv[2 * DIM];
     nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
     R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
+    // the neighbour sides are stored first, right after their loads: every load of the staging
+    // is in flight before the first wait
+    FN(0, 1, s.l) = nb.c[0] >= 0 ? nbv[0] : d[0];
+    FN(1, 1, s.l) = nb.c[1] >= 0 ? nbv[1] : d[K];
+    if (DIM == 2) {
+      FN(2, 1, s.l) = nbv[2];
+      FN(3, 1, s.l) = nbv[3];
+    } else {
+      const int iy = s.l % N1, iz = s.l / N1;
+      FN(2, 1, iy + N1 * iz) = nbv[2];
+      FN(3, 1, iy + N1 * iz) = nbv[3];
+      FN(4, 1, iy + N1 * iz) = nbv[4];
+      FN(5, 1, iy + N1 * iz) = nbv[5];
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       R a = 0.0;
@@ -585,26 +599,18 @@ __device__ __attribute__((always_inline)) inline void eval_line_faces(const R* _
     // x faces: node (0|K, row l); neighbour's opposite node of the same row
     FN(0, 0, s.l) = d[0];
     FN(1, 0, s.l) = d[K];
-    FN(0, 1, s.l) = nb.c[0] >= 0 ? nbv[0] : d[0];
-    FN(1, 1, s.l) = nb.c[1] >= 0 ? nbv[1] : d[K];
     if (DIM == 2) {
       // z faces: own rows 0 / K; neighbour rows K (below) / 0 (above), one node per row owner
       if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
       if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
-      FN(2, 1, s.l) = nbv[2];
-      FN(3, 1, s.l) = nbv[3];
     } else {
       const int iy = s.l % N1, iz = s.l / N1;
       // y faces: own nodes (ix, 0|K, iz) -> t = ix + N1 iz; neighbour node (ix = iy, K|0, iz)
       if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
       if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
-      FN(2, 1, iy + N1 * iz) = nbv[2];
-      FN(3, 1, iy + N1 * iz) = nbv[3];
       // z faces: own nodes (ix, iy, 0|K) -> t = ix + N1 iy; neighbour node (ix = iy, iy = iz, K|0)
       if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
       if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
-      FN(4, 1, iy + N1 * iz) = nbv[4];
-      FN(5, 1, iy + N1 * iz) = nbv[5];
     }
   }
   cell_sync<DIM, K, Q>();
@@ -692,30 +698,36 @@ __device__ __attribute__((always_inline)) inline void trace_line_faces(const R* 
   const R* u = f + (s.act ? s.cell : 0) * NPC;
   auto FN = [&](int face, int side, int t) -> R& { return fn[(face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
-    R nbv[2 * DIM];  // the neighbours' face nodes first, then the row
+    R nbv[2 * DIM];
     nbr_face_nodes<DIM, K, R>(f, nb, s.cell, s.l, nbv);
     R d[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) d[i] = u[s.l * N1 + i];
-    FN(0, 0, s.l) = d[0];
-    FN(1, 0, s.l) = d[K];
+    // the neighbour sides are stored first, right after their loads: every load of the staging
+    // is in flight before the first wait
     FN(0, 1, s.l) = nb.c[0] >= 0 ? nbv[0] : d[0];
     FN(1, 1, s.l) = nb.c[1] >= 0 ? nbv[1] : d[K];
     if (DIM == 2) {
-      if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
-      if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
       FN(2, 1, s.l) = nbv[2];
       FN(3, 1, s.l) = nbv[3];
     } else {
       const int iy = s.l % N1, iz = s.l / N1;
-      if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
-      if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
       FN(2, 1, iy + N1 * iz) = nbv[2];
       FN(3, 1, iy + N1 * iz) = nbv[3];
-      if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
-      if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
       FN(4, 1, iy + N1 * iz) = nbv[4];
       FN(5, 1, iy + N1 * iz) = nbv[5];
+    }
+    FN(0, 0, s.l) = d[0];
+    FN(1, 0, s.l) = d[K];
+    if (DIM == 2) {
+      if (s.l == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i) = d[i];
+      if (s.l == K) for (int i = 0; i < N1; ++i) FN(3, 0, i) = d[i];
+    } else {
+      const int iy = s.l % N1, iz = s.l / N1;
+      if (iy == 0) for (int i = 0; i < N1; ++i) FN(2, 0, i + N1 * iz) = d[i];
+      if (iy == K) for (int i = 0; i < N1; ++i) FN(3, 0, i + N1 * iz) = d[i];
+      if (iz == 0) for (int i = 0; i < N1; ++i) FN(4, 0, i + N1 * iy) = d[i];
+      if (iz == K) for (int i = 0; i < N1; ++i) FN(5, 0, i + N1 * iy) = d[i];
     }
   }
   cell_sync<DIM, K, Q>();
@@ -760,7 +772,6 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
   const int64_t cell = s.act ? s.cell : 0;
   auto FN = [&](int a, int face, int side, int t) -> R& { return fn[a * FNS + (face * 2 + side) * NFN + t]; };
   if (s.act && s.l < C::ROWS) {
-    // the neighbours' face nodes first, then the rows: every load is issued before the first use
     R nbv[NFF > 0 ? NFF : 1][2 * DIM];
 #pragma unroll
     for (int a = 0; a < NFF; ++a) nbr_face_nodes<DIM, K, R>(f[a], nb, cell, s.l, nbv[a]);
@@ -769,6 +780,23 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
     for (int a = 0; a < NF; ++a)
 #pragma unroll
       for (int i = 0; i < N1; ++i) d[a][i] = f[a][cell * NPC + s.l * N1 + i];
+    // the neighbour sides are stored first, right after their loads: every load of the staging
+    // is in flight before the first wait
+#pragma unroll
+    for (int a = 0; a < NFF; ++a) {
+      FN(a, 0, 1, s.l) = nb.c[0] >= 0 ? nbv[a][0] : d[a][0];
+      FN(a, 1, 1, s.l) = nb.c[1] >= 0 ? nbv[a][1] : d[a][K];
+      if (DIM == 2) {
+        FN(a, 2, 1, s.l) = nbv[a][2];
+        FN(a, 3, 1, s.l) = nbv[a][3];
+      } else {
+        const int iy = s.l % N1, iz = s.l / N1;
+        FN(a, 2, 1, iy + N1 * iz) = nbv[a][2];
+        FN(a, 3, 1, iy + N1 * iz) = nbv[a][3];
+        FN(a, 4, 1, iy + N1 * iz) = nbv[a][4];
+        FN(a, 5, 1, iy + N1 * iz) = nbv[a][5];
+      }
+    }
 #pragma unroll
     for (int a = 0; a < NF; ++a)
 #pragma unroll
@@ -782,23 +810,15 @@ __device__ __attribute__((always_inline)) inline void eval_lines_faces(const R* 
     for (int a = 0; a < NFF; ++a) {
       FN(a, 0, 0, s.l) = d[a][0];
       FN(a, 1, 0, s.l) = d[a][K];
-      FN(a, 0, 1, s.l) = nb.c[0] >= 0 ? nbv[a][0] : d[a][0];
-      FN(a, 1, 1, s.l) = nb.c[1] >= 0 ? nbv[a][1] : d[a][K];
       if (DIM == 2) {
         if (s.l == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i) = d[a][i];
         if (s.l == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i) = d[a][i];
-        FN(a, 2, 1, s.l) = nbv[a][2];
-        FN(a, 3, 1, s.l) = nbv[a][3];
       } else {
         const int iy = s.l % N1, iz = s.l / N1;
         if (iy == 0) for (int i = 0; i < N1; ++i) FN(a, 2, 0, i + N1 * iz) = d[a][i];
         if (iy == K) for (int i = 0; i < N1; ++i) FN(a, 3, 0, i + N1 * iz) = d[a][i];
-        FN(a, 2, 1, iy + N1 * iz) = nbv[a][2];
-        FN(a, 3, 1, iy + N1 * iz) = nbv[a][3];
         if (iz == 0) for (int i = 0; i < N1; ++i) FN(a, 4, 0, i + N1 * iy) = d[a][i];
         if (iz == K) for (int i = 0; i < N1; ++i) FN(a, 5, 0, i + N1 * iy) = d[a][i];
-        FN(a, 4, 1, iy + N1 * iz) = nbv[a][4];
-        FN(a, 5, 1, iy + N1 * iz) = nbv[a][5];
       }
     }
   }
