@@ -193,6 +193,7 @@ struct afl_operator {
   bool dinv_ready = false;
   // IMEX work vectors
   DevBuf rho2, rho3, rhoc, u2, u3, uc, p2, p3, pc, rhs_r, rhs_u, rhs_p, tmp1, tmp2, Bp, pfix, pfix_old, ufix;
+  DevBuf rhs_u0, rhs_p0;  // fixed-point loop invariants: E2 and the u_fixed-free part of E3
   // adjoint work vectors (Schur tangent and its transpose; step adjoint)
   DevBuf adjW0, adjT1, adjT2, adjC;
   static constexpr int kAdjBufs = 34;
@@ -215,7 +216,7 @@ struct afl_operator {
   ~afl_operator() {
     DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
-                     &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix, &adjW0, &adjT1, &adjT2, &adjC};
+                     &tmp1, &tmp2, &Bp, &pfix, &pfix_old, &ufix, &rhs_u0, &rhs_p0, &adjW0, &adjT1, &adjT2, &adjC};
     for (DevBuf* b : all) b->release();
     for (DevBuf& b : aj) b.release();
     for (DevBuf& b : adj_lam_prev) b.release();
@@ -1226,10 +1227,10 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
   hipStream_t st = op->o.stream;
   int rc;
   DevBuf* sc[] = {&op->rho2, &op->rho3, &op->rhoc, &op->p2, &op->p3, &op->pc, &op->rhs_r,
-                  &op->rhs_p, &op->tmp2, &op->pfix, &op->pfix_old};
+                  &op->rhs_p, &op->tmp2, &op->pfix, &op->pfix_old, &op->rhs_p0};
   for (DevBuf* b : sc)
     if ((rc = b->alloc(N))) return rc;
-  DevBuf* vc[] = {&op->u2, &op->u3, &op->uc, &op->rhs_u, &op->tmp1, &op->Bp, &op->ufix};
+  DevBuf* vc[] = {&op->u2, &op->u3, &op->uc, &op->rhs_u, &op->tmp1, &op->Bp, &op->ufix, &op->rhs_u0};
   for (DevBuf* b : vc)
     if ((rc = b->alloc(NU))) return rc;
   if ((rc = ensure_common(op))) return rc;
@@ -1240,24 +1241,29 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
                          double* p_out, double* u_out) -> int {
     vec_copy(st, N, p_start, op->pfix_old.p);
     vec_copy(st, NU, u_start, op->ufix.p);
+    // Loop invariants of pressure_fixed_point (Atmospheric_Flow.cc:545-673), which recomputes them on
+    // every iteration from the same inputs: the momentum right-hand side E2 (no source changes inside
+    // the loop), tmp1 = A(rho_new)^{-1} E2, and the part of E3 that does not read u_fixed. Each
+    // iteration adds E3's two u_fixed terms to that part (launch_rhs_p_ufix).
+    if (stage == 1) {
+      const double* sp_[5] = {rho, u, p, rho_new, op->ufix.p};
+      launch_rhs_p(o, 1, sp_, op->rhs_p0.p, false);
+      const double* su_[4] = {rho, u, p, rho_new};
+      launch_rhs_u(o, 1, su_, op->rhs_u0.p);
+    } else {
+      const double* sp_[8] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, rho_new, op->ufix.p};
+      launch_rhs_p(o, 2, sp_, op->rhs_p0.p, false);
+      const double* su_[7] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, rho_new};
+      launch_rhs_u(o, 2, su_, op->rhs_u0.p);
+    }
+    launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u0.p, op->tmp1.p);
     int it = 0, conv = 0, gm_total = 0;
     for (; it < op->sp.fp_max_iter; ++it) {
       // pressure_fixed_point, Atmospheric_Flow.cc:545-673
       vec_copy(st, N, op->pfix_old.p, op->p_fix.p);
       op->mg_coef_stale = true;
       vec_copy(st, NU, op->ufix.p, op->u_fix.p);
-      if (stage == 1) {
-        const double* sp_[5] = {rho, u, p, rho_new, op->ufix.p};
-        launch_rhs_p(o, 1, sp_, op->rhs_p.p);
-        const double* su_[4] = {rho, u, p, rho_new};
-        launch_rhs_u(o, 1, su_, op->rhs_u.p);
-      } else {
-        const double* sp_[8] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, rho_new, op->ufix.p};
-        launch_rhs_p(o, 2, sp_, op->rhs_p.p);
-        const double* su_[7] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, rho_new};
-        launch_rhs_u(o, 2, su_, op->rhs_u.p);
-      }
-      launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u.p, op->tmp1.p);
+      launch_rhs_p_ufix(o, stage, rho_new, op->ufix.p, op->rhs_p0.p, op->rhs_p.p);
       launch_apply_C(o, op->p_fix.p, op->tmp1.p, nullptr, op->tmp2.p);
       vec_axpby(st, N, -1.0, op->tmp2.p, 1.0, op->rhs_p.p);
       vec_copy(st, N, op->pfix_old.p, op->pfix.p);
@@ -1269,8 +1275,8 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
       if (r) return r;
       // update_velocity, Atmospheric_Flow.cc:681-781
       launch_apply_B(o, op->pfix.p, op->Bp.p);
-      vec_axpby(st, NU, -1.0, op->Bp.p, 1.0, op->rhs_u.p);
-      launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u.p, op->ufix.p);
+      vec_axpby(st, NU, 1.0, op->rhs_u0.p, -1.0, op->Bp.p);  // E2 - B p
+      launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->Bp.p, op->ufix.p);
       // relative inf-norm change of the nodal pressure, Atmospheric_Flow.cc:1140-1150
       vec_linf(st, N, op->pfix.p, op->red.p, op->dsc.p + kScA);
       vec_linf_diff(st, N, op->pfix.p, op->pfix_old.p, op->red.p, op->dsc.p + kScB);

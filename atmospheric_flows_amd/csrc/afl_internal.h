@@ -51,7 +51,10 @@ struct OpCtx {
 // operator kernels (afl_ops.hip)
 void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double* dst);
 void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* dst);
-void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* dst);
+// full = false at stages 1-2: E3 without its two u_fixed terms (launch_rhs_p_ufix adds them)
+void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* dst, bool full = true);
+void launch_rhs_p_ufix(const OpCtx& o, int stage, const double* rho_new, const double* u_fix, const double* base,
+                       double* dst);
 void launch_mass_apply(const OpCtx& o, const double* rho, double scale, int ncomp, const double* src, double* dst);
 void launch_inv_diag(const OpCtx& o, const double* rho, double scale, int ncomp, double* dst);
 void launch_apply_B(const OpCtx& o, const double* p, double* dst);

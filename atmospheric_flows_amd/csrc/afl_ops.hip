@@ -266,9 +266,12 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 // E3: vmult_rhs_pressure (euler_operator.h:743-1007 cell, 1010-1361 faces, boundary none).
 // Explicit states j < NE: (rho,u,p) = src[3j..]; kinetic weight w[j], enthalpy weight wt[j].
 // Kinetic correction with (src[3NE], src[3NE+1]); gravity work with (src[3j], src[3j+1]), j < NG.
+// FULL = false (implicit stages only) leaves out the two terms that read u_fixed, the kinetic
+// correction -1/2 Ma^2 rho_new |u_fix|^2 and the gravity work of (rho_new, u_fix): the rest does not
+// change across the fixed-point iterations of a stage, and k_rhs_p_ufix adds those two terms.
 struct E3Coef { double w[3], wt[3], gw[3]; };
 
-template <int DIM, int K, int NE, int NG, int STAGE>
+template <int DIM, int K, int NE, int NG, int STAGE, bool FULL = true>
 __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 2 : 1))) void k_rhs_p(Geo g, const Tab* __restrict__ Tp, Srcs src, E3Coef cf, double dt, double Ma2,
                                                    double iFr2, int64_t N, double* __restrict__ dst) {
   const auto& T = *Tp;
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
       for (int d = 0; d < DIM; ++d) G[d][q] += (kin + ent) * u[d][q];
     }
   }
-  {
+  if (FULL || STAGE == 3) {
     double r[NQ], u[NQ], uu[NQ];
     eval_line<DIM, K>(src.p[3 * NE], s, t1, t2, T, r);  // kinetic correction (rho, u) = src[3 NE], src[3 NE + 1]
 #pragma unroll
@@ -374,6 +377,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   }
 #pragma unroll
   for (int j = 0; j < NG; ++j) {
+    if (!FULL && STAGE < 3 && j == NG - 1) continue;  // (rho_new, u_fix): k_rhs_p_ufix
     double r[NQ], w[NQ];
     eval_line<DIM, K>(src.p[3 * j], s, t1, t2, T, r);  // gravity work: (rho, u) = src[3j], src[3j+1]
     eval_line<DIM, K>(src.p[3 * j + 1] + (DIM - 1) * N, s, t1, t2, T, w);
@@ -401,6 +405,40 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   integrate_line<DIM, K, true, true>(s, i1, i2, T, V, G, out);
   add_faces<DIM, K>(s, Fq, T, out, 0xffu);
   store_row<DIM, K>(dst, s, out);
+}
+
+// The u_fixed terms of E3 at an implicit stage (euler_operator.h:743-1007, the kinetic-energy
+// correction and the gravity work of the fixed-point velocity): dst = base + int psi V with
+// V = -1/2 Ma^2 rho_new |u_fix|^2 - cg rho_new u_fix,vertical, cg = (gravity weight) dt Ma^2 / Fr^2.
+// Cell-local (no faces); base = the u_fixed-free part of E3 from k_rhs_p<..., FULL = false>.
+template <int DIM, int K>
+__global__ __launch_bounds__(AFL_NT) void k_rhs_p_ufix(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho,
+                                                     const double* __restrict__ u, double half_ma2, double cg, int64_t N,
+                                                     const double* __restrict__ base, double* __restrict__ dst) {
+  const auto& T = *Tp;
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  AFL_SLOT_PTRS
+  LineGeo<DIM, K> lg;
+  lg.init(g, T, s.ic, s.l);
+  double r[NQ], V[1][NQ];
+  eval_line<DIM, K>(rho, s, t1, t2, T, r);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) V[0][q] = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double w[NQ];
+    eval_line<DIM, K>(u + d * N, s, t1, t2, T, w);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) V[0][q] += w[q] * w[q] * half_ma2 + (d == DIM - 1 ? cg * w[q] : 0.0);
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) V[0][q] = -r[q] * V[0][q] * lg.jxw(g, T, q);
+  double out[1][N1], b[N1];
+  load_row<DIM, K>(base, s, b);
+  integrate_lines_v<DIM, K, 1>(s, i1, i2, T, V, out);
+#pragma unroll
+  for (int i = 0; i < N1; ++i) out[0][i] += b[i];
+  store_row<DIM, K>(dst, s, out[0]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1401,7 +1439,7 @@ void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* d
   });
 }
 
-void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* dst) {
+void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* dst, bool full) {
   const TableauC& t = o.tab;
   const Srcs S = make_srcs(src, stage == 1 ? 5 : (stage == 2 ? 8 : 11));
   E3Coef cf{};
@@ -1420,12 +1458,27 @@ void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* d
   }
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells);
-    if (stage == 1)
+    if (stage == 1 && full)
       k_rhs_p<D_, K_, 1, 2, 1><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
-    else if (stage == 2)
+    else if (stage == 1)
+      k_rhs_p<D_, K_, 1, 2, 1, false><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+    else if (stage == 2 && full)
       k_rhs_p<D_, K_, 2, 3, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+    else if (stage == 2)
+      k_rhs_p<D_, K_, 2, 3, 2, false><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else
       k_rhs_p<D_, K_, 3, 3, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+  });
+}
+
+void launch_rhs_p_ufix(const OpCtx& o, int stage, const double* rho_new, const double* u_fix, const double* base,
+                       double* dst) {
+  const TableauC& t = o.tab;
+  const double Ma2 = o.Ma * o.Ma, iFr2 = 1.0 / (o.Fr * o.Fr);
+  const double cg = (stage == 1 ? t.at22 : t.at33) * o.dt * Ma2 * iFr2;
+  AFL_DISPATCH(o.dim, o.k, {
+    k_rhs_p_ufix<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(
+        o.geo, o.dT, rho_new, u_fix, 0.5 * Ma2, cg, o.N, base, dst);
   });
 }
 
