@@ -151,9 +151,20 @@ TableauC make_tableau() {
   return t;
 }
 
+// Device buffer owned by the handle (or by a scope): released by its destructor, so an early
+// return after a failed allocation leaks nothing. Move-only.
 struct DevBuf {
   double* p = nullptr;
   int64_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
   int alloc(int64_t count) {
     if (p && n >= count) return AFL_OK;
     if (p) hipFree(p);
@@ -213,6 +224,7 @@ struct afl_operator {
   double* host_sc = nullptr;  // pinned scalars
   Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
   KernelTimer timer;          // per-kernel HIP-event timing (afl_kernel_timer_*)
+  std::vector<double> timer_ms;  // durations of the timed launches of the last timing window
   ~afl_operator() {
     DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
                      &rho2, &rho3, &rhoc, &u2, &u3, &uc, &p2, &p3, &pc, &rhs_r, &rhs_u, &rhs_p,
@@ -645,6 +657,9 @@ int afl_destroy(afl_operator* op) {
 int afl_set_stream(afl_operator* op, void* s) {
   if (!valid_op(op)) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
   op->o.stream = (hipStream_t)s;
+  // the multigrid levels launch on their own copy of the stream: every level follows
+  for (auto& L : op->mg32.levels) L.stream = op->o.stream;
+  for (auto& L : op->mg64.levels) L.stream = op->o.stream;
   return AFL_OK;
 }
 
@@ -661,18 +676,23 @@ int afl_m(const afl_operator* op, int64_t* m) {
   return AFL_OK;
 }
 
+// dt, Ma and Fr scale the Schur operator, so the multigrid's Chebyshev interval (estimated for the
+// old values) is re-estimated before the next preconditioner apply
 int afl_set_dt(afl_operator* op, double v) {
   if (!op || !(v > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "dt must be > 0");
+  if (v != op->o.dt) op->mg_eig_stale = true;
   op->o.dt = v;
   return AFL_OK;
 }
 int afl_set_mach(afl_operator* op, double v) {
   if (!op || !(v > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "Ma must be > 0");
+  if (v != op->o.Ma) op->mg_eig_stale = true;
   op->o.Ma = v;
   return AFL_OK;
 }
 int afl_set_froude(afl_operator* op, double v) {
   if (!op || !(v > 0.0)) return fail(AFL_ERR_INVALID_ARGUMENT, "Fr must be > 0");
+  if (v != op->o.Fr) op->mg_eig_stale = true;
   op->o.Fr = v;
   return AFL_OK;
 }
@@ -950,10 +970,12 @@ int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {
   return check_device("vmult_schur_velocity_f32");
 }
 
-int afl_kernel_timer_start(afl_operator* op, int kernel, int max_launches) {
+int afl_kernel_timer_start(afl_operator* op, int kernels, int max_launches) {
   if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
-  if (kernel != AFL_TIME_LEVEL_W && kernel != AFL_TIME_LEVEL_C_CHEB)
-    return fail(AFL_ERR_INVALID_ARGUMENT, "unknown timed kernel");
+  constexpr int kAll = AFL_TIME_LEVEL_W | AFL_TIME_LEVEL_C_CHEB | AFL_TIME_RHS_RHO | AFL_TIME_RHS_U | AFL_TIME_RHS_P |
+                       AFL_TIME_RHS_P_UFIX | AFL_TIME_SCHUR_W | AFL_TIME_APPLY_C | AFL_TIME_MASS_SOLVE |
+                       AFL_TIME_APPLY_B;
+  if (kernels <= 0 || (kernels & ~kAll)) return fail(AFL_ERR_INVALID_ARGUMENT, "unknown timed kernel");
   if (max_launches < 1 || max_launches > (1 << 20)) return fail(AFL_ERR_INVALID_ARGUMENT, "max_launches out of range");
   KernelTimer& t = op->timer;
   while ((int)t.ev.size() < 2 * max_launches) {
@@ -961,29 +983,54 @@ int afl_kernel_timer_start(afl_operator* op, int kernel, int max_launches) {
     if (hipEventCreate(&e) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipEventCreate failed");
     t.ev.push_back(e);
   }
-  t.which = kernel;
+  t.flag.assign(t.ev.size() / 2, 0);
+  t.stage.assign(t.ev.size() / 2, 0);
+  t.bytes.assign(t.ev.size() / 2, 0.0);
+  t.mask = (unsigned)kernels;
   t.used = 0;
   t.dropped = 0;
+  op->o.timer = &t;
   op->mg32.timer = &t;
   op->mg64.timer = &t;
   return AFL_OK;
 }
-
 int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms) {
   if (!op || !launches || !total_ms) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   KernelTimer& t = op->timer;
-  t.which = 0;
+  t.mask = 0;
+  op->o.timer = nullptr;
   if (hipStreamSynchronize(op->o.stream) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
   double sum = 0.0;
+  op->timer_ms.assign(t.used, 0.0);
   for (int i = 0; i < t.used; ++i) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]) != hipSuccess)
       return fail(AFL_ERR_DEVICE, "hipEventElapsedTime failed");
+    op->timer_ms[i] = ms;
     sum += ms;
   }
   *launches = t.used;
   *total_ms = sum;
   if (t.dropped) return fail(AFL_ERR_STATE, "kernel timer: more launches than max_launches");
+  return AFL_OK;
+}
+int afl_kernel_timer_result(afl_operator* op, int kernel, int stage, int* launches, double* total_ms,
+                            double* bytes) {
+  if (!op || !launches || !total_ms || !bytes) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (stage < 0 || stage > 3) return fail(AFL_ERR_INVALID_ARGUMENT, "stage must be 0..3");
+  const KernelTimer& t = op->timer;
+  if (t.mask) return fail(AFL_ERR_STATE, "kernel timer still running: call afl_kernel_timer_stop first");
+  int n = 0;
+  double ms = 0.0, b = 0.0;
+  for (int i = 0; i < t.used && i < (int)op->timer_ms.size(); ++i)
+    if (t.flag[i] == kernel && (stage == 0 || t.stage[i] == stage)) {
+      ++n;
+      ms += op->timer_ms[i];
+      b += t.bytes[i];
+    }
+  *launches = n;
+  *total_ms = ms;
+  *bytes = b;
   return AFL_OK;
 }
 

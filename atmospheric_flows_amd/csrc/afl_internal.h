@@ -16,22 +16,29 @@ struct TableauC {
   double b1, b2, b3;
 };
 
-// HIP-event timing of every launch of one kernel (benchmark instrumentation). The launch itself
-// records the pair (hipExtLaunchKernelGGL start/stop events), so the events bracket exactly the
-// kernel's execution.
+// HIP-event timing of every launch of the kernels in a mask of AFL_TIME_* flags (benchmark
+// instrumentation). The launch itself records the pair (hipExtLaunchKernelGGL start/stop events), so
+// the events bracket exactly the kernel's execution. Each timed launch also records its flag, its
+// IMEX stage and its algorithmic bytes.
 struct KernelTimer {
-  int which = 0;  // 0 off; a KernelId below
+  unsigned mask = 0;  // 0: off
   int used = 0;
   int dropped = 0;
   std::vector<hipEvent_t> ev;  // pairs (start, stop)
-  // the event pair for the next launch of kernel id, or nullptr when not timing it
-  hipEvent_t* next(int id) {
-    if (which != id) return nullptr;
+  std::vector<int> flag, stage;
+  std::vector<double> bytes;
+  // the event pair for the next launch of kernel f, or nullptr when not timing it
+  hipEvent_t* next(int f, int stg = 0, double b = 0.0) {
+    if (!(mask & (unsigned)f)) return nullptr;
     if (2 * used + 1 >= (int)ev.size()) { ++dropped; return nullptr; }
+    flag[used] = f;
+    stage[used] = stg;
+    bytes[used] = b;
     return &ev[2 * used++];
   }
 };
-enum KernelId { kTimeLevelW = 1, kTimeLevelCCheb = 2 };
+enum KernelId { kTimeLevelW = 1, kTimeLevelCCheb = 2, kTimeRhsRho = 4, kTimeRhsU = 8, kTimeRhsP = 16,
+                kTimeRhsPUfix = 32, kTimeSchurW = 64, kTimeApplyC = 128, kTimeMassSolve = 256, kTimeApplyB = 512 };
 
 // Everything a launcher needs; a plain copy of the operator state.
 struct OpCtx {
@@ -46,7 +53,15 @@ struct OpCtx {
   hipStream_t stream = nullptr;
   double cell_tol = 1e-15;  // relative tolerance of the cell-local mass-type solves
   int cell_maxit = 40;
+  KernelTimer* timer = nullptr;  // set while afl_kernel_timer_* is active
 };
+
+// Launch on the context's stream; when the timer times kernel `flag`, bracket it with its events.
+template <typename F, typename... Args>
+inline void launch_timed(const OpCtx& o, int flag, int stage, double bytes, F kern, dim3 gr, dim3 bl, Args... args) {
+  hipEvent_t* ev = o.timer ? o.timer->next(flag, stage, bytes) : nullptr;
+  hipExtLaunchKernelGGL(kern, gr, bl, 0, o.stream, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, 0, args...);
+}
 
 // operator kernels (afl_ops.hip)
 void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double* dst);

@@ -336,10 +336,13 @@ static void level_schur_cheb(const MgLevelT<R>& L, const OpCtx& fine, const R* p
   const double coeff = fine.hyp == 1 ? fine.tab.at22 : fine.tab.at33;
   const R f = (R)(coeff * fine.dt / (fine.Ma * fine.Ma));
   const R cdt = (R)(coeff * fine.dt);
+  // algorithmic bytes: each distinct field once in the level precision, plus one scale per cell
+  const double bw = (double)sizeof(R) * (double)((1 + L.dim) * L.N + L.ncells);
+  const double bc = (double)sizeof(R) * (double)((7 + L.dim) * L.N + L.ncells);
   launch_schur_w_r<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.rho, L.lean() ? L.sinv : nullptr, f, (R)L.cell_tol,
-                      L.cell_maxit, L.N, p, L.w, tm && L.lean() ? tm->next(kTimeLevelW) : nullptr);
+                      L.cell_maxit, L.N, p, L.w, tm && L.lean() ? tm->next(kTimeLevelW, fine.hyp, bw) : nullptr);
   launch_apply_C_cheb<R>(L.dim, L.k, L.rq, L.geo, L.dT, L.stream, L.pfix, cdt, L.N, L.w, p, L.jbar, rin, rout, (R)c1, (R)c2,
-                         d, x, tm ? tm->next(kTimeLevelCCheb) : nullptr);
+                         d, x, tm ? tm->next(kTimeLevelCCheb, fine.hyp, bc) : nullptr);
 }
 
 template <typename R>

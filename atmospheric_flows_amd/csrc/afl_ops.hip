@@ -1402,14 +1402,15 @@ static Srcs make_srcs(const double* const* src, int n) {
 void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double* dst) {
   const Srcs S = make_srcs(src, 2 * stage);
   const TableauC& t = o.tab;
+  const double by = 8.0 * (double)o.N * (stage * (1 + o.dim) + 1);  // s (rho_j, u_j) in, rhs out
   AFL_DISPATCH(o.dim, o.k, {
-    dim3 gr = grid_for<D_, K_>(o.geo.ncells);
+    dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (stage == 1)
-      k_rhs_rho<D_, K_, 1><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a21, 0.0, 0.0, o.N, dst);
+      launch_timed(o, kTimeRhsRho, 1, by, k_rhs_rho<D_, K_, 1>, gr, bl, o.geo, o.dT, S, o.dt, t.a21, 0.0, 0.0, o.N, dst);
     else if (stage == 2)
-      k_rhs_rho<D_, K_, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.a31, t.a32, 0.0, o.N, dst);
+      launch_timed(o, kTimeRhsRho, 2, by, k_rhs_rho<D_, K_, 2>, gr, bl, o.geo, o.dT, S, o.dt, t.a31, t.a32, 0.0, o.N, dst);
     else
-      k_rhs_rho<D_, K_, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, o.dt, t.b1, t.b2, t.b3, o.N, dst);
+      launch_timed(o, kTimeRhsRho, 3, by, k_rhs_rho<D_, K_, 3>, gr, bl, o.geo, o.dT, S, o.dt, t.b1, t.b2, t.b3, o.N, dst);
   });
 }
 
@@ -1428,14 +1429,17 @@ void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* d
     cf.w[0] = t.b1; cf.w[1] = t.b2; cf.w[2] = t.b3; cf.wp[0] = t.b1; cf.wp[1] = t.b2; cf.wp[2] = t.b3;
     cf.gw[0] = t.b1; cf.gw[1] = t.b2; cf.gw[2] = t.b3;
   }
+  // sources: (rho, u, p) per explicit state plus rho_new at stages 1-2; rhs (dim fields) out
+  const int d = o.dim;
+  const double by = 8.0 * (double)o.N * (stage == 1 ? 3 + d + d : stage == 2 ? 2 * d + 5 + d : 3 * (2 + d) + d);
   AFL_DISPATCH(o.dim, o.k, {
-    dim3 gr = grid_for<D_, K_>(o.geo.ncells);
+    dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (stage == 1)
-      k_rhs_u<D_, K_, 1, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsU, 1, by, k_rhs_u<D_, K_, 1, 2>, gr, bl, o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
     else if (stage == 2)
-      k_rhs_u<D_, K_, 2, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsU, 2, by, k_rhs_u<D_, K_, 2, 3>, gr, bl, o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
     else
-      k_rhs_u<D_, K_, 3, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsU, 3, by, k_rhs_u<D_, K_, 3, 3>, gr, bl, o.geo, o.dT, S, cf, o.dt, iMa2, iFr2, o.N, dst);
   });
 }
 
@@ -1456,18 +1460,24 @@ void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* d
     cf.gw[0] = t.b1; cf.gw[1] = t.b2;
     cf.gw[2] = t.b3;
   }
+  // sources as ordered for the stage (without u_fixed when !full); rhs out
+  const int d = o.dim;
+  const int nin = stage == 1 ? (full ? 2 * d + 3 : d + 3) : stage == 2 ? (full ? 3 * d + 5 : 2 * d + 5) : 4 * d + 7;
+  const double by = 8.0 * (double)o.N * (nin + 1);
   AFL_DISPATCH(o.dim, o.k, {
-    dim3 gr = grid_for<D_, K_>(o.geo.ncells);
+    dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (stage == 1 && full)
-      k_rhs_p<D_, K_, 1, 2, 1><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsP, 1, by, k_rhs_p<D_, K_, 1, 2, 1>, gr, bl, o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else if (stage == 1)
-      k_rhs_p<D_, K_, 1, 2, 1, false><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsP, 1, by, k_rhs_p<D_, K_, 1, 2, 1, false>, gr, bl, o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N,
+                   dst);
     else if (stage == 2 && full)
-      k_rhs_p<D_, K_, 2, 3, 2><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsP, 2, by, k_rhs_p<D_, K_, 2, 3, 2>, gr, bl, o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
     else if (stage == 2)
-      k_rhs_p<D_, K_, 2, 3, 2, false><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsP, 2, by, k_rhs_p<D_, K_, 2, 3, 2, false>, gr, bl, o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N,
+                   dst);
     else
-      k_rhs_p<D_, K_, 3, 3, 3><<<gr, dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
+      launch_timed(o, kTimeRhsP, 3, by, k_rhs_p<D_, K_, 3, 3, 3>, gr, bl, o.geo, o.dT, S, cf, o.dt, Ma2, iFr2, o.N, dst);
   });
 }
 
@@ -1476,9 +1486,10 @@ void launch_rhs_p_ufix(const OpCtx& o, int stage, const double* rho_new, const d
   const TableauC& t = o.tab;
   const double Ma2 = o.Ma * o.Ma, iFr2 = 1.0 / (o.Fr * o.Fr);
   const double cg = (stage == 1 ? t.at22 : t.at33) * o.dt * Ma2 * iFr2;
+  const double by = 8.0 * (double)o.N * (o.dim + 3);  // rho_new, u_fix, base in; rhs out
   AFL_DISPATCH(o.dim, o.k, {
-    k_rhs_p_ufix<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(
-        o.geo, o.dT, rho_new, u_fix, 0.5 * Ma2, cg, o.N, base, dst);
+    launch_timed(o, kTimeRhsPUfix, stage, by, k_rhs_p_ufix<D_, K_>, grid_for<D_, K_>(o.geo.ncells),
+                 dim3(Cfg<D_, K_>::NT), o.geo, o.dT, rho_new, u_fix, 0.5 * Ma2, cg, o.N, base, dst);
   });
 }
 
@@ -1497,29 +1508,39 @@ void launch_inv_diag(const OpCtx& o, const double* rho, double scale, int ncomp,
 void launch_apply_B(const OpCtx& o, const double* p, double* dst) {
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double f = coeff * o.dt / (o.Ma * o.Ma);
+  const double by = 8.0 * (double)o.N * (1 + o.dim);
   AFL_DISPATCH(o.dim, o.k, {
-    k_apply_B<D_, K_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, f, o.N, p, dst);
+    launch_timed(o, kTimeApplyB, (int)o.hyp, by, k_apply_B<D_, K_>, grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT),
+                 o.geo, o.dT, f, o.N, p, dst);
   });
 }
 
 void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst) {
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double cdt = coeff * o.dt;
+  const double by = 8.0 * (double)o.N * (o.dim + (pD ? 3 : 2));  // pfix, w [, pD] in; out
   AFL_DISPATCH(o.dim, o.k, {
+    const dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (pD)
-      k_apply_C<D_, K_, true, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, pD, 1.0 / (kGamma - 1.0), dst);
+      launch_timed(o, kTimeApplyC, (int)o.hyp, by, k_apply_C<D_, K_, true, double>, gr, bl, o.geo, o.dT, cdt, o.N, pfix,
+                   w, pD, 1.0 / (kGamma - 1.0), dst);
     else
-      k_apply_C<D_, K_, false, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, cdt, o.N, pfix, w, nullptr, 0.0, dst);
+      launch_timed(o, kTimeApplyC, (int)o.hyp, by, k_apply_C<D_, K_, false, double>, gr, bl, o.geo, o.dT, cdt, o.N, pfix,
+                   w, (const double*)nullptr, 0.0, dst);
   });
 }
 
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x) {
   const double tol2 = o.cell_tol;
+  const double by = 8.0 * (double)o.N * (2 * ncomp + (rho ? 1 : 0));
   AFL_DISPATCH(o.dim, o.k, {
+    const dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (ncomp == 1)
-      k_mass_solve<D_, K_, 1><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
+      launch_timed(o, kTimeMassSolve, (int)o.hyp, by, k_mass_solve<D_, K_, 1>, gr, bl, o.geo, o.dT, rho, scale, o.N, b, x,
+                   tol2, o.cell_maxit);
     else
-      k_mass_solve<D_, K_, D_><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, scale, o.N, b, x, tol2, o.cell_maxit);
+      launch_timed(o, kTimeMassSolve, (int)o.hyp, by, k_mass_solve<D_, K_, D_>, gr, bl, o.geo, o.dT, rho, scale, o.N, b,
+                   x, tol2, o.cell_maxit);
   });
 }
 
@@ -1527,8 +1548,10 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   const double tol2 = o.cell_tol;
+  const double by = 8.0 * (double)o.N * (2 + o.dim);  // p, rho in; w out
   AFL_DISPATCH(o.dim, o.k, {
-    k_schur_w<D_, K_, double><<<grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, rho, f, o.N, p, w, tol2, o.cell_maxit);
+    launch_timed(o, kTimeSchurW, (int)o.hyp, by, k_schur_w<D_, K_, double>, grid_for<D_, K_>(o.geo.ncells),
+                 dim3(Cfg<D_, K_>::NT), o.geo, o.dT, rho, f, o.N, p, w, tol2, o.cell_maxit);
   });
 }
 

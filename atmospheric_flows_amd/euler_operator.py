@@ -62,6 +62,8 @@ def load_library():
     L.afl_write_vtu.argtypes = [vp, C.c_char_p, dp, dp, dp, C.c_int]
     L.afl_kernel_timer_start.argtypes = [vp, C.c_int, C.c_int]
     L.afl_kernel_timer_stop.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.afl_kernel_timer_result.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
     L.afl_compute_diagnostics.argtypes = [vp, dp, dp, dp, C.POINTER(Diagnostics)]
     L.afl_l2_difference.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     for n in ("afl_vjp_rhs_rho_update", "afl_vjp_rhs_velocity_fixed", "afl_vjp_rhs_pressure"):
@@ -83,11 +85,16 @@ def load_library():
     return L
 
 
-def _ptr(t, size=8):
+def _ptr(t, size=8, numel=None):
+    """Device address of a contiguous floating-point tensor of element size `size`, checked to
+    hold exactly `numel` elements when given (an undersized buffer would become an out-of-bounds
+    access on the device)."""
     if t is None:
         return None
     if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == size and t.is_contiguous()):
         raise AflError(f"expected a contiguous float{8 * size} device tensor")
+    if numel is not None and t.numel() != numel:
+        raise AflError(f"expected {numel} elements, got a tensor of {t.numel()}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -109,6 +116,10 @@ class EulerOperator:
         self._chk(self.L.afl_n_dofs(self.h, 1, C.byref(n)))
         self.N = n.value
 
+    def set_stream(self, stream):
+        """Order every later call on this torch stream (afl_set_stream)."""
+        self._chk(self.L.afl_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
     def __del__(self):
         if getattr(self, "h", None):
             self.L.afl_destroy(self.h)
@@ -117,6 +128,13 @@ class EulerOperator:
     def _chk(self, rc):
         if rc != 0:
             raise AflError(f"afl error {rc}: {self.L.afl_last_error().decode()}")
+
+    # device addresses checked against the field sizes: scalar (N), velocity (dim N), one value per
+    # observation point, and the FP32 variants
+    def _s(self, t): return _ptr(t, 8, self.N)
+    def _v(self, t): return _ptr(t, 8, self.dim * self.N)
+    def _sv(self, t, vec): return self._v(t) if vec else self._s(t)
+    def _obs(self, t): return _ptr(t, 8, self.npts)
 
     # ---- allocation helpers -----------------------------------------------------------------
     def new_scalar(self):
@@ -131,9 +149,9 @@ class EulerOperator:
     def set_Froude(self, v): self._chk(self.L.afl_set_froude(self.h, v))
     def set_HYPERBOLIC_stage(self, s): self._chk(self.L.afl_set_hyperbolic_stage(self.h, s))
     def set_NS_stage(self, s): self._chk(self.L.afl_set_ns_stage(self.h, s))
-    def set_rho_for_fixed(self, t): self._chk(self.L.afl_set_rho_for_fixed(self.h, _ptr(t)))
-    def set_pres_fixed(self, t): self._chk(self.L.afl_set_pres_fixed(self.h, _ptr(t)))
-    def set_u_fixed(self, t): self._chk(self.L.afl_set_u_fixed(self.h, _ptr(t)))
+    def set_rho_for_fixed(self, t): self._chk(self.L.afl_set_rho_for_fixed(self.h, self._s(t)))
+    def set_pres_fixed(self, t): self._chk(self.L.afl_set_pres_fixed(self.h, self._s(t)))
+    def set_u_fixed(self, t): self._chk(self.L.afl_set_u_fixed(self.h, self._v(t)))
     def set_solver_params(self, sp: SolverParams): self._chk(self.L.afl_set_solver_params(self.h, C.byref(sp)))
     def set_sponge(self, s: Sponge): self._chk(self.L.afl_set_sponge(self.h, C.byref(s)))
 
@@ -148,47 +166,63 @@ class EulerOperator:
         return n.value
 
     # ---- explicit-stage applies ---------------------------------------------------------------
-    def _rhs(self, fn, dst, srcs):
-        arr = (C.c_void_p * len(srcs))(*[t.data_ptr() for t in srcs])
-        for t in srcs:
-            _ptr(t)
-        self._chk(fn(self.h, _ptr(dst), arr, len(srcs)))
+    # positional sources: E1 alternates (rho_j, u_j); E2 and E3 take (rho, u, p) triples and a trailing
+    # (rho, u) pair, so position i is a velocity exactly when i % 3 == 1
+    @staticmethod
+    def _is_vec(kind, i):
+        return i % 2 == 1 if kind == "E1" else i % 3 == 1
+
+    def _rhs(self, fn, kind, dst, srcs, dst_vec):
+        ptrs = [self._sv(t, self._is_vec(kind, i)) for i, t in enumerate(srcs)]
+        arr = (C.c_void_p * len(srcs))(*[p.value for p in ptrs])
+        self._chk(fn(self.h, self._sv(dst, dst_vec), arr, len(srcs)))
         return dst
 
-    def vmult_rhs_rho_update(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_rho_update, dst, srcs)
-    def vmult_rhs_velocity_fixed(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_velocity_fixed, dst, srcs)
-    def vmult_rhs_pressure(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_pressure, dst, srcs)
+    def vmult_rhs_rho_update(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_rho_update, "E1", dst, srcs, False)
+    def vmult_rhs_velocity_fixed(self, dst, srcs):
+        return self._rhs(self.L.afl_vmult_rhs_velocity_fixed, "E2", dst, srcs, True)
+    def vmult_rhs_pressure(self, dst, srcs): return self._rhs(self.L.afl_vmult_rhs_pressure, "E3", dst, srcs, False)
 
     # ---- linear blocks -------------------------------------------------------------------------
-    def vmult(self, dst, src): self._chk(self.L.afl_vmult(self.h, _ptr(dst), _ptr(src))); return dst
-    def vmult_pressure(self, dst, src): self._chk(self.L.afl_vmult_pressure(self.h, _ptr(dst), _ptr(src))); return dst
-    def vmult_enthalpy(self, dst, src): self._chk(self.L.afl_vmult_enthalpy(self.h, _ptr(dst), _ptr(src))); return dst
+    def vmult(self, dst, src):
+        n = self.m()
+        self._chk(self.L.afl_vmult(self.h, _ptr(dst, 8, n), _ptr(src, 8, n)))
+        return dst
+
+    def vmult_pressure(self, dst, src):
+        self._chk(self.L.afl_vmult_pressure(self.h, self._v(dst), self._s(src)))
+        return dst
+
+    def vmult_enthalpy(self, dst, src):
+        self._chk(self.L.afl_vmult_enthalpy(self.h, self._s(dst), self._v(src)))
+        return dst
+
     def vmult_schur_velocity(self, w, p):
-        self._chk(self.L.afl_vmult_schur_velocity(self.h, _ptr(w), _ptr(p)))
+        self._chk(self.L.afl_vmult_schur_velocity(self.h, self._v(w), self._s(p)))
         return w
 
     # ---- discrete-adjoint building blocks -------------------------------------------------------
-    def _vjp(self, fn, rbar, srcs, outs):
-        for t in list(srcs) + list(outs):
-            _ptr(t)
+    def _vjp(self, fn, kind, rbar, srcs, outs, rbar_vec):
         if len(outs) != len(srcs):
             raise AflError("vector-Jacobian product: one output per source")
-        sarr = (C.c_void_p * len(srcs))(*[t.data_ptr() for t in srcs])
-        oarr = (C.c_void_p * len(outs))(*[t.data_ptr() for t in outs])
-        self._chk(fn(self.h, _ptr(rbar), oarr, sarr, len(srcs)))
+        sp = [self._sv(t, self._is_vec(kind, i)) for i, t in enumerate(srcs)]
+        op_ = [self._sv(t, self._is_vec(kind, i)) for i, t in enumerate(outs)]
+        sarr = (C.c_void_p * len(srcs))(*[p.value for p in sp])
+        oarr = (C.c_void_p * len(outs))(*[p.value for p in op_])
+        self._chk(fn(self.h, self._sv(rbar, rbar_vec), oarr, sarr, len(srcs)))
         return outs
 
     def vjp_rhs_rho_update(self, rbar, srcs, outs):
         """outs[i] = (d rhs_rho / d srcs[i])^T rbar for the E1 sources of the current stage."""
-        return self._vjp(self.L.afl_vjp_rhs_rho_update, rbar, srcs, outs)
+        return self._vjp(self.L.afl_vjp_rhs_rho_update, "E1", rbar, srcs, outs, False)
 
     def vjp_rhs_pressure(self, rbar, srcs, outs):
         """outs[i] = (d rhs_p / d srcs[i])^T rbar for the E3 sources of the current stage."""
-        return self._vjp(self.L.afl_vjp_rhs_pressure, rbar, srcs, outs)
+        return self._vjp(self.L.afl_vjp_rhs_pressure, "E3", rbar, srcs, outs, False)
 
     def vjp_rhs_velocity_fixed(self, rbar, srcs, outs):
         """outs[i] = (d rhs_u / d srcs[i])^T rbar for the E2 sources of the current stage."""
-        return self._vjp(self.L.afl_vjp_rhs_velocity_fixed, rbar, srcs, outs)
+        return self._vjp(self.L.afl_vjp_rhs_velocity_fixed, "E2", rbar, srcs, outs, True)
 
     def imex_step_vjp(self, rho, u, p, rho_bar_out, u_bar_out, p_bar_out):
         """Adjoint of one IMEX step at the input state (rho, u, p): returns (rho_bar, u_bar, p_bar),
@@ -196,8 +230,9 @@ class EulerOperator:
         import torch
         rb, ub, pb = torch.empty_like(rho), torch.empty_like(u), torch.empty_like(p)
         info = StepInfo()
-        self._chk(self.L.afl_imex_step_vjp(self.h, _ptr(rho), _ptr(u), _ptr(p), _ptr(rho_bar_out), _ptr(u_bar_out),
-                                           _ptr(p_bar_out), _ptr(rb), _ptr(ub), _ptr(pb), C.byref(info)))
+        self._chk(self.L.afl_imex_step_vjp(self.h, self._s(rho), self._v(u), self._s(p), self._s(rho_bar_out),
+                                           self._v(u_bar_out), self._s(p_bar_out), self._s(rb), self._v(ub),
+                                           self._s(pb), C.byref(info)))
         return rb, ub, pb, info
 
     def adjoint_run(self, rho0, u0, p0, nsteps, snapshots, rho_bar_T, u_bar_T, p_bar_T):
@@ -206,8 +241,9 @@ class EulerOperator:
         import torch
         rb, ub, pb = torch.empty_like(rho0), torch.empty_like(u0), torch.empty_like(p0)
         st = AdjointStats()
-        self._chk(self.L.afl_adjoint_run(self.h, _ptr(rho0), _ptr(u0), _ptr(p0), int(nsteps), int(snapshots),
-                                         _ptr(rho_bar_T), _ptr(u_bar_T), _ptr(p_bar_T), _ptr(rb), _ptr(ub), _ptr(pb),
+        self._chk(self.L.afl_adjoint_run(self.h, self._s(rho0), self._v(u0), self._s(p0), int(nsteps), int(snapshots),
+                                         self._s(rho_bar_T), self._v(u_bar_T), self._s(p_bar_T), self._s(rb),
+                                         self._v(ub), self._s(pb),
                                          C.byref(st)))
         return rb, ub, pb, {"forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
                             "snapshots_used": st.snapshots_used}
@@ -223,117 +259,128 @@ class EulerOperator:
     def observe_theta(self, st, rho, p):
         import torch
         out = torch.empty(self.npts, dtype=torch.float64, device="cuda")
-        self._chk(self.L.afl_observe_theta(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(out)))
+        self._chk(self.L.afl_observe_theta(self.h, C.byref(st), self._s(rho), self._s(p), self._obs(out)))
         return out
 
     def observe_theta_transpose(self, st, rho, p, wbar):
         import torch
         rb, pb = torch.empty_like(rho), torch.empty_like(p)
-        self._chk(self.L.afl_observe_theta_transpose(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(wbar), _ptr(rb),
-                                                     _ptr(pb)))
+        self._chk(self.L.afl_observe_theta_transpose(self.h, C.byref(st), self._s(rho), self._s(p), self._obs(wbar),
+                                                     self._s(rb), self._s(pb)))
         return rb, pb
 
     def vmult_pressure_transpose(self, dst, src):
-        self._chk(self.L.afl_vmult_pressure_transpose(self.h, _ptr(dst), _ptr(src)))
+        self._chk(self.L.afl_vmult_pressure_transpose(self.h, self._s(dst), self._v(src)))
         return dst
 
     def vmult_enthalpy_tangent(self, dst, dw, w0):
-        self._chk(self.L.afl_vmult_enthalpy_tangent(self.h, _ptr(dst), _ptr(dw), _ptr(w0)))
+        self._chk(self.L.afl_vmult_enthalpy_tangent(self.h, self._s(dst), self._v(dw), self._v(w0)))
         return dst
 
     def vmult_enthalpy_tangent_transpose(self, dst, cbar, w0):
-        self._chk(self.L.afl_vmult_enthalpy_tangent_transpose(self.h, _ptr(dst), _ptr(cbar), _ptr(w0)))
+        self._chk(self.L.afl_vmult_enthalpy_tangent_transpose(self.h, self._v(dst), self._s(cbar), self._v(w0)))
         return dst
 
     def vmult_enthalpy_coefficient_transpose(self, dst, cbar, w):
-        self._chk(self.L.afl_vmult_enthalpy_coefficient_transpose(self.h, _ptr(dst), _ptr(cbar), _ptr(w)))
+        self._chk(self.L.afl_vmult_enthalpy_coefficient_transpose(self.h, self._s(dst), self._s(cbar), self._v(w)))
         return dst
 
     def vjp_rho_for_fixed(self, dst, ybar, u):
-        self._chk(self.L.afl_vjp_rho_for_fixed(self.h, _ptr(dst), _ptr(ybar), _ptr(u)))
+        self._chk(self.L.afl_vjp_rho_for_fixed(self.h, self._s(dst), self._v(ybar), self._v(u)))
         return dst
 
     def vjp_rhs_pressure_u_fixed(self, dst, rbar, rho_new, u_fixed):
-        self._chk(self.L.afl_vjp_rhs_pressure_u_fixed(self.h, _ptr(dst), _ptr(rbar), _ptr(rho_new), _ptr(u_fixed)))
+        self._chk(self.L.afl_vjp_rhs_pressure_u_fixed(self.h, self._v(dst), self._s(rbar), self._s(rho_new),
+                                                      self._v(u_fixed)))
         return dst
 
     def vmult_schur_tangent(self, dst, src, p0):
-        self._chk(self.L.afl_vmult_schur_tangent(self.h, _ptr(dst), _ptr(src), _ptr(p0)))
+        self._chk(self.L.afl_vmult_schur_tangent(self.h, self._s(dst), self._s(src), self._s(p0)))
         return dst
 
     def vmult_schur_tangent_transpose(self, dst, src, p0):
-        self._chk(self.L.afl_vmult_schur_tangent_transpose(self.h, _ptr(dst), _ptr(src), _ptr(p0)))
+        self._chk(self.L.afl_vmult_schur_tangent_transpose(self.h, self._s(dst), self._s(src), self._s(p0)))
         return dst
 
     def mg_vmult(self, dst, src):
         """Schur preconditioner apply (p-multigrid V-cycle of the selected schur_precond)."""
-        self._chk(self.L.afl_mg_vmult(self.h, _ptr(dst), _ptr(src)))
+        self._chk(self.L.afl_mg_vmult(self.h, self._s(dst), self._s(src)))
         return dst
 
     def mg_level_vmult(self, dst, src):
         """The finest multigrid level's Schur operator (level precision and quadrature) applied to src."""
-        self._chk(self.L.afl_mg_level_vmult(self.h, _ptr(dst), _ptr(src)))
+        self._chk(self.L.afl_mg_level_vmult(self.h, self._s(dst), self._s(src)))
         return dst
 
     def vmult_schur_velocity_f32(self, w, p):
         """FP32 fine-level w = A^-1 B p of the FP32 V-cycle (float32 tensors)."""
-        self._chk(self.L.afl_vmult_schur_velocity_f32(self.h, _ptr(w, 4), _ptr(p, 4)))
+        self._chk(self.L.afl_vmult_schur_velocity_f32(self.h, _ptr(w, 4, self.dim * self.N), _ptr(p, 4, self.N)))
         return w
 
     # ---- benchmark instrumentation -------------------------------------------------------------
-    def kernel_timer_start(self, kernel, max_launches=100000):
-        """HIP events around every launch of one multigrid level kernel (AFL_TIME_* constants)."""
-        self._chk(self.L.afl_kernel_timer_start(self.h, kernel, max_launches))
+    def kernel_timer_start(self, kernels, max_launches=100000):
+        """HIP events around every launch of the kernels in `kernels` (an or-ed mask of AFL_TIME_*)."""
+        self._chk(self.L.afl_kernel_timer_start(self.h, kernels, max_launches))
 
     def kernel_timer_stop(self):
-        """(launches, total milliseconds) since kernel_timer_start."""
+        """(launches, total milliseconds) of every timed kernel since kernel_timer_start."""
         n, ms = C.c_int(), C.c_double()
         self._chk(self.L.afl_kernel_timer_stop(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def kernel_timer_result(self, kernel, stage=0):
+        """(launches, total milliseconds, total algorithmic bytes) of one timed kernel after
+        kernel_timer_stop, over IMEX stage `stage` (0: all)."""
+        n, ms, b = C.c_int(), C.c_double(), C.c_double()
+        self._chk(self.L.afl_kernel_timer_result(self.h, kernel, stage, C.byref(n), C.byref(ms), C.byref(b)))
+        return n.value, ms.value, b.value
 
     # ---- diagnostics (Atmospheric_Flow.cc:925-1087, 1225-1235) ---------------------------------
     def diagnostics(self, rho, u=None, p=None):
         """Density bounds, maximum celerity and speed, minimal cell diameter and CFL numbers."""
         d = Diagnostics()
-        self._chk(self.L.afl_compute_diagnostics(self.h, _ptr(rho), _ptr(u), _ptr(p), C.byref(d)))
+        self._chk(self.L.afl_compute_diagnostics(self.h, self._s(rho), None if u is None else self._v(u),
+                                                   None if p is None else self._s(p), C.byref(d)))
         return d.as_dict()
 
     def write_vtu(self, path, rho, u, p, high_order=False):
         """VTU file of the state (point data rho, u, p on the degree-k patch points of each cell)."""
-        self._chk(self.L.afl_write_vtu(self.h, os.fsencode(path), _ptr(rho), _ptr(u), _ptr(p), 1 if high_order else 0))
+        self._chk(self.L.afl_write_vtu(self.h, os.fsencode(path), self._s(rho), self._v(u), self._s(p),
+                                         1 if high_order else 0))
 
     def l2_difference(self, ncomp, a, b=None):
         """(||a - b||_L2, ||b||_L2) of ncomp-component fields."""
         e, r = C.c_double(), C.c_double()
-        self._chk(self.L.afl_l2_difference(self.h, ncomp, _ptr(a), _ptr(b), C.byref(e), C.byref(r)))
+        self._chk(self.L.afl_l2_difference(self.h, ncomp, _ptr(a, 8, ncomp * self.N),
+                                             None if b is None else _ptr(b, 8, ncomp * self.N), C.byref(e), C.byref(r)))
         return e.value, r.value
 
     def compute_diagonal(self): self._chk(self.L.afl_compute_diagonal(self.h))
 
     def get_matrix_diagonal_inverse(self, dst):
-        self._chk(self.L.afl_get_matrix_diagonal_inverse(self.h, _ptr(dst)))
+        self._chk(self.L.afl_get_matrix_diagonal_inverse(self.h, _ptr(dst, 8, self.m())))
         return dst
 
     # ---- solves --------------------------------------------------------------------------------
-    def solve_mass(self, x, b): self._chk(self.L.afl_solve_mass(self.h, _ptr(x), _ptr(b))); return x
+    def solve_mass(self, x, b): self._chk(self.L.afl_solve_mass(self.h, _ptr(x, 8, self.m()), _ptr(b, 8, self.m()))); return x
 
     def schur_solve(self, p, rhs):
         it = C.c_int(0)
         res = C.c_double(0.0)
-        self._chk(self.L.afl_schur_solve(self.h, _ptr(p), _ptr(rhs), C.byref(it), C.byref(res)))
+        self._chk(self.L.afl_schur_solve(self.h, self._s(p), self._s(rhs), C.byref(it), C.byref(res)))
         return it.value, res.value
 
     def imex_step(self, rho, u, p):
         info = StepInfo()
-        self._chk(self.L.afl_imex_step(self.h, _ptr(rho), _ptr(u), _ptr(p), C.byref(info)))
+        self._chk(self.L.afl_imex_step(self.h, self._s(rho), self._v(u), self._s(p), C.byref(info)))
         return info
 
     def init_hill_state(self, st: HillState):
         rho, u, p = self.new_scalar(), self.new_vector(), self.new_scalar()
-        self._chk(self.L.afl_init_hill_state(self.h, C.byref(st), _ptr(rho), _ptr(u), _ptr(p)))
+        self._chk(self.L.afl_init_hill_state(self.h, C.byref(st), self._s(rho), self._v(u), self._s(p)))
         return rho, u, p
 
     def background(self, st: HillState):
         rho, p, th = self.new_scalar(), self.new_scalar(), self.new_scalar()
-        self._chk(self.L.afl_background(self.h, C.byref(st), _ptr(rho), _ptr(p), _ptr(th)))
+        self._chk(self.L.afl_background(self.h, C.byref(st), self._s(rho), self._s(p), self._s(th)))
         return rho, p, th

@@ -1,17 +1,27 @@
 """Benchmark of the IMEX-DG hot path on one MI355X.
 
-Workload: 2D nonhydrostatic flow over a Witch-of-Agnesi hill, 512x100 terrain-following
-cells, degree 4 (N = 1.28 M nodes per field, 5.12 M DoF in rho, u, w, p), hydrostatic
-constant-N background plus seeded N(0, 0.01 K) theta noise, dt = 1 s, Schur GMRES tolerance
-1e-10, fixed-point tolerance 1e-10. A "step" is one full IMEX-ARK2 step on the device.
+Headline workload (the mixed-precision 3D configuration): isolated Agnesi hill
+h = 400 m / (1 + (r / 1 km)^2) in a 96x96x40-cell terrain-following box of 144 x 144 x 25 km,
+degree 3 (N = 23.6 M nodes per field, 118 M DoF in rho, u, v, w, p), hydrostatic constant-N
+background (N = 0.01 1/s, theta0 = 280 K, p0 = 1e5 Pa, u = 10 m/s) plus seeded N(0, 0.01 K) theta
+noise, dt = 1 s. A "step" is one full IMEX-ARK2 step on the device. The FP64 state, explicit stages
+and outer flexible GMRES (true residual 1e-8) are preconditioned by an FP32 multigrid V-cycle; the
+all-FP64 Jacobi-GMRES path and the FP64 V-cycle are timed on the same inputs, and the theta' and w
+differences of the FP32-preconditioned state from the all-FP64 one are reported.
 
-Prints one JSON line: IMEX-step DoF/s (value), isolated explicit-stage and Schur applies,
-the roofline of the dominant kernel (HIP events on the library's stream) and the CPU
-baseline (the oracle's FP64 restatement on the host cores, bounded sample).
+Further legs: isolated operator applies on the headline mesh (HIP events on the library's stream);
+per-kernel HIP-event timing inside the FP32-preconditioned steps (the roofline object is quoted for
+the kernel with the most time per step); the 2D 512x100 degree-4 hill (FP32-MG vs FP64 steps,
+isolated applies over buffer sets rotated beyond the 256 MB Infinity Cache); the adjoint gradient on
+the 2D 256x60 degree-3 hill (200 steps, 20 snapshots and every-step storage, against the forward
+run); and the CPU baseline, the oracle's FP64 restatement on the host cores (bounded sample).
+
+Prints one JSON line.
 """
 import argparse
 import json
 import os
+import platform
 import subprocess
 import sys
 import time
@@ -20,211 +30,180 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 FR = 1.0 / 9.80616 ** 0.5
+R_GAS, CP = 287.0, 1.4 * 287.0 / 0.4
+HEADLINE = "c3"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed IMEX steps per path")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--reps", type=int, default=200, help="isolated-apply repetitions")
-    ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--nz", type=int, default=100)
-    ap.add_argument("--degree", type=int, default=4)
+    ap.add_argument("--reps", type=int, default=20, help="isolated-apply repetitions")
+    ap.add_argument("--legs", default="c3,applies,kernels,c2,adjoint",
+                    help="comma list of c3 (headline, always run), applies, kernels, c2, adjoint")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-snapshots", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--cpu-cols", type=int, default=64, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
-def workload(nx, nz, k):
-    from atmospheric_flows_amd import make_mesh, make_hill_state, make_solver_params
-    mesh = make_mesh(dim=2, degree=k, n_cells=(nx, 1, nz), x_range=(-72e3, 72e3), z_top=25e3,
+def c3_workload():
+    from atmospheric_flows_amd import make_mesh, make_hill_state
+    mesh = make_mesh(dim=3, degree=3, n_cells=(96, 96, 40), x_range=(-72e3, 72e3), y_range=(-72e3, 72e3),
+                     z_top=25e3, hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
+    st = make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01, seed=2024)
+    return mesh, st
+
+
+def c2_workload():
+    from atmospheric_flows_amd import make_mesh, make_hill_state
+    mesh = make_mesh(dim=2, degree=4, n_cells=(512, 1, 100), x_range=(-72e3, 72e3), z_top=25e3,
                      hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
     st = make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01, seed=2024)
-    sp = make_solver_params(gmres_eps=1e-10, gmres_restart=28, fp_tol=1e-10, fp_max_iter=100)
-    return mesh, st, sp
+    return mesh, st
 
 
-def cpu_baseline(args):
-    """Oracle (FP64 restatement) IMEX step on a bounded sample of the same workload: a periodic
-    slab of cpu_cols x nz cells at the same resolution (dx = 144 km / nx), timed on the host."""
+def params(precond, eps):
+    from atmospheric_flows_amd import make_solver_params
+    return make_solver_params(gmres_eps=eps, gmres_restart=28, fp_tol=1e-10, fp_max_iter=100, schur_precond=precond)
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU baseline: the oracle (FP64 restatement, OpenMP over cells) on a bounded sample of the headline
+# workload, timed on the host cores in a subprocess
+def cpu_sample():
     import numpy as np
     from atmospheric_flows_amd import make_mesh
+    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI
     from oracle.pyoracle import OracleOperator
-    nx = args.cpu_cols
-    half = 72e3 * nx / args.nx
-    _, st, sp = workload(args.nx, args.nz, args.degree)
-    mesh = make_mesh(dim=2, degree=args.degree, n_cells=(nx, 1, args.nz), x_range=(-half, half), z_top=25e3,
-                     hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
-    op = OracleOperator(mesh, mach=1.0, froude=FR, dt=1.0)
-    op.set_solver_params(sp)
+    mesh, st = c3_workload()
+    # an 8x8x40-cell periodic column of the same mesh spacing (dx = dy = 1.5 km, 40 levels)
+    half = 72e3 * 8 / 96
+    sample = make_mesh(dim=3, degree=3, n_cells=(8, 8, 40), x_range=(-half, half), y_range=(-half, half),
+                       z_top=25e3, hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
+    op = OracleOperator(sample, mach=1.0, froude=FR, dt=1.0)
+    op.set_solver_params(params(AFL_SCHUR_JACOBI, 1e-8))
     rho, u, p = op.init_hill_state(st)
     t0 = time.perf_counter()
     op.imex_step(rho, u, p)
     dt = time.perf_counter() - t0
-    dofs = 4 * op.N
+    dofs = 5 * op.N
+    assert np.isfinite(p).all()
     return {"value": dofs / dt, "unit": "DoF/s", "seconds": dt,
-            "sample": f"1 IMEX step of the same hill problem on a {nx}x{args.nz}-cell periodic slab "
-                      f"(k={args.degree}, {dofs} DoF, same dx, dz, dt and tolerances)"}
+            "sample": f"1 IMEX step (plain FP64 Jacobi-GMRES, tolerance 1e-8) of the same 3D hill problem on an "
+                      f"8x8x40-cell periodic column (k=3, {dofs} DoF, same dx, dy, dz, dt)"}
 
 
-def run_cpu_subprocess(args, threads):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def run_cpu_subprocess():
+    threads = min(16, os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-only", "--nx", str(args.nx), "--nz", str(args.nz),
-           "--degree", str(args.degree), "--cpu-cols", str(args.cpu_cols)]
-    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only"], capture_output=True, text=True,
+                       env=env, timeout=900)
+    base = {"unit": "DoF/s", "cores": threads, "kind": "port", "host_nproc": os.cpu_count(),
+            "cpu_model": cpu_model(),
+            "build": "g++ -O3 -march=x86-64-v4 (AVX-512) -fopenmp -ffp-contract=off, OpenMP over cells"}
     if r.returncode != 0:
-        return {"value": None, "unit": "DoF/s", "cores": threads, "kind": "port", "error": r.stderr[-400:]}
-    out = json.loads(r.stdout.strip().splitlines()[-1])
-    out.update({"cores": threads, "kind": "port"})
-    return out
+        base.update({"value": None, "error": r.stderr[-400:]})
+        return base
+    base.update(json.loads(r.stdout.strip().splitlines()[-1]))
+    return base
 
 
-def profiled_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    of this workload (calibrated against an L2-cold copy of known size); None if not profiled."""
-    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{args.nx}x{args.nz}_k{args.degree}.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith(kernel + ":"):
-            for tok in line.split():
-                if tok.startswith("bytes_per_launch="):
-                    return float(tok.split("=")[1])
-    return None
-
-
+# ---------------------------------------------------------------------------------------------
 def main():
     args = parse()
     if args.cpu_only:
-        print(json.dumps(cpu_baseline(args)))
+        print(json.dumps(cpu_sample()))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+              "(the multi-GPU decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
     import torch
     from atmospheric_flows_amd import EulerOperator
-    mesh, st, sp = workload(args.nx, args.nz, args.degree)
-    op = EulerOperator(mesh, mach=1.0, froude=FR, dt=1.0)
-    op.set_solver_params(sp)
-    N, dim = op.N, op.dim
-    rho, u, p = op.init_hill_state(st)
-    torch.cuda.synchronize()
+    from atmospheric_flows_amd import _abi as A
+
     stream = torch.cuda.current_stream()
 
-    def timed(fn, reps):
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    def events_time(fn, reps):
         fn()
         torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for _ in range(reps):
-            fn()
+        for i in range(reps):
+            fn(i)
         e1.record(stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
+        return e0.elapsed_time(e1) / reps * 1e-3  # seconds per call
 
-    # calibration for the rocprofv3 FETCH_SIZE / WRITE_SIZE passes: an L2-cold scale-by-one of
-    # 4N doubles (torch elementwise kernel; reads and writes exactly 32 N bytes)
-    cal_src = torch.rand(4 * N, device="cuda", dtype=torch.float64)
-    flush = torch.empty(16 * 1024 * 1024, device="cuda", dtype=torch.float64).fill_(1.0)
-    cal_dst = torch.mul(cal_src, 1.0)
-    torch.cuda.synchronize()
-    del cal_src, cal_dst, flush
-
-    # isolated applies on the initial state (stage-3 explicit applies, Schur pieces)
-    r2, u2, p2 = rho.clone(), u.clone(), p.clone()
-    dst_s, dst_v = op.new_scalar(), op.new_vector()
-    op.set_HYPERBOLIC_stage(3)
-    src_u = [rho, u, p, r2, u2, p2, rho, u, p]
-    t_e2 = timed(lambda: op.vmult_rhs_velocity_fixed(dst_v, src_u), args.reps)
-    src_p = src_u + [r2, u2]
-    t_e3 = timed(lambda: op.vmult_rhs_pressure(dst_s, src_p), args.reps)
-    src_r = [rho, u, r2, u2, rho, u]
-    t_e1 = timed(lambda: op.vmult_rhs_rho_update(dst_s, src_r), args.reps)
-    op.set_HYPERBOLIC_stage(1)
-    op.set_rho_for_fixed(rho)
-    op.set_pres_fixed(p)
-    op.set_NS_stage(2)
-    t_schur = timed(lambda: op.vmult(dst_s, p), args.reps)
-    t_sw = timed(lambda: op.vmult_schur_velocity(dst_v, p), args.reps)
-    # the FP32 level operator of the preconditioner: w = A^-1 B p in single precision (the kernel
-    # that dominates the FP32-preconditioned step) and one full V-cycle
-    from atmospheric_flows_amd import make_solver_params
-    from atmospheric_flows_amd._abi import AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32, AFL_TIME_LEVEL_W, AFL_TIME_LEVEL_C_CHEB
-    op.set_solver_params(make_solver_params(gmres_eps=1e-10, gmres_restart=28, fp_tol=1e-10,
-                                            schur_precond=AFL_SCHUR_MG_FP32))
-    p32 = p.float()
-    w32 = torch.empty(dim * N, device="cuda", dtype=torch.float32)
-    t_sw32 = timed(lambda: op.vmult_schur_velocity_f32(w32, p32), args.reps)
-    t_mg = timed(lambda: op.mg_vmult(dst_s, p), max(1, args.reps // 10))
-
-    # IMEX steps: FP64 plain path (Jacobi GMRES) and FP32 p-multigrid FGMRES, same initial
-    # state, same true-residual tolerance 1e-10
-
-    def run_path(precond):
+    def run_path(mesh, st, precond, eps, steps, warmup):
         o = EulerOperator(mesh, mach=1.0, froude=FR, dt=1.0)
-        o.set_solver_params(make_solver_params(gmres_eps=1e-10, gmres_restart=28, fp_tol=1e-10,
-                                               schur_precond=precond))
+        o.set_solver_params(params(precond, eps))
         r0, u0, p0 = o.init_hill_state(st)
-        infos = []
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             o.imex_step(r0, u0, p0)
         torch.cuda.synchronize()
+        infos = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             infos.append(o.imex_step(r0, u0, p0))
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / args.steps, infos, (r0, u0, p0), o
+        return (time.perf_counter() - t0) / steps, infos, (r0, u0, p0), o
 
-    step_fp64, infos64, s64, _ = run_path(AFL_SCHUR_JACOBI)
-    step_s, infos, s32, o32 = run_path(AFL_SCHUR_MG_FP32)
-    dev = {name: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
-           for name, a, b in (("rho", s32[0], s64[0]), ("u", s32[1], s64[1]), ("p", s32[2], s64[2]))}
-    dofs = (2 + dim) * N
+    def theta_prime(o, st, rho, p):
+        _, _, th_bar = o.background(st)
+        th = p / (rho * R_GAS) * torch.pow(st.p0 / p, R_GAS / CP)
+        return th - th_bar
 
-    # per-kernel time of the two multigrid level kernels inside the same FP32 steps (HIP events
-    # recorded by the library on its stream around every launch; a second pass of args.steps
-    # steps continuing from the timed state)
-    ncells = N // (args.degree + 1) ** dim
-    level = {}
-    for kid, name, fields in ((AFL_TIME_LEVEL_W, "k_schur_w_lvl<float>", 1 + dim),
-                              (AFL_TIME_LEVEL_C_CHEB, "k_apply_C_cheb<float>", 7 + dim)):
-        o32.kernel_timer_start(kid)
-        for _ in range(args.steps):
-            o32.imex_step(*s32)
-        n, ms = o32.kernel_timer_stop()
-        # algorithmic bytes: each distinct field once (FP32), plus one 4-byte scale per cell
-        level[name] = {"launches_per_step": n / args.steps, "avg_us": 1e3 * ms / max(n, 1),
-                       "ms_per_step": ms / args.steps, "bytes_per_launch": fields * N * 4 + ncells * 4}
-    # gradient of a final-time functional <c, x_T> with respect to x_0 over the same number of
-    # steps (binomial checkpointing with two snapshots; FP32-preconditioned solves, also in the
-    # adjoint stage solves), against the forward run of those steps
-    gen = torch.Generator(device="cuda").manual_seed(1)
-    cbar = (torch.randn(N, dtype=torch.float64, device="cuda", generator=gen),
-            torch.randn(dim * N, dtype=torch.float64, device="cuda", generator=gen),
-            1e-5 * torch.randn(N, dtype=torch.float64, device="cuda", generator=gen))
-    x0 = tuple(t.clone() for t in s32)
-    adjoint = []
-    # two snapshots, a snapshot per step (each reverse step recomputes its own step), and every
-    # step's stage vectors kept from one forward sweep (-1: no recomputation)
-    for snaps in sorted({min(2, args.steps - 1), args.steps - 1}) + [-1]:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        *_, adj_stats = o32.adjoint_run(*x0, args.steps, snaps, *cbar)
-        torch.cuda.synchronize()
-        t_adj = time.perf_counter() - t0
-        adjoint.append({"steps": args.steps, "snapshots": snaps, "ms": t_adj * 1e3,
-                        "forward_run_ms": step_s * args.steps * 1e3,
-                        "cost_vs_forward_run": t_adj / (step_s * args.steps), "schedule": adj_stats})
-    dom = max(level, key=lambda k: level[k]["ms_per_step"])
-    bytes_dom = level[dom]["bytes_per_launch"]
-    ach = bytes_dom / (level[dom]["avg_us"] * 1e-6) / 1e9
-    field_bytes = N * 8
-    res = {
+    def rel_l2(o, ncomp, a, b):
+        e, r = o.l2_difference(ncomp, a, b)
+        return e / r
+
+    res = {}
+    # ---- headline: 3D 96x96x40, k=3, FP32-MG FGMRES vs FP64 Jacobi GMRES vs FP64-MG FGMRES -------------
+    mesh, st = c3_workload()
+    N = A.n_per_field(mesh)
+    dim, dofs = 3, 5 * N
+    paths = {}
+    for name, pc in (("fp32_mg", A.AFL_SCHUR_MG_FP32), ("fp64_jacobi", A.AFL_SCHUR_JACOBI),
+                     ("fp64_mg", A.AFL_SCHUR_MG_FP64)):
+        log(f"c3 path {name}: {args.warmup} warm-up + {args.steps} timed steps")
+        t, infos, state, o = run_path(mesh, st, pc, 1e-8, args.steps, args.warmup)
+        paths[name] = {"t": t, "infos": infos, "state": state, "op": o}
+        if name != "fp32_mg":
+            del o
+            paths[name]["op"] = None
+    o32 = paths["fp32_mg"]["op"]
+    ref = paths["fp64_jacobi"]["state"]
+    th_ref = theta_prime(o32, st, ref[0], ref[2])
+    errs = {}
+    for name in ("fp32_mg", "fp64_mg"):
+        s = paths[name]["state"]
+        th = theta_prime(o32, st, s[0], s[2])
+        errs[name] = {"theta_prime": rel_l2(o32, 1, th, th_ref),
+                      "w": rel_l2(o32, 1, s[1][2 * N:].contiguous(), ref[1][2 * N:].contiguous()),
+                      "rho": rel_l2(o32, 1, s[0], ref[0]), "u": rel_l2(o32, dim, s[1], ref[1]),
+                      "p": rel_l2(o32, 1, s[2], ref[2])}
+    step_s = paths["fp32_mg"]["t"]
+    res.update({
         "metric": "IMEX step wall-time, FP32-MG vs FP64 path at equal residual; DoF/s, % HBM peak",
         "value": dofs / step_s,
         "unit": "DoF/s",
@@ -237,35 +216,195 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"2D Agnesi hill {args.nx}x{args.nz} cells, k={args.degree}, FP64 state, "
-                               f"FP32 p-multigrid FGMRES Schur solve, dt=1 s, true residual 1e-10",
-                   "n_per_field": N, "dof": dofs, "solver": "fgmres+fp32-pmg"},
-        "fp_iters_per_step": [list(i.fp_iters) for i in infos],
-        "gmres_iters_per_step": [list(i.gmres_iters) for i in infos],
-        "fp64_plain_path": {"ms_per_step": step_fp64 * 1e3, "value": dofs / step_fp64,
-                            "gmres_iters_per_step": [list(i.gmres_iters) for i in infos64],
-                            "fp_iters_per_step": [list(i.fp_iters) for i in infos64]},
-        "speedup_fp32mg_vs_fp64": step_fp64 / step_s,
-        "rel_l2_fp32mg_vs_fp64_after_steps": dev,
-        "isolated_applies_us": {"E1_stage3": t_e1 * 1e6, "E2_stage3": t_e2 * 1e6, "E3_stage3": t_e3 * 1e6,
-                                "schur_vmult": t_schur * 1e6, "schur_velocity_kernel": t_sw * 1e6,
-                                "schur_velocity_kernel_fp32": t_sw32 * 1e6, "fp32_vcycle": t_mg * 1e6},
-        "isolated_applies_GBs": {"E1_stage3": 10 * field_bytes / t_e1 / 1e9,
-                                 "E2_stage3": 14 * field_bytes / t_e2 / 1e9,
-                                 "E3_stage3": 16 * field_bytes / t_e3 / 1e9,
-                                 "schur_vmult": 9 * field_bytes / t_schur / 1e9,
-                                 "schur_velocity_kernel": (2 + dim) * field_bytes / t_sw / 1e9},
-        "level_kernels_in_step": level,
-        "adjoint_gradient": adjoint,
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": ach, "bytes_per_launch": bytes_dom,
-                     "avg_launch_us": level[dom]["avg_us"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(dom.replace("<float>", f"<{dim}, {args.degree}, float, {args.degree + 1}>"), args)},
-    }
+        "config": {"workload": "3D Agnesi hill 96x96x40 cells (144x144x25 km), k=3, FP64 state and explicit stages, "
+                               "FP64 flexible GMRES (true residual 1e-8) with an FP32 p-multigrid Chebyshev V-cycle, "
+                               "dt=1 s, seeded N(0, 0.01 K) theta noise",
+                   "n_per_field": N, "dof": dofs, "solver": "fgmres+fp32-mg"},
+        "fp_iters_per_step": [list(i.fp_iters) for i in paths["fp32_mg"]["infos"]],
+        "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp32_mg"]["infos"]],
+        "fp64_plain_path": {"ms_per_step": paths["fp64_jacobi"]["t"] * 1e3, "value": dofs / paths["fp64_jacobi"]["t"],
+                            "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp64_jacobi"]["infos"]]},
+        "fp64_mg_path": {"ms_per_step": paths["fp64_mg"]["t"] * 1e3, "value": dofs / paths["fp64_mg"]["t"],
+                         "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp64_mg"]["infos"]]},
+        "speedup_fp32mg_vs_fp64": paths["fp64_jacobi"]["t"] / step_s,
+        "speedup_fp32mg_vs_fp64mg": paths["fp64_mg"]["t"] / step_s,
+        "rel_l2_vs_fp64_path_after_steps": {**errs, "steps": args.steps + args.warmup},
+    })
+    for name in ("fp64_jacobi", "fp64_mg"):
+        paths[name]["state"] = None
+
+    # ---- isolated applies on the headline mesh (every working set is far beyond the Infinity Cache) ----
+    if "applies" in legs:
+        log("c3 isolated applies")
+        rho, u, p = paths["fp32_mg"]["state"]
+        o = o32
+        dst_s, dst_v = o.new_scalar(), o.new_vector()
+        o.set_HYPERBOLIC_stage(3)
+        r2, u2, p2 = rho.clone(), u.clone(), p.clone()
+        src_u = [rho, u, p, r2, u2, p2, rho, u, p]
+        ap = {}
+        fb = N * 8
+        ap["E2_stage3"] = (events_time(lambda i=0: o.vmult_rhs_velocity_fixed(dst_v, src_u), args.reps), 18 * fb)
+        src_p = src_u + [r2, u2]
+        ap["E3_stage3"] = (events_time(lambda i=0: o.vmult_rhs_pressure(dst_s, src_p), args.reps), 20 * fb)
+        src_r = [rho, u, r2, u2, rho, u]
+        ap["E1_stage3"] = (events_time(lambda i=0: o.vmult_rhs_rho_update(dst_s, src_r), args.reps), 13 * fb)
+        o.set_HYPERBOLIC_stage(1)
+        o.set_rho_for_fixed(rho)
+        o.set_pres_fixed(p)
+        o.set_NS_stage(2)
+        ap["schur_vmult_fp64"] = (events_time(lambda i=0: o.vmult(dst_s, p), args.reps), 11 * fb)
+        ap["schur_w_fp64"] = (events_time(lambda i=0: o.vmult_schur_velocity(dst_v, p), args.reps), 5 * fb)
+        ap["apply_B"] = (events_time(lambda i=0: o.vmult_pressure(dst_v, p), args.reps), 4 * fb)
+        ap["apply_C"] = (events_time(lambda i=0: o.vmult_enthalpy(dst_s, u), args.reps), 5 * fb)
+        o.set_NS_stage(3)
+        ap["A_solve"] = (events_time(lambda i=0: o.solve_mass(dst_v, u), args.reps), 7 * fb)
+        p32, w32 = p.float(), torch.empty(dim * N, device="cuda", dtype=torch.float32)
+        ap["schur_w_fp32_level"] = (events_time(lambda i=0: o.vmult_schur_velocity_f32(w32, p32), args.reps), 5 * N * 4)
+        o.set_NS_stage(2)
+        ap["fp32_vcycle"] = (events_time(lambda i=0: o.mg_vmult(dst_s, p), max(2, args.reps // 4)), None)
+        res["isolated_applies_c3"] = {k: {"us": t * 1e6, **({"GBs": b / t / 1e9, "hbm_frac": b / t / 1e9 / HBM_PEAK_GBS}
+                                                             if b else {})} for k, (t, b) in ap.items()}
+        del dst_s, dst_v, r2, u2, p2, p32, w32
+        torch.cuda.synchronize()
+
+    # ---- per-kernel times inside the FP32-preconditioned steps; roofline of the dominant kernel --------
+    if "kernels" in legs:
+        log("c3 per-kernel timing inside FP32-MG steps")
+        state = paths["fp32_mg"]["state"]
+        mask = 0
+        for f in A.TIMED_KERNELS:
+            mask |= f
+        o32.kernel_timer_start(mask, 2000 + 1000 * args.steps)
+        for _ in range(args.steps):
+            o32.imex_step(*state)
+        o32.kernel_timer_stop()
+        kern = {}
+        for f, name in A.TIMED_KERNELS.items():
+            n, ms, by = o32.kernel_timer_result(f, 0)
+            if n:
+                kern[name] = {"launches_per_step": n / args.steps, "ms_per_step": ms / args.steps,
+                              "avg_us": 1e3 * ms / n, "bytes_per_launch": by / n,
+                              "GBs": by / (ms * 1e-3) / 1e9}
+                for stg in (1, 2, 3):
+                    ns_, mss, bys = o32.kernel_timer_result(f, stg)
+                    if ns_:
+                        kern[name][f"stage{stg}"] = {"launches": ns_, "avg_us": 1e3 * mss / ns_,
+                                                     "bytes_per_launch": bys / ns_, "GBs": bys / (mss * 1e-3) / 1e9}
+        res["kernels_in_fp32_step"] = kern
+        dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+        d = kern[dom]
+        res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "bytes_per_launch": d["bytes_per_launch"],
+                           "avg_launch_us": d["avg_us"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(dom)}
+    res["fp32_step_state_finite"] = bool(torch.isfinite(paths["fp32_mg"]["state"][2]).all())
+    del paths, o32
+    torch.cuda.synchronize()
+
+    # ---- 2D 512x100, k=4 (c2): steps and isolated applies over rotated buffer sets ------------------------
+    if "c2" in legs:
+        log("c2 leg")
+        mesh2, st2 = c2_workload()
+        t32, inf32, s32, o = run_path(mesh2, st2, A.AFL_SCHUR_MG_FP32, 1e-10, args.steps, args.warmup)
+        t64, inf64, s64, _ = run_path(mesh2, st2, A.AFL_SCHUR_JACOBI, 1e-10, args.steps, args.warmup)
+        N2 = o.N
+        sets = 4  # E2 stage-3 sources: 9 fields x 10 MB per set, 4 sets = 368 MB > 256 MB Infinity Cache
+        gen = torch.Generator(device="cuda").manual_seed(3)
+        rot = [[t * (1 + 1e-6 * torch.randn(t.numel(), dtype=torch.float64, device="cuda", generator=gen))
+                for t in s32] for _ in range(sets)]
+        dsts, dstv = [o.new_scalar() for _ in range(sets)], [o.new_vector() for _ in range(sets)]
+        fb = N2 * 8
+        o.set_HYPERBOLIC_stage(3)
+        ap2 = {}
+        ap2["E2_stage3"] = (events_time(lambda i=0: o.vmult_rhs_velocity_fixed(
+            dstv[i % sets], [*rot[i % sets], *rot[(i + 1) % sets], *rot[(i + 2) % sets]]), args.reps * 5), 14 * fb)
+        ap2["E3_stage3"] = (events_time(lambda i=0: o.vmult_rhs_pressure(
+            dsts[i % sets], [*rot[i % sets], *rot[(i + 1) % sets], *rot[(i + 2) % sets],
+                             rot[(i + 3) % sets][0], rot[(i + 3) % sets][1]]), args.reps * 5), 16 * fb)
+        o.set_HYPERBOLIC_stage(1)
+        o.set_rho_for_fixed(s32[0])
+        o.set_pres_fixed(s32[2])
+        o.set_NS_stage(2)
+        ap2["schur_vmult_fp64"] = (events_time(lambda i=0: o.vmult(dsts[i % sets], rot[i % sets][2]),
+                                               args.reps * 5), 9 * fb)
+        res["c2"] = {"workload": "2D Agnesi hill 512x100 cells, k=4, dt=1 s, true residual 1e-10",
+                     "dof": 4 * N2, "fp32_mg_ms_per_step": t32 * 1e3, "fp64_ms_per_step": t64 * 1e3,
+                     "value_fp32_mg": 4 * N2 / t32, "speedup_fp32mg_vs_fp64": t64 / t32,
+                     "gmres_iters_per_step_fp32_mg": [list(i.gmres_iters) for i in inf32],
+                     "gmres_iters_per_step_fp64": [list(i.gmres_iters) for i in inf64],
+                     "isolated_applies_rotated": {k: {"us": t * 1e6, "GBs": b / t / 1e9} for k, (t, b) in ap2.items()}}
+        del o, rot, dsts, dstv, s32, s64
+        torch.cuda.synchronize()
+
+    # ---- adjoint gradient on the 2D 256x60, k=3 hill --------------------------------------------------------
+    if "adjoint" in legs:
+        log(f"adjoint leg: {args.adjoint_steps} steps")
+        res["adjoint"] = adjoint_leg(args, torch, EulerOperator, A)
+
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        res["cpu_baseline"] = run_cpu_subprocess(args, threads)
+        log("CPU baseline")
+        res["cpu_baseline"] = run_cpu_subprocess()
     print(json.dumps(res))
+
+
+def adjoint_leg(args, torch, EulerOperator, A):
+    """J = 1/2 sum_k (theta'(x_k, T) - obs_k)^2 over 512 seeded-uniform points, obs from a seed-7
+    perturbed initial state; gradient with respect to the initial state by binomial checkpointing
+    and by every-step storage, against the forward run of the same steps."""
+    import numpy as np
+    from atmospheric_flows_amd import make_mesh, make_hill_state
+    mesh = make_mesh(dim=2, degree=3, n_cells=(256, 1, 60), x_range=(-72e3, 72e3), z_top=25e3,
+                     hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
+    st = make_hill_state(seed=2024)
+    o = EulerOperator(mesh, mach=1.0, froude=FR, dt=1.0)
+    o.set_solver_params(params(A.AFL_SCHUR_MG_FP32, 1e-10))
+    rng = np.random.default_rng(11)
+    pts = np.stack([rng.uniform(-70e3, 70e3, 512), rng.uniform(500.0, 20e3, 512)], axis=1)
+    o.set_observation_points(pts)
+    n = args.adjoint_steps
+    truth = o.init_hill_state(make_hill_state(seed=7))
+    for _ in range(n):
+        o.imex_step(*truth)
+    obs = o.observe_theta(st, truth[0], truth[2])
+    x0 = o.init_hill_state(st)
+    x = tuple(t.clone() for t in x0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        o.imex_step(*x)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    resid = o.observe_theta(st, x[0], x[2]) - obs
+    rb, pb = o.observe_theta_transpose(st, x[0], x[2], resid)
+    ub = torch.zeros_like(x[1])
+    out = {"workload": f"2D Agnesi hill 256x60 cells, k=3, {n} steps at dt=1 s, 512 observation points, "
+                       "FP32-MG preconditioned stage solves", "forward_run_s": t_fwd, "modes": []}
+    for snaps in (args.adjoint_snapshots, A_STORE_ALL):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = o.adjoint_run(*x0, n, snaps, rb, ub, pb)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out["modes"].append({"snapshots": snaps, "seconds": t, "cost_vs_forward_run": t / t_fwd,
+                             "schedule": g[3], "gradient_finite": bool(torch.isfinite(g[2]).all())})
+    return out
+
+
+A_STORE_ALL = -1
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    the headline workload (calibrated against a copy of known size); None if not profiled."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_hill3d_96x96x40_k3.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel + ":"):
+            for tok in line.split():
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=")[1])
+    return None
 
 
 if __name__ == "__main__":

@@ -179,11 +179,16 @@ int afl_observe_theta_transpose(afl_operator* op, const afl_hill_state* bg, cons
                                 const double* wbar, double* rho_bar, double* p_bar);
 
 /* Benchmark instrumentation, no counterpart in the reference (deal.II's TimerOutput times whole
- * phases, not kernels): HIP events recorded on the operator's stream around every launch of one
- * multigrid level kernel (enum afl_timed_kernel) until afl_kernel_timer_stop, at most max_launches
- * launches. Stop synchronises the stream and returns the launch count and summed duration. */
-int afl_kernel_timer_start(afl_operator* op, int kernel, int max_launches);
+ * phases, not kernels): HIP events recorded on the operator's stream around every launch of the
+ * kernels in `kernels` (an or-ed mask of enum afl_timed_kernel) until afl_kernel_timer_stop, at most
+ * max_launches launches in all. Stop synchronises the stream and returns the launch count and summed
+ * duration over every timed kernel; afl_kernel_timer_result then gives one kernel's launches, summed
+ * duration and summed algorithmic bytes (each distinct input and output field once), restricted to
+ * one IMEX stage (1..3; 0: every stage). */
+int afl_kernel_timer_start(afl_operator* op, int kernels, int max_launches);
 int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms);
+int afl_kernel_timer_result(afl_operator* op, int kernel, int stage, int* launches, double* total_ms,
+                            double* bytes);
 
 /* ---- diagnostics ------------------------------------------------------------------------ */
 /* VTU output of a state (EulerSolver::output_results, Atmospheric_Flow.cc:886-918): point data

@@ -105,10 +105,18 @@ typedef struct afl_sponge {
   int32_t pad_;
 } afl_sponge;
 
-/* Kernels that afl_kernel_timer_start can time (benchmark instrumentation). */
+/* Kernels that afl_kernel_timer_start can time (benchmark instrumentation); flags, or-ed into a mask. */
 enum afl_timed_kernel {
-  AFL_TIME_LEVEL_W = 1,      /* multigrid level operator, first half: w = P^-1 B p */
-  AFL_TIME_LEVEL_C_CHEB = 2  /* level C/D apply fused with one Chebyshev smoothing step */
+  AFL_TIME_LEVEL_W = 1,        /* multigrid level operator, first half: w = P^-1 B p */
+  AFL_TIME_LEVEL_C_CHEB = 2,   /* level C/D apply fused with one Chebyshev smoothing step */
+  AFL_TIME_RHS_RHO = 4,        /* E1, vmult_rhs_rho_update */
+  AFL_TIME_RHS_U = 8,          /* E2, vmult_rhs_velocity_fixed */
+  AFL_TIME_RHS_P = 16,         /* E3, vmult_rhs_pressure (whole, or its u_fixed-free part) */
+  AFL_TIME_RHS_P_UFIX = 32,    /* E3's u_fixed terms inside the fixed point */
+  AFL_TIME_SCHUR_W = 64,       /* FP64 Schur first half w = A^-1 B p */
+  AFL_TIME_APPLY_C = 128,      /* FP64 C apply (with D p: the Schur second half) */
+  AFL_TIME_MASS_SOLVE = 256,   /* FP64 cell-local M / A / D solves */
+  AFL_TIME_APPLY_B = 512       /* FP64 B apply */
 };
 
 /* Diagnostics of a state (EulerSolver::get_min_density / get_max_density /

@@ -226,3 +226,35 @@ def test_kernel_timer_counts_level_launches():
     gpu.kernel_timer_start(AFL_TIME_LEVEL_W, 8)
     n, ms = gpu.kernel_timer_stop()
     assert n == 0 and ms == 0.0
+
+
+def test_mg_solve_follows_a_stream_switch():
+    """afl_set_stream after a multigrid solve moves the V-cycle (every level) to the new stream:
+    the same solve on the second stream gives the same answer, with the first stream idle."""
+    say("[gpu mg] stream switch between two FP32-MG Schur solves")
+    import torch
+    mesh = make_mesh(dim=2, degree=3, n_cells=(16, 1, 8), x_range=(-20e3, 20e3), z_top=25e3)
+    gpu = EulerOperator(mesh, mach=1.0, froude=FR, dt=2.0)
+    gpu.set_solver_params(make_solver_params(gmres_eps=1e-10, schur_precond=AFL_SCHUR_MG_FP32, mg_levels=2,
+                                             mg_coarse_degree=8))
+    rho, u, p = gpu.init_hill_state(make_hill_state(theta_sigma=0.5, seed=5))
+    gpu.set_HYPERBOLIC_stage(1)
+    gpu.set_rho_for_fixed(rho)
+    gpu.set_pres_fixed(p)
+    gpu.set_NS_stage(2)
+    target = p * (1 + 1e-3 * torch.randn(p.numel(), dtype=torch.float64, device="cuda",
+                                         generator=torch.Generator(device="cuda").manual_seed(4)))
+    rhs = gpu.vmult(gpu.new_scalar(), target)
+    x1 = p.clone()
+    it1, _ = gpu.schur_solve(x1, rhs)
+    torch.cuda.synchronize()
+    s2 = torch.cuda.Stream()
+    gpu.set_stream(s2)
+    with torch.cuda.stream(s2):
+        x2 = p.clone()
+        it2, _ = gpu.schur_solve(x2, rhs)
+    s2.synchronize()
+    torch.cuda.synchronize()
+    say(f"  iterations {it1} / {it2}")
+    assert it1 == it2
+    assert float(torch.linalg.norm(x1 - x2) / torch.linalg.norm(x1)) < 1e-12
