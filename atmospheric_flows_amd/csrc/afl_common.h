@@ -186,7 +186,7 @@ struct LineGeo {
   R dhx, dhy;
   R wxy;    // horizontal quadrature weight product
   R hx, hy, hz, iH, zeta0;
-  __device__ void init(const Geo& g, const TabT<R>& T, const int* ic, int l) {
+  __device__ __attribute__((always_inline)) void init(const Geo& g, const TabT<R>& T, const int* ic, int l) {
     using C = Cfg<DIM, K, Q>;
     hx = R(g.hc[0]);
     hy = DIM == 3 ? R(g.hc[1]) : R(1);
@@ -208,11 +208,11 @@ struct LineGeo {
     c = hz * (R(1) - h * iH);
   }
   // JxW at vertical quadrature index qz
-  __device__ R jxw(const Geo& g, const TabT<R>& T, int qz) const {
+  __device__ __attribute__((always_inline)) R jxw(const Geo& g, const TabT<R>& T, int qz) const {
     return (DIM == 2 ? hx : hx * hy) * c * wxy * T.W[qz];
   }
   // gradient contribution (physical G, without JxW) -> reference coordinates * JxW
-  __device__ void gref(const Geo& g, const TabT<R>& T, const int* ic, int qz, const R* G, R* Rf) const {
+  __device__ __attribute__((always_inline)) void gref(const Geo& g, const TabT<R>& T, const int* ic, int qz, const R* G, R* Rf) const {
     const R zeta = zeta0 + T.QP[qz] * hz;
     const R fz = R(1) - zeta * iH;
     const R w = wxy * T.W[qz];
@@ -228,7 +228,7 @@ struct LineGeo {
     }
   }
   // transpose of gref: reference gradient gr -> physical gradient * JxW (P = Mq^T gr)
-  __device__ void gref_t(const Geo& g, const TabT<R>& T, int qz, const R* gr, R* P) const {
+  __device__ __attribute__((always_inline)) void gref_t(const Geo& g, const TabT<R>& T, int qz, const R* gr, R* P) const {
     const R zeta = zeta0 + T.QP[qz] * hz;
     const R fz = R(1) - zeta * iH;
     const R w = wxy * T.W[qz];

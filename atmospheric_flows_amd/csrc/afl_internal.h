@@ -40,6 +40,22 @@ struct KernelTimer {
 enum KernelId { kTimeLevelW = 1, kTimeLevelCCheb = 2, kTimeRhsRho = 4, kTimeRhsU = 8, kTimeRhsP = 16,
                 kTimeRhsPUfix = 32, kTimeSchurW = 64, kTimeApplyC = 128, kTimeMassSolve = 256, kTimeApplyB = 512 };
 
+// Positional sources of an explicit-stage apply (kernel argument by value).
+struct Srcs { const double* p[12]; };
+
+// src.p[i] for a runtime i without indexing the kernel argument through memory
+__device__ __attribute__((always_inline)) inline const double* src_at(const Srcs& s, int i) {
+  const double* v = s.p[0];
+#pragma unroll
+  for (int j = 1; j < 12; ++j) v = (i == j) ? s.p[j] : v;
+  return v;
+}
+
+// Stage weights of E2: advective w[j], pressure wp[j] per explicit state, gravity gw[j] per density.
+struct E2Coef { double w[3], wp[3], gw[3]; };
+// Stage weights of E3: kinetic w[j], enthalpy wt[j] per explicit state, gravity work gw[j].
+struct E3Coef { double w[3], wt[3], gw[3]; };
+
 // Everything a launcher needs; a plain copy of the operator state.
 struct OpCtx {
   int dim = 2, k = 1;
@@ -71,6 +87,22 @@ void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* d
 void launch_rhs_p_ufix(const OpCtx& o, int stage, const double* rho_new, const double* u_fix, const double* base,
                        double* dst);
 void launch_mass_apply(const OpCtx& o, const double* rho, double scale, int ncomp, const double* src, double* dst);
+// 3D, k <= 3: wave-per-cell kernels (afl_ops3d.hip); `by` is the launch's algorithmic bytes
+void launch3_mass_solve(const OpCtx& o, double by, const double* rho, double scale, int ncomp, const double* b,
+                        double* x);
+void launch3_apply_B(const OpCtx& o, double by, double f, const double* p, double* dst);
+void launch3_schur_w(const OpCtx& o, double by, double f, const double* rho, const double* p, double* w);
+void launch3_apply_C(const OpCtx& o, double by, double cdt, const double* pfix, const double* w, const double* pD,
+                     double* dst);
+void launch3_rhs_rho(const OpCtx& o, double by, int stage, const Srcs& S, double w0, double w1, double w2,
+                     double* dst);
+void launch3_rhs_u(const OpCtx& o, double by, int stage, const Srcs& S, const E2Coef& cf, double iMa2, double iFr2,
+                   double* dst);
+void launch3_rhs_p(const OpCtx& o, double by, int stage, bool full, const Srcs& S, const E3Coef& cf, double Ma2,
+                   double iFr2, double* dst);
+void launch3_rhs_p_ufix(const OpCtx& o, double by, int stage, const double* rho_new, const double* u_fix,
+                        double half_ma2, double cg, const double* base, double* dst);
+inline bool use_w3(const OpCtx& o) { return o.dim == 3 && o.k <= 3; }
 void launch_inv_diag(const OpCtx& o, const double* rho, double scale, int ncomp, double* dst);
 void launch_apply_B(const OpCtx& o, const double* p, double* dst);
 void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const double* pD, double* dst);

@@ -23,15 +23,6 @@
 
 namespace afl {
 
-struct Srcs { const double* p[12]; };
-
-// src.p[i] for a runtime i without indexing the kernel argument through memory
-__device__ __attribute__((always_inline)) inline const double* src_at(const Srcs& s, int i) {
-  const double* v = s.p[0];
-#pragma unroll
-  for (int j = 1; j < 12; ++j) v = (i == j) ? s.p[j] : v;
-  return v;
-}
 
 template <int DIM, int K, int Q = 2 * K + 1>
 struct Lds {
@@ -148,7 +139,6 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 // E2: vmult_rhs_velocity_fixed (euler_operator.h:1620-1886 cell, 1889-2184 faces, 2187-2305 walls).
 // Explicit states j < NE: (rho, u, p) = src[3j..3j+2], advective weight w[j], pressure weight wp[j].
 // Gravity densities: src[3j] with weight gw[j], j < NG (rho_old, rho_tmp_2, rho_tmp_3 / rho_curr).
-struct E2Coef { double w[3], wp[3], gw[3]; };
 
 template <int DIM, int K, int NE, int NG>
 __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? AFL_E2_WAVES : 1))) void k_rhs_u(Geo g, const Tab* __restrict__ Tp, Srcs src, E2Coef cf, double dt, double iMa2,
@@ -269,7 +259,6 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
 // FULL = false (implicit stages only) leaves out the two terms that read u_fixed, the kinetic
 // correction -1/2 Ma^2 rho_new |u_fix|^2 and the gravity work of (rho_new, u_fix): the rest does not
 // change across the fixed-point iterations of a stage, and k_rhs_p_ufix adds those two terms.
-struct E3Coef { double w[3], wt[3], gw[3]; };
 
 template <int DIM, int K, int NE, int NG, int STAGE, bool FULL = true>
 __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 2 : 1))) void k_rhs_p(Geo g, const Tab* __restrict__ Tp, Srcs src, E3Coef cf, double dt, double Ma2,
@@ -1403,6 +1392,13 @@ void launch_rhs_rho(const OpCtx& o, int stage, const double* const* src, double*
   const Srcs S = make_srcs(src, 2 * stage);
   const TableauC& t = o.tab;
   const double by = 8.0 * (double)o.N * (stage * (1 + o.dim) + 1);  // s (rho_j, u_j) in, rhs out
+  if (use_w3(o)) {
+    const double w0 = stage == 1 ? t.a21 : stage == 2 ? t.a31 : t.b1;
+    const double w1 = stage == 1 ? 0.0 : stage == 2 ? t.a32 : t.b2;
+    const double w2 = stage == 3 ? t.b3 : 0.0;
+    launch3_rhs_rho(o, by, stage, S, w0, w1, w2, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (stage == 1)
@@ -1432,6 +1428,10 @@ void launch_rhs_u(const OpCtx& o, int stage, const double* const* src, double* d
   // sources: (rho, u, p) per explicit state plus rho_new at stages 1-2; rhs (dim fields) out
   const int d = o.dim;
   const double by = 8.0 * (double)o.N * (stage == 1 ? 3 + d + d : stage == 2 ? 2 * d + 5 + d : 3 * (2 + d) + d);
+  if (use_w3(o)) {
+    launch3_rhs_u(o, by, stage, S, cf, iMa2, iFr2, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (stage == 1)
@@ -1464,6 +1464,10 @@ void launch_rhs_p(const OpCtx& o, int stage, const double* const* src, double* d
   const int d = o.dim;
   const int nin = stage == 1 ? (full ? 2 * d + 3 : d + 3) : stage == 2 ? (full ? 3 * d + 5 : 2 * d + 5) : 4 * d + 7;
   const double by = 8.0 * (double)o.N * (nin + 1);
+  if (use_w3(o)) {
+    launch3_rhs_p(o, by, stage, full, S, cf, Ma2, iFr2, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (stage == 1 && full)
@@ -1487,6 +1491,10 @@ void launch_rhs_p_ufix(const OpCtx& o, int stage, const double* rho_new, const d
   const double Ma2 = o.Ma * o.Ma, iFr2 = 1.0 / (o.Fr * o.Fr);
   const double cg = (stage == 1 ? t.at22 : t.at33) * o.dt * Ma2 * iFr2;
   const double by = 8.0 * (double)o.N * (o.dim + 3);  // rho_new, u_fix, base in; rhs out
+  if (use_w3(o)) {
+    launch3_rhs_p_ufix(o, by, stage, rho_new, u_fix, 0.5 * Ma2, cg, base, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     launch_timed(o, kTimeRhsPUfix, stage, by, k_rhs_p_ufix<D_, K_>, grid_for<D_, K_>(o.geo.ncells),
                  dim3(Cfg<D_, K_>::NT), o.geo, o.dT, rho_new, u_fix, 0.5 * Ma2, cg, o.N, base, dst);
@@ -1509,6 +1517,10 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst) {
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   const double by = 8.0 * (double)o.N * (1 + o.dim);
+  if (use_w3(o)) {
+    launch3_apply_B(o, by, f, p, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     launch_timed(o, kTimeApplyB, (int)o.hyp, by, k_apply_B<D_, K_>, grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT),
                  o.geo, o.dT, f, o.N, p, dst);
@@ -1519,6 +1531,10 @@ void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const d
   const double coeff = o.hyp == 1 ? o.tab.at22 : o.tab.at33;
   const double cdt = coeff * o.dt;
   const double by = 8.0 * (double)o.N * (o.dim + (pD ? 3 : 2));  // pfix, w [, pD] in; out
+  if (use_w3(o)) {
+    launch3_apply_C(o, by, cdt, pfix, w, pD, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     const dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (pD)
@@ -1533,6 +1549,10 @@ void launch_apply_C(const OpCtx& o, const double* pfix, const double* w, const d
 void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncomp, const double* b, double* x) {
   const double tol2 = o.cell_tol;
   const double by = 8.0 * (double)o.N * (2 * ncomp + (rho ? 1 : 0));
+  if (use_w3(o)) {
+    launch3_mass_solve(o, by, rho, scale, ncomp, b, x);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     const dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (ncomp == 1)
@@ -1549,6 +1569,10 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   const double f = coeff * o.dt / (o.Ma * o.Ma);
   const double tol2 = o.cell_tol;
   const double by = 8.0 * (double)o.N * (2 + o.dim);  // p, rho in; w out
+  if (use_w3(o)) {
+    launch3_schur_w(o, by, f, rho, p, w);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     launch_timed(o, kTimeSchurW, (int)o.hyp, by, k_schur_w<D_, K_, double>, grid_for<D_, K_>(o.geo.ncells),
                  dim3(Cfg<D_, K_>::NT), o.geo, o.dT, rho, f, o.N, p, w, tol2, o.cell_maxit);
