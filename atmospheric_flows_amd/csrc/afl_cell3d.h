@@ -133,6 +133,32 @@ struct Cell {
   }
 };
 
+// Outward normals n[side] and JxW of the two faces normal to direction dir at the lane's face point
+// (lane < Q^2), the values face_geo gives. The x and y faces have constant normals; the z faces lie
+// over the lane's own vertical line, so their hill slope is the line's (no second hill evaluation).
+template <int K>
+__device__ __attribute__((always_inline)) inline void dir_geo(const Geo& g, const Tab& T, const int* ic,
+                                                               const LineGeo<3, K>& lg, int dir, int lane,
+                                                               double (&n)[2][3], double (&jxw)[2]) {
+  if (dir < 2) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) face_geo<3, K>(g, T, ic, 2 * dir + s, lane, n[s], jxw[s]);
+  } else {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const double zeta = (ic[2] + s) * g.hc[2];
+      const double fz = 1.0 - zeta * lg.iH;
+      const double vx = -lg.dhx * fz, vy = -lg.dhy * fz;
+      const double nv = sqrt(1.0 + vx * vx + vy * vy);
+      const double sg = s ? 1.0 : -1.0;
+      n[s][0] = sg * vx / nv;
+      n[s][1] = sg * vy / nv;
+      n[s][2] = sg / nv;
+      jxw[s] = lg.hx * lg.hy * nv * lg.wxy;
+    }
+  }
+}
+
 // Values at the quadrature points of the lane's vertical line of NF fields staged in LDS
 // (field a at nod + a * fs). wk: NF * Sz::EV values.
 template <int K, int Q, int NF>

@@ -10,7 +10,7 @@
 namespace afl {
 namespace w3 {
 
-constexpr int kWPB = 4;  // wavefronts (cells) per block
+constexpr int kWPB = 2;  // wavefronts (cells) per block
 
 // Common prologue: block tables, the wave's cell (inactive waves leave after the table barrier).
 #define W3_PROLOGUE(PER)                                   \
@@ -118,15 +118,16 @@ __device__ __attribute__((always_inline)) inline void apply_B_nodes(const Cell<K
     double trp[1][2][2];
     traces<K, Q, 1>(c, tb, dir, nod, Z::NPC, nbf, Z::NF2, wk, trp);
     double Fa[3][2] = {};
-    if (c.line)
+    if (c.line) {
+      double n[2][3], jxw[2];
+      dir_geo<K>(g, T, c.ic, lg, dir, c.lane, n, jxw);
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
-        double n[3], jxw;
-        face_geo<3, K>(g, T, c.ic, 2 * dir + side, c.lane, n, jxw);
         const double avg = 0.5 * (trp[0][side][0] + trp[0][side][1]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) Fa[a][side] = f * avg * n[a] * jxw;
+        for (int a = 0; a < 3; ++a) Fa[a][side] = f * avg * n[side][a] * jxw[side];
       }
+    }
     face_integrate<K, Q, 3>(c, tb, dir, wk, Fa, fadd);
   }
   double pq[1][Q];
@@ -235,13 +236,15 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     double tr[4][2][2];
     traces<K, Q, 4>(c, tb, dir, nod, NPC, nbf, NF2, wk, tr);
     double Fo[1][2] = {};
-    if (c.line)
+    if (c.line) {
+      double fn[2][3], fj[2];
+      dir_geo<K>(g, T, c.ic, lg, dir, c.lane, fn, fj);
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
         const int fc = 2 * dir + side;
         if (c.is_wall(fc)) continue;  // no boundary term (euler_operator.h:188-192)
-        double n[3], jxw;
-        face_geo<3, K>(g, T, c.ic, fc, c.lane, n, jxw);
+        const double* n = fn[side];
+        const double jxw = fj[side];
         const double pp = tr[0][side][0], pm = tr[0][side][1];
         double avg = 0.0, wnp = 0.0, wnm = 0.0;
 #pragma unroll
@@ -253,6 +256,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
         const double lam = fmax(fabs(wnp), fabs(wnm));
         Fo[0][side] = sgn * cdt * (avg + 0.5 * lam * igm1 * (pp - pm)) * jxw;
       }
+    }
     face_integrate<K, Q, 1>(c, tb, dir, wk, Fo, fadd);
   }
   double v[4][Q];
@@ -281,18 +285,18 @@ template <int K, int NS>
 __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) void k3_rhs_rho(
     Geo g, const Tab* __restrict__ Tp, Srcs src, double dt, double w0, double w1, double w2, int64_t N,
     double* __restrict__ dst) {
-  constexpr int PER_ = 4 * NS * Sz<K, 2 * K + 1>::NPC + 4 * Sz<K, 2 * K + 1>::NF2 +
+  constexpr int PER_ = 4 * NS * Sz<K, 2 * K + 1>::NPC + 4 * NS * Sz<K, 2 * K + 1>::NF2 +
                        Max4<Sz<K, 2 * K + 1>::ING, 4 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
                             Sz<K, 2 * K + 1>::FI>::v;
   W3_PROLOGUE(PER_)
   double* nod = wbase;                 // [4 NS][NPC]: rho_j, u_j (3) per state
-  double* nbf = nod + 4 * NS * NPC;    // [4][NF2] of the state and direction being processed
-  double* wk = nbf + 4 * NF2;
+  double* nbf = nod + 4 * NS * NPC;    // [4 NS][NF2]: every state's neighbour face nodes of one direction
+  double* wk = nbf + 4 * NS * NF2;
   const double wj[3] = {w0, w1, w2};
   auto fld = [&](int j, int m) -> const double* {
     return m == 0 ? src.p[2 * j] : src.p[2 * j + 1] + (m - 1) * N;
   };
-  double nv[4];
+  double nv[4 * NS];  // neighbour face nodes of the next direction, in flight during this one
   {
     double ov[4 * NS];
 #pragma unroll
@@ -300,7 +304,9 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
 #pragma unroll
       for (int m = 0; m < 4; ++m) ov[4 * j + m] = c.own(fld(j, m));
 #pragma unroll
-    for (int m = 0; m < 4; ++m) nv[m] = c.nbr(fld(0, m), 0);
+    for (int j = 0; j < NS; ++j)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) nv[4 * j + m] = c.nbr(fld(j, m), 0);
     if (c.lane < NPC)
 #pragma unroll
       for (int a = 0; a < 4 * NS; ++a) nod[a * NPC + c.lane] = ov[a];
@@ -312,26 +318,25 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
   for (int dir = 0; dir < 3; ++dir) {
     double F[2] = {0.0, 0.0};
 #pragma unroll
+    for (int a = 0; a < 4 * NS; ++a) c.put_nbr(nbf + a * NF2, nv[a]);
+    if (dir < 2)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) nv[4 * j + m] = c.nbr(fld(j, m), dir + 1);
+    wsync();
+    double fn[2][3], fj[2];
+    if (c.line) dir_geo<K>(g, T, c.ic, lg, dir, c.lane, fn, fj);
+#pragma unroll
     for (int j = 0; j < NS; ++j) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) c.put_nbr(nbf + m * NF2, nv[m]);
-      if (j + 1 < NS) {
-#pragma unroll
-        for (int m = 0; m < 4; ++m) nv[m] = c.nbr(fld(j + 1 < NS ? j + 1 : j, m), dir);
-      } else if (dir < 2) {
-#pragma unroll
-        for (int m = 0; m < 4; ++m) nv[m] = c.nbr(fld(0, m), dir + 1);
-      }
-      wsync();
       double tr[4][2][2];
-      traces<K, Q, 4>(c, tb, dir, nod + 4 * j * NPC, NPC, nbf, NF2, wk, tr);
+      traces<K, Q, 4>(c, tb, dir, nod + 4 * j * NPC, NPC, nbf + 4 * j * NF2, NF2, wk, tr);
       if (c.line)
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
           const int fc = 2 * dir + side;
           if (c.is_wall(fc)) continue;
-          double n[3], jxw;
-          face_geo<3, K>(g, T, c.ic, fc, c.lane, n, jxw);
+          const double* n = fn[side];
           const double rp = tr[0][side][0], rm = tr[0][side][1];
           double avg = 0.0, unp = 0.0, unm = 0.0;
 #pragma unroll
@@ -347,11 +352,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     double Fo[1][2] = {};
     if (c.line)
 #pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        double n[3], jxw;
-        face_geo<3, K>(g, T, c.ic, 2 * dir + side, c.lane, n, jxw);
-        Fo[0][side] = -F[side] * jxw;
-      }
+      for (int side = 0; side < 2; ++side) Fo[0][side] = -F[side] * fj[side];
     face_integrate<K, Q, 1>(c, tb, dir, wk, Fo, fadd);
   }
   double V[1][Q] = {}, Gp[3][Q] = {};
@@ -390,57 +391,53 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     double* __restrict__ dst) {
   constexpr bool XG = NG > NE;
   constexpr int NF = 5 * NE + (XG ? 1 : 0);
-  constexpr int PER_ = NF * Sz<K, 2 * K + 1>::NPC + 5 * Sz<K, 2 * K + 1>::NF2 +
+  constexpr int PER_ = NF * Sz<K, 2 * K + 1>::NPC + 5 * NE * Sz<K, 2 * K + 1>::NF2 +
                        Max4<Sz<K, 2 * K + 1>::ING, 5 * Sz<K, 2 * K + 1>::TR, 3 * Sz<K, 2 * K + 1>::EV,
                             3 * Sz<K, 2 * K + 1>::FI>::v;
   W3_PROLOGUE(PER_)
   double* nod = wbase;               // [NF][NPC]: (rho, u0, u1, u2, p) per state [, rho_new]
-  double* nbf = nod + NF * NPC;      // [5][NF2] of the state and direction being processed
-  double* wk = nbf + 5 * NF2;
+  double* nbf = nod + NF * NPC;      // [5 NE][NF2]: every state's neighbour face nodes of one direction
+  double* wk = nbf + 5 * NE * NF2;
   auto fld = [&](int a) -> const double* {
     const int j = a / 5, m = a % 5;
     if (a == 5 * NE) return src.p[3 * NE];
     return m == 0 ? src.p[3 * j] : m == 4 ? src.p[3 * j + 2] : src.p[3 * j + 1] + (m - 1) * N;
   };
-  double nv[5];
+  double nv[5 * NE];  // neighbour face nodes of the next direction, in flight during this one
   {
     double ov[NF];
 #pragma unroll
     for (int a = 0; a < NF; ++a) ov[a] = c.own(fld(a));
 #pragma unroll
-    for (int m = 0; m < 5; ++m) nv[m] = c.nbr(fld(m), 0);
+    for (int a = 0; a < 5 * NE; ++a) nv[a] = c.nbr(fld(a), 0);
     if (c.lane < NPC)
 #pragma unroll
       for (int a = 0; a < NF; ++a) nod[a * NPC + c.lane] = ov[a];
   }
   LineGeo<3, K> lg;
   lg.init(g, T, c.ic, c.lane);
-  // faces first, one direction at a time: flux per face of the direction and momentum component
   double fadd[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int dir = 0; dir < 3; ++dir) {
     double F[3][2] = {};
 #pragma unroll
+    for (int a = 0; a < 5 * NE; ++a) c.put_nbr(nbf + a * NF2, nv[a]);
+    if (dir < 2)
+#pragma unroll
+      for (int a = 0; a < 5 * NE; ++a) nv[a] = c.nbr(fld(a), dir + 1);
+    wsync();
+    double fn[2][3], fj[2];
+    if (c.line) dir_geo<K>(g, T, c.ic, lg, dir, c.lane, fn, fj);
+#pragma unroll
     for (int j = 0; j < NE; ++j) {
       const double wj = cf.w[j], wpj = cf.wp[j];
-#pragma unroll
-      for (int m = 0; m < 5; ++m) c.put_nbr(nbf + m * NF2, nv[m]);
-      if (j + 1 < NE) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) nv[m] = c.nbr(fld(5 * (j + 1 < NE ? j + 1 : j) + m), dir);
-      } else if (dir < 2) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) nv[m] = c.nbr(fld(m), dir + 1);
-      }
-      wsync();
       double tr[5][2][2];
-      traces<K, Q, 5>(c, tb, dir, nod + 5 * j * NPC, NPC, nbf, NF2, wk, tr);
+      traces<K, Q, 5>(c, tb, dir, nod + 5 * j * NPC, NPC, nbf + 5 * j * NF2, NF2, wk, tr);
       if (c.line)
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
           const int fc = 2 * dir + side;
-          double n[3], jxw;
-          face_geo<3, K>(g, T, c.ic, fc, c.lane, n, jxw);
+          const double* n = fn[side];
           const double pp = tr[4][side][0];
           if (!c.is_wall(fc)) {
             const double rp = tr[0][side][0], rm = tr[0][side][1], pm = tr[4][side][1];
@@ -465,12 +462,9 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     }
     if (c.line)
 #pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        double n[3], jxw;
-        face_geo<3, K>(g, T, c.ic, 2 * dir + side, c.lane, n, jxw);
+      for (int side = 0; side < 2; ++side)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) F[a][side] *= -jxw;
-      }
+        for (int a = 0; a < 3; ++a) F[a][side] *= -fj[side];
     face_integrate<K, Q, 3>(c, tb, dir, wk, F, fadd);
   }
   // cells second, state by state: the symmetric flux tensor rho u (x) u (with the pressure term on
@@ -532,32 +526,33 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
 // Explicit states j < NE: (rho, u, p) = src[3j..]; kinetic correction with (src[3NE], src[3NE+1]);
 // gravity work with (src[3j], src[3j+1]), j < NG (j = NE: the correction pair). FULL = false leaves
 // out the two terms of stages 1-2 that read u_fixed (added by k_rhs_p_ufix).
+// (the stage-2 form with every term holds the u_old traces as well: one wave per SIMD, no spills)
 template <int K, int NE, int NG, int STAGE, bool FULL>
-__global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) void k3_rhs_p(
+__global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(STAGE == 2 && FULL ? 1 : 2))) void k3_rhs_p(
     Geo g, const Tab* __restrict__ Tp, Srcs src, E3Coef cf, double dt, double Ma2, double iFr2, int64_t N,
     double* __restrict__ dst) {
   constexpr bool KC = FULL || STAGE == 3;  // kinetic correction (and the extra gravity pair) present
   constexpr int NF = 5 * NE + (KC ? 4 : 0);  // staged states [, correction pair rho, u]
-  constexpr int PER_ = NF * Sz<K, 2 * K + 1>::NPC + 5 * Sz<K, 2 * K + 1>::NF2 +
-                       Max4<Sz<K, 2 * K + 1>::ING, 5 * Sz<K, 2 * K + 1>::TR, 3 * Sz<K, 2 * K + 1>::EV,
+  constexpr int PER_ = NF * Sz<K, 2 * K + 1>::NPC + 5 * NE * Sz<K, 2 * K + 1>::NF2 +
+                       Max4<Sz<K, 2 * K + 1>::ING, 5 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
                             Sz<K, 2 * K + 1>::FI>::v;
   W3_PROLOGUE(PER_)
   constexpr double gm1 = kGamma - 1.0;
   double* nod = wbase;                 // [NF][NPC]
-  double* nbf = nod + NF * NPC;        // [5][NF2] of the state and direction being processed
-  double* wk = nbf + 5 * NF2;
+  double* nbf = nod + NF * NPC;        // [5 NE][NF2]: every state's neighbour face nodes of one direction
+  double* wk = nbf + 5 * NE * NF2;
   auto fld = [&](int a) -> const double* {
     if (a >= 5 * NE) return a == 5 * NE ? src.p[3 * NE] : src.p[3 * NE + 1] + (a - 5 * NE - 1) * N;
     const int j = a / 5, m = a % 5;
     return m == 0 ? src.p[3 * j] : m == 4 ? src.p[3 * j + 2] : src.p[3 * j + 1] + (m - 1) * N;
   };
-  double nv[5];
+  double nv[5 * NE];  // neighbour face nodes of the next direction, in flight during this one
   {
     double ov[NF];
 #pragma unroll
     for (int a = 0; a < NF; ++a) ov[a] = c.own(fld(a));
 #pragma unroll
-    for (int m = 0; m < 5; ++m) nv[m] = c.nbr(fld(m), 0);
+    for (int a = 0; a < 5 * NE; ++a) nv[a] = c.nbr(fld(a), 0);
     if (c.lane < NPC)
 #pragma unroll
       for (int a = 0; a < NF; ++a) nod[a * NPC + c.lane] = ov[a];
@@ -570,20 +565,18 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     double F[2] = {0.0, 0.0};
     double tu0[3][2][2];  // u_old traces (stage-2 kinetic jump as written, euler_operator.h:1182-1183)
 #pragma unroll
+    for (int a = 0; a < 5 * NE; ++a) c.put_nbr(nbf + a * NF2, nv[a]);
+    if (dir < 2)
+#pragma unroll
+      for (int a = 0; a < 5 * NE; ++a) nv[a] = c.nbr(fld(a), dir + 1);
+    wsync();
+    double fn[2][3], fj[2];
+    if (c.line) dir_geo<K>(g, T, c.ic, lg, dir, c.lane, fn, fj);
+#pragma unroll
     for (int j = 0; j < NE; ++j) {
       const double wj = cf.w[j], wpj = cf.wt[j];
-#pragma unroll
-      for (int m = 0; m < 5; ++m) c.put_nbr(nbf + m * NF2, nv[m]);
-      if (j + 1 < NE) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) nv[m] = c.nbr(fld(5 * (j + 1 < NE ? j + 1 : j) + m), dir);
-      } else if (dir < 2) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) nv[m] = c.nbr(fld(m), dir + 1);
-      }
-      wsync();
       double tr[5][2][2];
-      traces<K, Q, 5>(c, tb, dir, nod + 5 * j * NPC, NPC, nbf, NF2, wk, tr);
+      traces<K, Q, 5>(c, tb, dir, nod + 5 * j * NPC, NPC, nbf + 5 * j * NF2, NF2, wk, tr);
       if (STAGE == 2 && j == 0)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -597,8 +590,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
         for (int side = 0; side < 2; ++side) {
           const int fc = 2 * dir + side;
           if (c.is_wall(fc)) continue;
-          double n[3], jxw;
-          face_geo<3, K>(g, T, c.ic, fc, c.lane, n, jxw);
+          const double* n = fn[side];
           const double rp = tr[0][side][0], rm = tr[0][side][1], pp = tr[4][side][0], pm = tr[4][side][1];
           double uup = 0.0, uum = 0.0, unp = 0.0, unm = 0.0;
 #pragma unroll
@@ -641,11 +633,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     double Fo[1][2] = {};
     if (c.line)
 #pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        double n[3], jxw;
-        face_geo<3, K>(g, T, c.ic, 2 * dir + side, c.lane, n, jxw);
-        Fo[0][side] = -F[side] * jxw;
-      }
+      for (int side = 0; side < 2; ++side) Fo[0][side] = -F[side] * fj[side];
     face_integrate<K, Q, 1>(c, tb, dir, wk, Fo, fadd);
   }
   double V[1][Q] = {}, Gp[3][Q] = {};
@@ -654,13 +642,14 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     const double wj = cf.w[j], wpj = cf.wt[j];
     double zl[5][K + 1];
     {
-      double z3[3][K + 1], z2[2][K + 1];
-      eval_lines<K, Q, 3>(c, nod + 5 * j * NPC, NPC, wk, z3);        // rho, u0, u1
-      eval_lines<K, Q, 2>(c, nod + (5 * j + 3) * NPC, NPC, wk, z2);  // u2, p
+      double za[2][K + 1], zb[2][K + 1], zc[1][K + 1];
+      eval_lines<K, Q, 2>(c, nod + 5 * j * NPC, NPC, wk, za);        // rho, u0
+      eval_lines<K, Q, 2>(c, nod + (5 * j + 2) * NPC, NPC, wk, zb);  // u1, u2
+      eval_lines<K, Q, 1>(c, nod + (5 * j + 4) * NPC, NPC, wk, zc);  // p
 #pragma unroll
       for (int i = 0; i <= K; ++i) {
-        zl[0][i] = z3[0][i]; zl[1][i] = z3[1][i]; zl[2][i] = z3[2][i];
-        zl[3][i] = z2[0][i]; zl[4][i] = z2[1][i];
+        zl[0][i] = za[0][i]; zl[1][i] = za[1][i]; zl[2][i] = zb[0][i];
+        zl[3][i] = zb[1][i]; zl[4][i] = zc[0][i];
       }
     }
 #pragma unroll
