@@ -461,6 +461,18 @@ inline double dotv(const double* a, const double* b, int dim) {
   return s;
 }
 
+// Planted faults, for the oracle's own tests only (orc_set_fault; 0 everywhere else). They show that
+// the closed-form pins of the Rusanov terms and the sponge fail when a term is wrong:
+//   1: the jump weighted lambda instead of lambda / 2     2: the jump's sign flipped
+//   3: lambda = min instead of max of |u.n|               4: the sponge explicit, u + dt a (u_ref - u)
+int g_fault = 0;
+
+// Rusanov speed max(|a|, |b|) (euler_operator.h:521-522) and the weight 1/2 of its jump term.
+inline double rus_speed(double a, double b) {
+  return g_fault == 3 ? std::min(std::fabs(a), std::fabs(b)) : std::max(std::fabs(a), std::fabs(b));
+}
+inline double rus_half() { return g_fault == 1 ? 1.0 : (g_fault == 2 ? -0.5 : 0.5); }
+
 // gather the dim components of a vector field at cell quadrature points: out[d*nqc+q]
 void eval_cell_vec(const Op& op, const double* f, int c, double* out, Scratch& s) {
   for (int d = 0; d < op.dim; ++d) op.eval_cell(f + (int64_t)d * op.N, c, out + (size_t)d * op.nqc, s);
@@ -509,8 +521,8 @@ int assemble_rhs_rho(Op& op, double* dst, const double* const* src, int n_src) {
         for (int d = 0; d < dim; ++d) { upv[d] = up[(size_t)d * nfq + q]; umv[d] = um[(size_t)d * nfq + q]; }
         double avg_n = 0.0;
         for (int d = 0; d < dim; ++d) avg_n += 0.5 * (rp[q] * upv[d] + rm[q] * umv[d]) * n[d];
-        const double lam = std::max(std::fabs(dotv(upv, n, dim)), std::fabs(dotv(umv, n, dim)));
-        const double F = w[j] * dt * (avg_n + 0.5 * lam * (rp[q] - rm[q]));
+        const double lam = rus_speed(dotv(upv, n, dim), dotv(umv, n, dim));
+        const double F = w[j] * dt * (avg_n + rus_half() * lam * (rp[q] - rm[q]));
         Fp[q] -= F; Fm[q] += F;
       }
     }
@@ -585,12 +597,12 @@ int assemble_rhs_u(Op& op, double* dst, const double* const* src, int n_src) {
         double upv[3], umv[3];
         for (int d = 0; d < dim; ++d) { upv[d] = up[(size_t)d * nfq + q]; umv[d] = um[(size_t)d * nfq + q]; }
         const double unp = dotv(upv, n, dim), unm = dotv(umv, n, dim);
-        const double lam = std::max(std::fabs(unp), std::fabs(unm));
+        const double lam = rus_speed(unp, unm);
         const double avg_p = 0.5 * (pp[q] + pm[q]);
         for (int a0 = 0; a0 < dim; ++a0) {
           const double avg_tn = 0.5 * (rp[q] * upv[a0] * unp + rm[q] * umv[a0] * unm);
           const double jump = rp[q] * upv[a0] - rm[q] * umv[a0];
-          const double F = w[j] * dt * avg_tn + wp[j] * dt / Ma2 * avg_p * n[a0] + w[j] * dt * 0.5 * lam * jump;
+          const double F = w[j] * dt * avg_tn + wp[j] * dt / Ma2 * avg_p * n[a0] + w[j] * dt * rus_half() * lam * jump;
           Fp[(size_t)a0 * nfq + q] -= F; Fm[(size_t)a0 * nfq + q] += F;
         }
       }
@@ -706,12 +718,12 @@ int assemble_rhs_p(Op& op, double* dst, const double* const* src, int n_src) {
           avg_ent_n += 0.5 * (((Ep - 0.5 * Ma2 * uup) * rp[q] + pp[q]) * upv[d] +
                               ((Em - 0.5 * Ma2 * uum) * rm[q] + pm[q]) * umv[d]) * n[d];
         }
-        const double lam = std::max(std::fabs(dotv(upv, n, dim)), std::fabs(dotv(umv, n, dim)));
+        const double lam = rus_speed(dotv(upv, n, dim), dotv(umv, n, dim));
         double F;
         if (st == 3) {
           // euler_operator.h:1325-1352: jump of rho E
           const double jump_rhoE = rp[q] * Ep - rm[q] * Em;
-          F = w[j] * dt * Ma2 * avg_kin_n + w[j] * dt * avg_ent_n + w[j] * dt * 0.5 * lam * jump_rhoE;
+          F = w[j] * dt * Ma2 * avg_kin_n + w[j] * dt * avg_ent_n + w[j] * dt * rus_half() * lam * jump_rhoE;
         } else {
           double jump_kin;
           if (st == 2 && j == 1)  // euler_operator.h:1182-1183: u_tmp_2 . u_old as written
@@ -719,7 +731,8 @@ int assemble_rhs_p(Op& op, double* dst, const double* const* src, int n_src) {
           else
             jump_kin = rp[q] * 0.5 * uup - rm[q] * 0.5 * uum;
           const double jump_rhoe = rp[q] * (Ep - 0.5 * Ma2 * uup) - rm[q] * (Em - 0.5 * Ma2 * uum);
-          F = w[j] * dt * Ma2 * (avg_kin_n + 0.5 * lam * jump_kin) + wt[j] * dt * (avg_ent_n + 0.5 * lam * jump_rhoe);
+          F = w[j] * dt * Ma2 * (avg_kin_n + rus_half() * lam * jump_kin) +
+              wt[j] * dt * (avg_ent_n + rus_half() * lam * jump_rhoe);
         }
         Fp[q] -= F; Fm[q] += F;
       }
@@ -811,9 +824,9 @@ void apply_C(const Op& op, double* dst, const double* src) {
       double wpv[3], wmv[3], avg_n = 0.0;
       for (int d = 0; d < dim; ++d) { wpv[d] = wp[(size_t)d * nfq + q]; wmv[d] = wm[(size_t)d * nfq + q]; }
       for (int d = 0; d < dim; ++d) avg_n += 0.5 * gr * (pp[q] * wpv[d] + pm[q] * wmv[d]) * n[d];
-      const double lam = std::max(std::fabs(dotv(wpv, n, dim)), std::fabs(dotv(wmv, n, dim)));
+      const double lam = rus_speed(dotv(wpv, n, dim), dotv(wmv, n, dim));
       const double jump = 1.0 / gm1 * (pp[q] - pm[q]);
-      const double F = cdt * (avg_n + 0.5 * lam * jump);
+      const double F = cdt * (avg_n + rus_half() * lam * jump);
       Fp[q] = F; Fm[q] = -F;
     }
     op.integrate_face(Fp.data(), s.fJxW.data(), cm, dir, 1, dst);
@@ -1640,6 +1653,11 @@ int orc_set_u_fixed(orc_operator* op, const double* src) {
 }
 int orc_set_solver_params(orc_operator* op, const afl_solver_params* sp) { op->sp = *sp; return 0; }
 int orc_set_sponge(orc_operator* op, const afl_sponge* s) { op->sponge = *s; return 0; }
+int orc_set_fault(int fault) {
+  if (fault < 0 || fault > 4) return 1;
+  g_fault = fault;
+  return 0;
+}
 
 int64_t orc_n_dofs(const orc_operator* op, int idx) { return idx == 0 ? op->N * op->dim : op->N; }
 
@@ -1804,7 +1822,10 @@ int orc_imex_step(orc_operator* op, double* rho, double* u, double* p, afl_step_
       if (z <= zs) continue;
       const double s = std::sin(0.5 * M_PI * (z - zs) / (H - zs));
       const double a = op->sponge.alpha_max * s * s * op->dt;
-      for (int d = 0; d < dim; ++d) u[(int64_t)d * N + i] = (u[(int64_t)d * N + i] + a * op->sponge.u_ref[d]) / (1.0 + a);
+      for (int d = 0; d < dim; ++d) {
+        double& ud = u[(int64_t)d * N + i];
+        ud = g_fault == 4 ? ud + a * (op->sponge.u_ref[d] - ud) : (ud + a * op->sponge.u_ref[d]) / (1.0 + a);
+      }
     }
   }
   if (info) *info = inf;

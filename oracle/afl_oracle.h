@@ -39,6 +39,8 @@ int orc_set_pres_fixed(orc_operator* op, const double* src);
 int orc_set_u_fixed(orc_operator* op, const double* src);
 int orc_set_solver_params(orc_operator* op, const afl_solver_params* sp);
 int orc_set_sponge(orc_operator* op, const afl_sponge* sponge);
+/* Planted faults for the oracle's own tests (0 = none; see g_fault in afl_oracle.cpp). */
+int orc_set_fault(int fault);
 
 int64_t orc_n_dofs(const orc_operator* op, int dof_index); /* 0 velocity, 1 pressure, 2 density */
 

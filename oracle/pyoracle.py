@@ -43,6 +43,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, _P]
         L.orc_set_solver_params.argtypes = [C.c_void_p, C.POINTER(SolverParams)]
         L.orc_set_sponge.argtypes = [C.c_void_p, C.POINTER(Sponge)]
+        L.orc_set_fault.argtypes = [C.c_int]
         L.orc_n_dofs.restype = C.c_int64
         L.orc_n_dofs.argtypes = [C.c_void_p, C.c_int]
         for name in ("orc_vmult_rhs_rho_update", "orc_vmult_rhs_velocity_fixed", "orc_vmult_rhs_pressure"):
@@ -71,6 +72,13 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, _P, _P, _P]
         _lib = L
     return _lib
+
+
+def set_fault(fault):
+    """Plants a fault in the oracle (its own tests only; 0 removes it): 1 Rusanov jump weighted
+    lambda instead of lambda/2, 2 jump sign flipped, 3 lambda = min |u.n|, 4 explicit sponge."""
+    if lib().orc_set_fault(int(fault)) != 0:
+        raise ValueError(f"unknown fault {fault}")
 
 
 def _ptr(a):
