@@ -133,7 +133,9 @@ struct Slot {
   int64_t cell;  // global cell index
   bool act;      // slot maps to an existing cell
   int ic[3];     // cell coordinates
-  __device__ Slot(const Geo& g) {
+  __device__ Slot(const Geo& g) : Slot(g, blockIdx.x) {}
+  // the slot in cell block vb (a kernel that strides over cell blocks)
+  __device__ Slot(const Geo& g, int64_t vb) {
     using C = Cfg<DIM, K, Q>;
     if (C::WAVE_LOCAL) {
       const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, lc = lane / C::LT;
@@ -143,7 +145,7 @@ struct Slot {
       cb = threadIdx.x / C::LT;
       l = threadIdx.x % C::LT;
     }
-    cell = (int64_t)blockIdx.x * C::CPB + cb;
+    cell = vb * C::CPB + cb;
     act = (cb < C::CPB) && (cell < g.ncells);
     const int c = act ? (int)cell : 0;
     ic[0] = c % g.nc[0];

@@ -70,33 +70,36 @@ __global__ __launch_bounds__(AFL_NT) void k_diag_finish(const double* __restrict
 }
 
 // Per-block partial sums of sum_q JxW (a - b)^2 and sum_q JxW b^2 over ncomp components
-// (b == nullptr: b = 0). part[b] and part[nblocks + b].
+// (b == nullptr: b = 0): block i of the grid takes the cell blocks i, i + grid, ... < ncb and writes
+// part[i] and part[gridDim.x + i] (every block the same trip count, so the cell barriers match).
 template <int DIM, int K>
-__global__ __launch_bounds__(AFL_NT) void k_l2_partial(Geo g, const Tab* __restrict__ Tp, int ncomp, int64_t N, const double* __restrict__ a,
-                                                       const double* __restrict__ b, int64_t nblocks,
-                                                       double* __restrict__ part) {
+__global__ __launch_bounds__(AFL_NT) void k_l2_partial(Geo g, const Tab* __restrict__ Tp, int ncomp, int64_t N,
+                                                       const double* __restrict__ a, const double* __restrict__ b,
+                                                       int64_t ncb, double* __restrict__ part) {
   const auto& T = *Tp;
   using C = Cfg<DIM, K>;
   constexpr int NQ = C::NQ;
   __shared__ double shw[C::CPB * (C::EV1 + C::EV2)];
-  Slot<DIM, K> s(g);
-  const int cbs = s.cb < C::CPB ? s.cb : 0;
-  double* t1 = shw + cbs * (C::EV1 + C::EV2);
-  double* t2 = t1 + C::EV1;
-  LineGeo<DIM, K> lg;
-  lg.init(g, T, s.ic, s.l);
   double se = 0.0, sn = 0.0;
-  for (int c = 0; c < ncomp; ++c) {
-    double va[NQ], vb[NQ];
-    eval_line<DIM, K>(a + c * N, s, t1, t2, T, va);
-    if (b) eval_line<DIM, K>(b + c * N, s, t1, t2, T, vb);
-    else for (int q = 0; q < NQ; ++q) vb[q] = 0.0;
-    if (s.act) {
+  for (int64_t vb = blockIdx.x; vb < ncb; vb += gridDim.x) {
+    Slot<DIM, K> s(g, vb);
+    const int cbs = s.cb < C::CPB ? s.cb : 0;
+    double* t1 = shw + cbs * (C::EV1 + C::EV2);
+    double* t2 = t1 + C::EV1;
+    LineGeo<DIM, K> lg;
+    lg.init(g, T, s.ic, s.l);
+    for (int c = 0; c < ncomp; ++c) {
+      double va[NQ], vbq[NQ];
+      eval_line<DIM, K>(a + c * N, s, t1, t2, T, va);
+      if (b) eval_line<DIM, K>(b + c * N, s, t1, t2, T, vbq);
+      else for (int q = 0; q < NQ; ++q) vbq[q] = 0.0;
+      if (s.act) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const double w = lg.jxw(g, T, q), d = va[q] - vb[q];
-        se += w * d * d;
-        sn += w * vb[q] * vb[q];
+        for (int q = 0; q < NQ; ++q) {
+          const double w = lg.jxw(g, T, q), d = va[q] - vbq[q];
+          se += w * d * d;
+          sn += w * vbq[q] * vbq[q];
+        }
       }
     }
   }
@@ -104,15 +107,27 @@ __global__ __launch_bounds__(AFL_NT) void k_l2_partial(Geo g, const Tab* __restr
   sn = diag_block_reduce<false>(sn);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = se;
-    part[nblocks + blockIdx.x] = sn;
+    part[gridDim.x + blockIdx.x] = sn;
   }
 }
 
+int64_t l2_blocks(const OpCtx& o, int64_t max_blocks) {
+  int cpb = 1;
+  switch (o.dim * 10 + o.k) {
+#define AFL_L2_CPB(D_, K_) case D_ * 10 + K_: cpb = Cfg<D_, K_>::CPB; break;
+    AFL_L2_CPB(2, 1) AFL_L2_CPB(2, 2) AFL_L2_CPB(2, 3) AFL_L2_CPB(2, 4)
+    AFL_L2_CPB(3, 1) AFL_L2_CPB(3, 2) AFL_L2_CPB(3, 3) AFL_L2_CPB(3, 4)
+#undef AFL_L2_CPB
+    default: break;
+  }
+  const int64_t ncb = (o.geo.ncells + cpb - 1) / cpb;
+  return max_blocks > 0 && ncb > max_blocks ? max_blocks : ncb;
+}
+
+// Work of the diagnostics: 4 x kDiagBlocks nodal-statistics partials, or 2 partial sums per block of
+// the L2 pass (one block per block of Cfg::CPB cells)
 int64_t diag_work_size(const OpCtx& o) {
-  const int n1 = o.k + 1, nq = 2 * o.k + 1;
-  const int lt = o.dim == 2 ? nq : nq * nq;
-  const int64_t nb = (o.geo.ncells + AFL_NT / lt - 1) / (AFL_NT / lt);
-  (void)n1;
+  const int64_t nb = l2_blocks(o, 0);
   return 2 * nb > 4 * kDiagBlocks ? 2 * nb : 4 * kDiagBlocks;
 }
 
@@ -124,13 +139,15 @@ void launch_nodal_stats(const OpCtx& o, const double* rho, const double* u, cons
   k_diag_finish<true><<<dim3(4), dim3(AFL_NT), 0, o.stream>>>(work, nb, kDiagBlocks, out4);
 }
 
-void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const double* b, double* work, double* out2) {
+void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const double* b, double* work, double* out2,
+                          int64_t max_blocks) {
   switch (o.dim * 10 + o.k) {
 #define AFL_L2_CASE(D_, K_)                                                                            \
   case D_ * 10 + K_: {                                                                                 \
     using C = Cfg<D_, K_>;                                                                             \
-    const int64_t nb = (o.geo.ncells + C::CPB - 1) / C::CPB;                                           \
-    k_l2_partial<D_, K_><<<dim3((unsigned)nb), dim3(C::NT), 0, o.stream>>>(o.geo, o.dT, ncomp, o.N, a, b, nb, \
+    const int64_t ncb = (o.geo.ncells + C::CPB - 1) / C::CPB;                                          \
+    const int64_t nb = max_blocks > 0 && ncb > max_blocks ? max_blocks : ncb;                          \
+    k_l2_partial<D_, K_><<<dim3((unsigned)nb), dim3(C::NT), 0, o.stream>>>(o.geo, o.dT, ncomp, o.N, a, b, ncb, \
                                                                             work);                     \
     k_diag_finish<false><<<dim3(2), dim3(AFL_NT), 0, o.stream>>>(work, nb, nb, out2);                  \
   } break;

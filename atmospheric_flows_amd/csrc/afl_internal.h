@@ -161,7 +161,11 @@ void launch_cell_rho_mass_inv(int dim, int k, bool rq, const Geo& geo, const Tab
 int64_t diag_work_size(const OpCtx& o);
 void launch_nodal_stats(const OpCtx& o, const double* rho, const double* u, const double* p, double* work,
                         double* out4);
-void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const double* b, double* work, double* out2);
+// L2 difference sums into out2[0..1]; work holds 2 x (grid blocks) partial sums, the grid capped at
+// max_blocks (> 0; 0: one block per cell block). l2_blocks gives the grid it uses.
+void launch_l2_difference(const OpCtx& o, int ncomp, const double* a, const double* b, double* work, double* out2,
+                          int64_t max_blocks = 0);
+int64_t l2_blocks(const OpCtx& o, int64_t max_blocks);
 
 // vector kernels (afl_blas.hip); reductions leave their result in device memory
 void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);
