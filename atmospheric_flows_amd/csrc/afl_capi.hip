@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -272,6 +273,37 @@ double min_cell_diameter(const afl_mesh_desc* md) {
         best = std::min(best, diam);
       }
   return best;
+}
+
+// ---- device scratch of the allocation-free entry point ---------------------------------------
+// afl_l2_difference_noalloc keeps its partial sums here instead of in operator buffers: at most
+// kL2ScratchBlocks per-block pairs (the partial pass strides over the cell blocks, so any mesh fits)
+// and the two finished sums, 32.8 KB. Each GPU holds its own copy until the process ends. Host code
+// never passes the array's name to a kernel or a copy (in host code it is a stand-in, not a device
+// address): the address of this device's copy is looked up once per device and kept in
+// g_scratch_dev. Calls that overlap on one GPU share the array and race.
+constexpr int kL2ScratchBlocks = 2048;
+constexpr int kL2ScratchSize = 2 * kL2ScratchBlocks + 2;
+static __device__ __align__(256) double scratch[kL2ScratchSize];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// the current device's copy of scratch (looked up on the first call for that device)
+int current_device_scratch(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices) return fail(AFL_ERR_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(AFL_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return AFL_OK;
 }
 
 constexpr int kMaxRestart = 60;
@@ -634,6 +666,11 @@ int afl_create(const afl_mesh_desc* md, double mach, double froude, double dt, a
   int rc;
   if ((rc = op->rho_fix.alloc(o.N)) || (rc = op->p_fix.alloc(o.N)) || (rc = op->u_fix.alloc(o.N * o.dim)) ||
       (rc = ensure_common(op))) {
+    delete op;
+    return rc;
+  }
+  double* scr = nullptr;  // the device's copy of the no-alloc scratch, resolved at set-up
+  if ((rc = current_device_scratch(&scr))) {
     delete op;
     return rc;
   }
@@ -1217,6 +1254,30 @@ int afl_l2_difference(afl_operator* op, int ncomp, const double* a, const double
   *err_l2 = std::sqrt(std::max(0.0, op->host_sc[kScA + 4]));
   *ref_l2 = std::sqrt(std::max(0.0, op->host_sc[kScA + 5]));
   return check_device("l2_difference");
+}
+
+int afl_l2_difference_noalloc(afl_operator* op, int ncomp, const double* a, const double* b, double* err_l2,
+                              double* ref_l2) {
+  if (!op || !a || !err_l2 || !ref_l2) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (ncomp != 1 && ncomp != op->o.dim) return fail(AFL_ERR_INVALID_ARGUMENT, "ncomp must be 1 or dim");
+  int rc;
+  double* scr = nullptr;
+  if ((rc = current_device_scratch(&scr))) return rc;
+  const int64_t nb = l2_blocks(op->o, kL2ScratchBlocks);
+  if (2 * nb + 2 > kL2ScratchSize)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "l2_difference_noalloc: " + std::to_string(nb) +
+                                              " partial blocks exceed the scratch of " +
+                                              std::to_string(kL2ScratchBlocks));
+  // every partial slot the finishing pass reads is written by the partial pass first: nothing to clear
+  double* sums = scr + 2 * kL2ScratchBlocks;
+  launch_l2_difference(op->o, ncomp, a, b, scr, sums, kL2ScratchBlocks);
+  double h[2];
+  if (hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, op->o.stream) != hipSuccess)
+    return fail(AFL_ERR_DEVICE, "hipMemcpyAsync failed");
+  if (hipStreamSynchronize(op->o.stream) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
+  *err_l2 = std::sqrt(std::max(0.0, h[0]));
+  *ref_l2 = std::sqrt(std::max(0.0, h[1]));
+  return check_device("l2_difference_noalloc");
 }
 
 int afl_compute_diagonal(afl_operator* op) {

@@ -66,6 +66,7 @@ def load_library():
                                           C.POINTER(C.c_double)]
     L.afl_compute_diagnostics.argtypes = [vp, dp, dp, dp, C.POINTER(Diagnostics)]
     L.afl_l2_difference.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.afl_l2_difference_noalloc.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     for n in ("afl_vjp_rhs_rho_update", "afl_vjp_rhs_velocity_fixed", "afl_vjp_rhs_pressure"):
         getattr(L, n).argtypes = [vp, dp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
     L.afl_vmult_pressure_transpose.argtypes = [vp, dp, dp]
@@ -353,6 +354,14 @@ class EulerOperator:
         e, r = C.c_double(), C.c_double()
         self._chk(self.L.afl_l2_difference(self.h, ncomp, _ptr(a, 8, ncomp * self.N),
                                              None if b is None else _ptr(b, 8, ncomp * self.N), C.byref(e), C.byref(r)))
+        return e.value, r.value
+
+    def l2_difference_noalloc(self, ncomp, a, b=None):
+        """l2_difference through the entry point that allocates no device memory."""
+        e, r = C.c_double(), C.c_double()
+        self._chk(self.L.afl_l2_difference_noalloc(self.h, ncomp, _ptr(a, 8, ncomp * self.N),
+                                                     None if b is None else _ptr(b, 8, ncomp * self.N),
+                                                     C.byref(e), C.byref(r)))
         return e.value, r.value
 
     def compute_diagonal(self): self._chk(self.L.afl_compute_diagonal(self.h))

@@ -213,6 +213,14 @@ int afl_compute_diagnostics(afl_operator* op, const double* rho, const double* u
  * the mapped cells. Synchronises the stream. */
 int afl_l2_difference(afl_operator* op, int ncomp, const double* a, const double* b, double* err_l2,
                       double* ref_l2);
+/* afl_l2_difference without any device allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and results; the partial sums live in a fixed 32.8 KB device array
+ * of the library (one copy per GPU, its address resolved at afl_create or on the first call on a
+ * device), so no size limit applies beyond afl_create's. Calls that overlap on one GPU share that
+ * array and race. Synchronises the stream. */
+int afl_l2_difference_noalloc(afl_operator* op, int ncomp, const double* a, const double* b, double* err_l2,
+                              double* ref_l2);
 /* compute_diagonal / get_matrix_diagonal_inverse (euler_operator.h:2589-2626) */
 int afl_compute_diagonal(afl_operator* op);
 int afl_get_matrix_diagonal_inverse(afl_operator* op, double* dst);
