@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-snapshots", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first scale a 4N-double buffer by one in place (a kernel of known HBM traffic for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE calibration)")
     ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -182,6 +185,12 @@ def main():
     mesh, st = c3_workload()
     N = A.n_per_field(mesh)
     dim, dofs = 3, 5 * N
+    if args.calibrate:  # reads and writes exactly 4 N doubles (755 MB, beyond the 256 MB Infinity Cache)
+        buf = torch.ones(4 * N, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        buf.mul_(1.0)
+        torch.cuda.synchronize()
+        del buf
     paths = {}
     for name, pc in (("fp32_mg", A.AFL_SCHUR_MG_FP32), ("fp64_jacobi", A.AFL_SCHUR_JACOBI),
                      ("fp64_mg", A.AFL_SCHUR_MG_FP64)):
@@ -296,7 +305,8 @@ def main():
         d = kern[dom]
         res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "bytes_per_launch": d["bytes_per_launch"],
                            "avg_launch_us": d["avg_us"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(dom)}
+                           "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
+                           "fp64_vector": profiled_fp64(dom)}
     res["fp32_step_state_finite"] = bool(torch.isfinite(paths["fp32_mg"]["state"][2]).all())
     del paths, o32
     torch.cuda.synchronize()
@@ -404,6 +414,28 @@ def profiled_traffic(kernel):
             for tok in line.split():
                 if tok.startswith("bytes_per_launch="):
                     return float(tok.split("=")[1])
+    return None
+
+
+def profiled_fp64(kernel):
+    """FP64 vector rate of `kernel` from the committed instruction-counter summary of the headline workload
+    ((2 FMA + ADD + MUL) x 64 lanes x waves / duration, every lane counted), against the 78.6 TFLOP/s FP64
+    vector peak, and its VALU issue-slot use; None if not profiled."""
+    traffic = os.path.join(ROOT, "profiles", "pmc_traffic_hill3d_96x96x40_k3.txt")
+    mix = os.path.join(ROOT, "profiles", "pmc_valu_hill3d_96x96x40_k3.txt")
+    if not (os.path.exists(traffic) and os.path.exists(mix)):
+        return None
+    name = None
+    for line in open(traffic):
+        if line.startswith(kernel + ":"):
+            name = [t for t in line.split() if t.startswith("kernel=")][0][len("kernel="):]
+    if name is None:
+        return None
+    for line in open(mix):
+        tok = line[42:].split()
+        if len(tok) == 12 and "".join(line[:42].split()) == name:
+            return {"tflops": float(tok[-3]), "peak_tflops": 78.6, "frac": float(tok[-2]) / 100.0,
+                    "valu_issue_frac": float(tok[-1]) / 100.0, "source": "profiles/pmc_valu_hill3d_96x96x40_k3.txt"}
     return None
 
 
