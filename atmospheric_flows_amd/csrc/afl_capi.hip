@@ -2000,9 +2000,21 @@ __global__ void k_pack(int64_t N, int npc, int ncomp, const double* __restrict__
 }
 }  // namespace
 
+extern "C" int afl_mg_transfer(afl_operator* op, int k_coarse, int k_fine, int transpose, const double* src,
+                               double* dst) {
+  if (!op || !src || !dst) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (k_coarse < 1 || k_fine > 4 || k_coarse >= k_fine)
+    return fail(AFL_ERR_INVALID_ARGUMENT, "mg_transfer: degrees must satisfy 1 <= k_coarse < k_fine <= 4");
+  if (mg_transfer(op->o, op->tabs_by_k, k_coarse, k_fine, transpose != 0, src, dst))
+    return fail(AFL_ERR_INVALID_ARGUMENT, "mg_transfer: unsupported degree pair");
+  return check_device("mg_transfer");
+}
+
 extern "C" int afl_pack_dofs(afl_operator* op, int ncomp, const double* caller, const int64_t* perm, int64_t cs,
                              int64_t as, int64_t ns, double* out) {
   if (!op || !caller || !out || ncomp < 1) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (ncomp != 1 && ncomp != op->o.dim) return fail(AFL_ERR_INVALID_ARGUMENT, "ncomp must be 1 or dim");
+  if (cs < 0 || as < 0 || ns < 0) return fail(AFL_ERR_INVALID_ARGUMENT, "strides must be non-negative");
   const int n1 = op->o.k + 1, npc = op->o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
   int64_t b = (op->o.N * ncomp + AFL_NT - 1) / AFL_NT;
   k_pack<<<dim3((unsigned)(b > 65536 ? 65536 : b)), dim3(AFL_NT), 0, op->o.stream>>>(op->o.N, npc, ncomp, caller, perm, cs, as, ns, out, true);
@@ -2012,6 +2024,8 @@ extern "C" int afl_pack_dofs(afl_operator* op, int ncomp, const double* caller, 
 extern "C" int afl_unpack_dofs(afl_operator* op, int ncomp, const double* in, const int64_t* perm, int64_t cs,
                                int64_t as, int64_t ns, double* caller) {
   if (!op || !caller || !in || ncomp < 1) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (ncomp != 1 && ncomp != op->o.dim) return fail(AFL_ERR_INVALID_ARGUMENT, "ncomp must be 1 or dim");
+  if (cs < 0 || as < 0 || ns < 0) return fail(AFL_ERR_INVALID_ARGUMENT, "strides must be non-negative");
   const int n1 = op->o.k + 1, npc = op->o.dim == 2 ? n1 * n1 : n1 * n1 * n1;
   int64_t b = (op->o.N * ncomp + AFL_NT - 1) / AFL_NT;
   k_pack<<<dim3((unsigned)(b > 65536 ? 65536 : b)), dim3(AFL_NT), 0, op->o.stream>>>(op->o.N, npc, ncomp, in, perm, cs, as, ns, caller, false);

@@ -65,4 +65,11 @@ struct MgT {
   ~MgT() { release(); }
 };
 
+// The multigrid's transfer between degrees kc < kf (same mesh and node family, one component):
+// transpose = false: out (degree kf) = P in (degree kc), the nodal interpolation of the coarse polynomial
+// (the V-cycle's prolongation); true: out (degree kc) = P^T in (degree kf), its restriction. Returns
+// nonzero for degrees outside 1 <= kc < kf <= 4.
+int mg_transfer(const OpCtx& fine, const Tab* tabs_by_k, int kc, int kf, bool transpose, const double* in,
+                double* out);
+
 }  // namespace afl

@@ -581,6 +581,21 @@ static void interp_matrix(const Tab& to, int kto, const Tab& from, int kfrom, do
     }
 }
 
+int mg_transfer(const OpCtx& fine, const Tab* tabs_by_k, int kc, int kf, bool transpose, const double* in,
+                double* out) {
+  if (kc < 1 || kf > 4 || kc >= kf) return 1;
+  double P[5][5], Pt[5][5];
+  interp_matrix(tabs_by_k[kf], kf, tabs_by_k[kc], kc, P);  // the V-cycle's prolongation matrix
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) Pt[i][j] = P[j][i];
+  const int64_t nc = fine.geo.ncells;
+  const int64_t Nc = nc * (fine.dim == 2 ? (kc + 1) * (kc + 1) : (kc + 1) * (kc + 1) * (kc + 1));
+  const int64_t Nf = nc * (fine.dim == 2 ? (kf + 1) * (kf + 1) : (kf + 1) * (kf + 1) * (kf + 1));
+  if (!transpose) tensor_map<double>(fine.dim, kf, kc, false, nc, Nc, Nf, P, in, out, fine.stream);
+  else tensor_map<double>(fine.dim, kc, kf, false, nc, Nf, Nc, Pt, in, out, fine.stream);
+  return 0;
+}
+
 template struct MgT<float>;
 template struct MgT<double>;
 

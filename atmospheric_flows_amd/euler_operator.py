@@ -80,8 +80,9 @@ def load_library():
               "afl_vmult_enthalpy_tangent", "afl_vmult_enthalpy_tangent_transpose", "afl_vmult_schur_tangent",
               "afl_vmult_schur_tangent_transpose"):
         getattr(L, n).argtypes = [vp, dp, dp, dp]
+    L.afl_mg_transfer.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
     for n in ("afl_pack_dofs", "afl_unpack_dofs"):
-        getattr(L, n).argtypes = [vp, C.c_int, dp, dp, C.c_int64, C.c_int64, C.c_int64, dp]
+        getattr(L, n).argtypes = [vp, C.c_int, vp, vp, C.c_int64, C.c_int64, C.c_int64, vp]
     _lib = L
     return L
 
@@ -363,6 +364,58 @@ class EulerOperator:
                                                      None if b is None else _ptr(b, 8, ncomp * self.N),
                                                      C.byref(e), C.byref(r)))
         return e.value, r.value
+
+    def mg_transfer(self, k_coarse, k_fine, transpose, src, dst):
+        """Multigrid transfer: dst (k_fine) = P src (k_coarse), or with transpose dst (k_coarse) = P^T src."""
+        ncells = self.N // (self.mesh.degree + 1) ** self.dim
+        nc, nf = ncells * (k_coarse + 1) ** self.dim, ncells * (k_fine + 1) ** self.dim
+        n_in, n_out = (nf, nc) if transpose else (nc, nf)
+        self._chk(self.L.afl_mg_transfer(self.h, k_coarse, k_fine, 1 if transpose else 0, _ptr(src, 8, n_in),
+                                         _ptr(dst, 8, n_out)))
+        return dst
+
+    # ---- caller layouts (afl_pack_dofs / afl_unpack_dofs) ---------------------------------------
+    def _layout_args(self, ncomp, caller, cell_perm, cell_stride, comp_stride, node_stride):
+        """Checks a caller layout against the buffers: caller index of (cell c, component a, node i) =
+        cell_perm[c] cell_stride + a comp_stride + i node_stride must stay inside `caller`."""
+        torch = self.torch
+        npc = (self.mesh.degree + 1) ** self.dim
+        ncells = self.N // npc
+        if ncomp not in (1, self.dim):
+            raise AflError("ncomp must be 1 or dim")
+        if min(cell_stride, comp_stride, node_stride) < 0:
+            raise AflError("strides must be non-negative")
+        if cell_perm is None:
+            top = ncells - 1
+            pp = None
+        else:
+            if not (cell_perm.is_cuda and cell_perm.dtype == torch.int64 and cell_perm.is_contiguous()
+                    and cell_perm.numel() == ncells):
+                raise AflError(f"cell_perm must be a contiguous int64 device tensor of {ncells} entries")
+            lo, top = int(cell_perm.min()), int(cell_perm.max())
+            if lo < 0:
+                raise AflError("cell_perm holds a negative cell index")
+            pp = C.c_void_p(cell_perm.data_ptr())
+        extent = top * cell_stride + (ncomp - 1) * comp_stride + (npc - 1) * node_stride + 1
+        if not (caller.is_cuda and caller.dtype == torch.float64 and caller.is_contiguous()):
+            raise AflError("expected a contiguous float64 device tensor")
+        if caller.numel() < extent:
+            raise AflError(f"caller layout reaches index {extent - 1}, the buffer holds {caller.numel()}")
+        return C.c_void_p(caller.data_ptr()), pp
+
+    def pack_dofs(self, ncomp, caller, cell_perm, cell_stride, comp_stride, node_stride, out):
+        """Caller layout -> this library's layout (out: ncomp * N)."""
+        cp, pp = self._layout_args(ncomp, caller, cell_perm, cell_stride, comp_stride, node_stride)
+        self._chk(self.L.afl_pack_dofs(self.h, ncomp, cp, pp, cell_stride, comp_stride, node_stride,
+                                       _ptr(out, 8, ncomp * self.N)))
+        return out
+
+    def unpack_dofs(self, ncomp, src, cell_perm, cell_stride, comp_stride, node_stride, caller):
+        """This library's layout (src: ncomp * N) -> caller layout."""
+        cp, pp = self._layout_args(ncomp, caller, cell_perm, cell_stride, comp_stride, node_stride)
+        self._chk(self.L.afl_unpack_dofs(self.h, ncomp, _ptr(src, 8, ncomp * self.N), pp, cell_stride, comp_stride,
+                                         node_stride, cp))
+        return caller
 
     def compute_diagonal(self): self._chk(self.L.afl_compute_diagonal(self.h))
 

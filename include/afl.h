@@ -246,6 +246,14 @@ int afl_init_hill_state(afl_operator* op, const afl_hill_state* st, double* rho,
 /* Noise-free background (rho_bar, p_bar, theta_bar); any output may be NULL. */
 int afl_background(afl_operator* op, const afl_hill_state* st, double* rho_bar, double* p_bar, double* theta_bar);
 
+/* The p-multigrid transfer between degrees of the preconditioner's hierarchy, exposed so that callers and tests
+ * can check it (no counterpart in the reference, whose multigrid is geometric: MGTransferMatrixFree,
+ * Atmospheric_Flow.cc:478-486). One component on this operator's mesh and node family:
+ * transpose = 0: dst (degree k_fine) = P src (degree k_coarse), the nodal interpolation of the coarse
+ * polynomial (the V-cycle's prolongation); transpose = 1: dst (degree k_coarse) = P^T src (degree k_fine),
+ * its restriction. 1 <= k_coarse < k_fine <= 4. */
+int afl_mg_transfer(afl_operator* op, int k_coarse, int k_fine, int transpose, const double* src, double* dst);
+
 /* ---- layout conversion ---------------------------------------------------------------
  * dst[c*NPC*ncomp + ...] style caller layouts: caller index of (cell c, component a,
  * node i) = cell_perm[c]*cell_stride + a*comp_stride + i*node_stride (cell_perm may be
