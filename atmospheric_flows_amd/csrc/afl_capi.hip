@@ -225,6 +225,30 @@ struct afl_operator {
   double* host_sc = nullptr;  // pinned scalars
   Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
   KernelTimer timer;          // per-kernel HIP-event timing (afl_kernel_timer_*)
+  // phase timing of afl_imex_step (afl_section_timer_*): segment s spans events 2s, 2s+1
+  struct SectionTimer {
+    static constexpr size_t kMaxSegments = 1 << 16;
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> sec;
+    size_t nseg = 0;
+    // opens a segment of `section`: returns its index, or -1 when off or full
+    int open(hipStream_t st, int section) {
+      if (!on || nseg >= kMaxSegments) return -1;
+      while (ev.size() < 2 * (nseg + 1)) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ev.push_back(e);
+      }
+      if (sec.size() < nseg + 1) sec.resize(nseg + 1);
+      sec[nseg] = section;
+      hipEventRecord(ev[2 * nseg], st);
+      return (int)nseg++;
+    }
+    void close(hipStream_t st, int s) {
+      if (s >= 0) hipEventRecord(ev[2 * (size_t)s + 1], st);
+    }
+  } sect;
   std::vector<double> timer_ms;  // durations of the timed launches of the last timing window
   ~afl_operator() {
     DevBuf* all[] = {&rho_fix, &p_fix, &u_fix, &inv_diag, &dinv_D, &red, &dsc, &wS, &gV, &gW, &ydev, &gZ, &mgw, &diagw,
@@ -237,6 +261,7 @@ struct afl_operator {
     if (host_sc) hipHostFree(host_sc);
     if (dtab) hipFree(dtab);
     for (hipEvent_t e : timer.ev) hipEventDestroy(e);
+    for (hipEvent_t e : sect.ev) hipEventDestroy(e);
   }
   int64_t N() const { return o.N; }
   int dim() const { return o.dim; }
@@ -1007,6 +1032,38 @@ int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {
   return check_device("vmult_schur_velocity_f32");
 }
 
+int afl_section_timer_start(afl_operator* op) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  op->sect.nseg = 0;
+  op->sect.on = true;
+  return AFL_OK;
+}
+
+int afl_section_timer_stop(afl_operator* op) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  op->sect.on = false;
+  return AFL_OK;
+}
+
+int afl_section_timer_result(afl_operator* op, int section, int* calls, double* total_ms) {
+  if (!op || !calls || !total_ms) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  if (section < 0 || section >= AFL_N_SECTIONS) return fail(AFL_ERR_INVALID_ARGUMENT, "unknown section");
+  if (hipStreamSynchronize(op->o.stream) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
+  int n = 0;
+  double ms = 0.0;
+  for (size_t s = 0; s < op->sect.nseg; ++s) {
+    if (op->sect.sec[s] != section) continue;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, op->sect.ev[2 * s], op->sect.ev[2 * s + 1]) != hipSuccess)
+      return fail(AFL_ERR_DEVICE, "hipEventElapsedTime failed");
+    ++n;
+    ms += t;
+  }
+  *calls = n;
+  *total_ms = ms;
+  return AFL_OK;
+}
+
 int afl_kernel_timer_start(afl_operator* op, int kernels, int max_launches) {
   if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
   constexpr int kAll = AFL_TIME_LEVEL_W | AFL_TIME_LEVEL_C_CHEB | AFL_TIME_RHS_RHO | AFL_TIME_RHS_U | AFL_TIME_RHS_P |
@@ -1108,7 +1165,6 @@ int afl_write_vtu(afl_operator* op, const char* path, const double* rho, const d
   if (!op || !path || !rho || !u || !p) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
   const OpCtx& o = op->o;
   const int dim = o.dim, k = o.k, n1 = k + 1;
-  if (high_order && dim != 2) return fail(AFL_ERR_INVALID_ARGUMENT, "high-order VTU output is written for 2D meshes");
   const int64_t N = o.N, ncells = o.geo.ncells;
   const int npc = dim == 2 ? n1 * n1 : n1 * n1 * n1;
   std::vector<double> hr(N), hu(N * dim), hp(N);
@@ -1168,6 +1224,38 @@ int afl_write_vtu(afl_operator* op, const char* path, const double* rho, const d
   std::vector<uint8_t> types;
   auto pid = [&](int64_t c, int a, int b, int cc) { return c * npc + (int64_t)(cc * n1 + b) * n1 + a; };
   for (int64_t c = 0; c < ncells; ++c) {
+    if (high_order && dim == 3) {
+      // VTK_LAGRANGE_HEXAHEDRON (file version 2.2 ordering): corners (0,0,0) (k,0,0) (k,k,0) (0,k,0), the
+      // same four at z = k; the bottom edges y = 0, x = k, y = k, x = 0 (each in increasing x or y), the
+      // same four at z = k, the vertical edges at (x, y) = (0,0) (k,0) (k,k) (0,k); the faces x = 0, x = k
+      // (y fastest), y = 0, y = k (x fastest), z = 0, z = k (x fastest); the interior, x fastest
+      const int c0[8][3] = {{0, 0, 0}, {k, 0, 0}, {k, k, 0}, {0, k, 0}, {0, 0, k}, {k, 0, k}, {k, k, k}, {0, k, k}};
+      for (const auto& v : c0) conn.push_back(pid(c, v[0], v[1], v[2]));
+      for (int zc : {0, k}) {
+        for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, 0, zc));
+        for (int j = 1; j < k; ++j) conn.push_back(pid(c, k, j, zc));
+        for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, k, zc));
+        for (int j = 1; j < k; ++j) conn.push_back(pid(c, 0, j, zc));
+      }
+      const int vx[4][2] = {{0, 0}, {k, 0}, {k, k}, {0, k}};
+      for (const auto& v : vx)
+        for (int l = 1; l < k; ++l) conn.push_back(pid(c, v[0], v[1], l));
+      for (int xf : {0, k})
+        for (int l = 1; l < k; ++l)
+          for (int j = 1; j < k; ++j) conn.push_back(pid(c, xf, j, l));
+      for (int yf : {0, k})
+        for (int l = 1; l < k; ++l)
+          for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, yf, l));
+      for (int zf : {0, k})
+        for (int j = 1; j < k; ++j)
+          for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, j, zf));
+      for (int l = 1; l < k; ++l)
+        for (int j = 1; j < k; ++j)
+          for (int i = 1; i < k; ++i) conn.push_back(pid(c, i, j, l));
+      offs.push_back((int64_t)conn.size());
+      types.push_back(72);
+      continue;
+    }
     if (high_order) {
       // VTK_LAGRANGE_QUADRILATERAL: corners, then the edges (0,1), (1,2), (3,2), (0,3), then the interior
       conn.push_back(pid(c, 0, 0, 0)); conn.push_back(pid(c, k, 0, 0));
@@ -1201,7 +1289,8 @@ int afl_write_vtu(afl_operator* op, const char* path, const double* rho, const d
   }
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(AFL_ERR_INVALID_ARGUMENT, std::string("write_vtu: cannot open ") + path);
-  std::string hdr = "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
+  // version 2.2: the Lagrange hexahedron's vertical edges in corner order (VTK >= 9.1)
+  std::string hdr = "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"2.2\" "
                     "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\"" +
                     std::to_string(npts) + "\" NumberOfCells=\"" + std::to_string(types.size()) + "\">\n";
   auto arr = [&](const char* type, const char* name, int ncomp, const std::string& data) {
@@ -1402,39 +1491,57 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
     return AFL_OK;
   };
 
+  auto& sect = op->sect;
+  int seg;
   // stage 1
   o.hyp = 1;
+  seg = sect.open(st, AFL_SECTION_UPDATE_DENSITY);
   { const double* s[2] = {rho, u}; launch_rhs_rho(o, 1, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho2.p);
+  sect.close(st, seg);
   vec_copy(st, N, op->rho2.p, op->rho_fix.p);
   op->mg_coef_stale = true;
   op->mg_eig_stale = true;
-  if ((rc = fixed_point(1, op->rho2.p, p, u, op->p2.p, op->u2.p))) return rc;
+  seg = sect.open(st, AFL_SECTION_FIXED_POINT_PRESSURE);
+  rc = fixed_point(1, op->rho2.p, p, u, op->p2.p, op->u2.p);
+  sect.close(st, seg);
+  if (rc) return rc;
   // stage 2
   o.hyp = 2;
+  seg = sect.open(st, AFL_SECTION_UPDATE_DENSITY);
   { const double* s[4] = {rho, u, op->rho2.p, op->u2.p}; launch_rhs_rho(o, 2, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rho3.p);
+  sect.close(st, seg);
   vec_copy(st, N, op->rho3.p, op->rho_fix.p);
   op->mg_coef_stale = true;
   op->mg_eig_stale = true;
-  if ((rc = fixed_point(2, op->rho3.p, op->p2.p, op->u2.p, op->p3.p, op->u3.p))) return rc;
+  seg = sect.open(st, AFL_SECTION_FIXED_POINT_PRESSURE);
+  rc = fixed_point(2, op->rho3.p, op->p2.p, op->u2.p, op->p3.p, op->u3.p);
+  sect.close(st, seg);
+  if (rc) return rc;
   // stage 3
   o.hyp = 3;
+  seg = sect.open(st, AFL_SECTION_UPDATE_DENSITY);
   { const double* s[6] = {rho, u, op->rho2.p, op->u2.p, op->rho3.p, op->u3.p}; launch_rhs_rho(o, 3, s, op->rhs_r.p); }
   launch_mass_solve(o, nullptr, 1.0, 1, op->rhs_r.p, op->rhoc.p);
+  sect.close(st, seg);
   vec_copy(st, N, op->rhoc.p, op->rho_fix.p);
   op->mg_coef_stale = true;
   op->mg_eig_stale = true;
+  seg = sect.open(st, AFL_SECTION_UPDATE_VELOCITY);
   {
     const double* s[9] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p};
     launch_rhs_u(o, 3, s, op->rhs_u.p);
   }
   launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u.p, op->uc.p);
+  sect.close(st, seg);
+  seg = sect.open(st, AFL_SECTION_UPDATE_PRESSURE);
   {
     const double* s[11] = {rho, u, p, op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p, op->rhoc.p, op->uc.p};
     launch_rhs_p(o, 3, s, op->rhs_p.p);
   }
   launch_mass_solve(o, nullptr, 1.0 / (kGamma - 1.0), 1, op->rhs_p.p, op->pc.p);
+  sect.close(st, seg);
   vec_copy(st, N, op->rhoc.p, rho);
   vec_copy(st, NU, op->uc.p, u);
   vec_copy(st, N, op->pc.p, p);

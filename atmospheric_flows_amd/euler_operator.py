@@ -81,6 +81,9 @@ def load_library():
               "afl_vmult_schur_tangent_transpose"):
         getattr(L, n).argtypes = [vp, dp, dp, dp]
     L.afl_mg_transfer.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+    for n in ("afl_section_timer_start", "afl_section_timer_stop"):
+        getattr(L, n).argtypes = [vp]
+    L.afl_section_timer_result.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     for n in ("afl_pack_dofs", "afl_unpack_dofs"):
         getattr(L, n).argtypes = [vp, C.c_int, vp, vp, C.c_int64, C.c_int64, C.c_int64, vp]
     _lib = L
@@ -320,6 +323,18 @@ class EulerOperator:
         return w
 
     # ---- benchmark instrumentation -------------------------------------------------------------
+    SECTIONS = ("Update density", "Fixed point pressure", "Update velocity", "Update pressure")
+
+    def section_timer_start(self): self._chk(self.L.afl_section_timer_start(self.h))
+
+    def section_timer_stop(self): self._chk(self.L.afl_section_timer_stop(self.h))
+
+    def section_timer_result(self, section):
+        """(timed phases, summed milliseconds) of one afl_imex_step section (index into SECTIONS)."""
+        n, ms = C.c_int(), C.c_double()
+        self._chk(self.L.afl_section_timer_result(self.h, section, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
     def kernel_timer_start(self, kernels, max_launches=100000):
         """HIP events around every launch of the kernels in `kernels` (an or-ed mask of AFL_TIME_*)."""
         self._chk(self.L.afl_kernel_timer_start(self.h, kernels, max_launches))

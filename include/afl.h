@@ -185,6 +185,14 @@ int afl_observe_theta_transpose(afl_operator* op, const afl_hill_state* bg, cons
  * duration over every timed kernel; afl_kernel_timer_result then gives one kernel's launches, summed
  * duration and summed algorithmic bytes (each distinct input and output field once), restricted to
  * one IMEX stage (1..3; 0: every stage). */
+/* Phase timing of afl_imex_step, the sections of the reference's TimerOutput time_table
+ * (Atmospheric_Flow.cc:246-249, 432, 546, 682, 790; enum afl_timed_section): HIP events on the
+ * operator's stream around every phase of every step between start and stop (start clears earlier
+ * records). afl_section_timer_result synchronises the stream and returns a section's number of timed
+ * phases and their summed duration. */
+int afl_section_timer_start(afl_operator* op);
+int afl_section_timer_stop(afl_operator* op);
+int afl_section_timer_result(afl_operator* op, int section, int* calls, double* total_ms);
 int afl_kernel_timer_start(afl_operator* op, int kernels, int max_launches);
 int afl_kernel_timer_stop(afl_operator* op, int* launches, double* total_ms);
 int afl_kernel_timer_result(afl_operator* op, int kernel, int stage, int* launches, double* total_ms,
@@ -194,8 +202,9 @@ int afl_kernel_timer_result(afl_operator* op, int kernel, int stage, int* launch
 /* VTU output of a state (EulerSolver::output_results, Atmospheric_Flow.cc:886-918): point data
  * "rho", "u" (3 components) and "p" on k+1 equidistant patch points per direction of every cell
  * (DataOut::build_patches with n_subdivisions = degree, mapped with the terrain-following map),
- * written as k^dim linear sub-cells (high_order = 0) or, in 2D, one VTK Lagrange quadrilateral
- * per cell (high_order = 1, the reference's second file). Binary (base64) XML, UInt64 headers.
+ * written as k^dim linear sub-cells (high_order = 0) or one VTK Lagrange quadrilateral (2D) or
+ * hexahedron (3D, VTK file version 2.2 point order) per cell (high_order = 1, the reference's second
+ * file, :909-913). Binary (base64) XML, UInt64 headers.
  * Copies the fields to the host; synchronises the stream. */
 int afl_write_vtu(afl_operator* op, const char* path, const double* rho, const double* u, const double* p,
                   int high_order);

@@ -119,6 +119,16 @@ enum afl_timed_kernel {
   AFL_TIME_APPLY_B = 512       /* FP64 B apply */
 };
 
+/* Phases of afl_imex_step timed by afl_section_timer_*: the reference's TimerOutput sections
+ * (Atmospheric_Flow.cc:432, 546, 682, 790). */
+enum afl_timed_section {
+  AFL_SECTION_UPDATE_DENSITY = 0,         /* E1 and M^-1, every stage */
+  AFL_SECTION_FIXED_POINT_PRESSURE = 1,   /* the fixed-point loop of an implicit stage (its velocity updates included) */
+  AFL_SECTION_UPDATE_VELOCITY = 2,        /* stage 3: E2 and A^-1 */
+  AFL_SECTION_UPDATE_PRESSURE = 3,        /* stage 3: E3 and D^-1 */
+  AFL_N_SECTIONS = 4
+};
+
 /* Diagnostics of a state (EulerSolver::get_min_density / get_max_density /
  * compute_max_celerity / get_max_velocity and the Courant numbers printed in run,
  * Atmospheric_Flow.cc:992-1087, 1225-1235). */

@@ -38,7 +38,36 @@ def _lagrange(nodes, x):
     return L
 
 
-@pytest.mark.parametrize("dim,k,nc,high", [(2, 3, (6, 1, 4), False), (2, 4, (5, 1, 3), True), (3, 2, (3, 2, 2), False)])
+def _vtk_hex_index(i, j, l, k):
+    """VTK's point index of lattice point (i, j, l) in a degree-k Lagrange hexahedron (VTK >= 9.1, file
+    version 2.2: vtkHigherOrderHexahedron::PointIndexFromIJK with the vertical edges in corner order)."""
+    ib, jb, lb = i in (0, k), j in (0, k), l in (0, k)
+    nb = ib + jb + lb
+    if nb == 3:
+        return ((2 if j else 1) if i else (3 if j else 0)) + (4 if l else 0)
+    off = 8
+    if nb == 2:
+        if not ib:
+            return (i - 1) + ((k - 1) * 2 if j else 0) + (4 * (k - 1) if l else 0) + off
+        if not jb:
+            return (j - 1) + ((k - 1) if i else 3 * (k - 1)) + (4 * (k - 1) if l else 0) + off
+        off += 8 * (k - 1)
+        return (l - 1) + (k - 1) * ((2 if j else 1) if i else (3 if j else 0)) + off
+    off += 12 * (k - 1)
+    if nb == 1:
+        if ib:
+            return (j - 1) + (k - 1) * (l - 1) + ((k - 1) ** 2 if i else 0) + off
+        off += 2 * (k - 1) ** 2
+        if jb:
+            return (i - 1) + (k - 1) * (l - 1) + ((k - 1) ** 2 if j else 0) + off
+        off += 2 * (k - 1) ** 2
+        return (i - 1) + (k - 1) * (j - 1) + ((k - 1) ** 2 if l else 0) + off
+    off += 6 * (k - 1) ** 2
+    return off + (i - 1) + (k - 1) * ((j - 1) + (k - 1) * (l - 1))
+
+
+@pytest.mark.parametrize("dim,k,nc,high", [(2, 3, (6, 1, 4), False), (2, 4, (5, 1, 3), True), (3, 2, (3, 2, 2), False),
+                                           (3, 3, (3, 2, 2), True), (3, 4, (2, 2, 2), True)])
 def test_vtu_matches_numpy(oracle_mod, tmp_path, dim, k, nc, high):
     say(f"[gpu vtu] dim={dim} k={k} cells={nc} high_order={high}")
     import torch
@@ -96,9 +125,18 @@ def test_vtu_matches_numpy(oracle_mod, tmp_path, dim, k, nc, high):
     types = arrays["types"]
     offs = arrays["offsets"]
     conn = arrays["connectivity"]
-    if high:
+    if high and dim == 2:
         assert np.all(types == 70) and len(types) == ncells and offs[-1] == ncells * npc
         assert sorted(conn[:npc]) == list(range(npc))
+    elif high:
+        assert root.get("version") == "2.2"
+        assert np.all(types == 72) and len(types) == ncells and offs[-1] == ncells * npc
+        expect = np.empty(ncells * npc, dtype=np.int64)
+        for c in range(ncells):
+            for q in range(npc):
+                i, j, l = q % n1, (q // n1) % n1, q // (n1 * n1)
+                expect[c * npc + _vtk_hex_index(i, j, l, k)] = c * npc + q
+        assert np.array_equal(conn, expect)
     else:
         sub = k ** dim
         assert len(types) == ncells * sub and np.all(types == (9 if dim == 2 else 12))
