@@ -224,6 +224,7 @@ struct afl_operator {
   }
   double* host_sc = nullptr;  // pinned scalars
   Tab* dtab = nullptr;        // device copy of the basis tables (o.dT)
+  int verbose = 0;            // progress lines on stderr (afl_set_verbose)
   KernelTimer timer;          // per-kernel HIP-event timing (afl_kernel_timer_*)
   // phase timing of afl_imex_step (afl_section_timer_*): segment s spans events 2s, 2s+1
   struct SectionTimer {
@@ -1030,6 +1031,12 @@ int afl_vmult_schur_velocity_f32(afl_operator* op, float* w, const float* p) {
   if ((rc = ensure_common(op)) || (rc = ensure_mg(op))) return rc;
   op->mg32.schur_velocity(op->o, p, w);
   return check_device("vmult_schur_velocity_f32");
+}
+
+int afl_set_verbose(afl_operator* op, int level) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
+  op->verbose = level;
+  return AFL_OK;
 }
 
 int afl_section_timer_start(afl_operator* op) {
@@ -1860,6 +1867,11 @@ struct AdjRun {
     if (rc) return rc;
     cur = nx;
     ++st.adjoint_steps;
+    if (op->verbose) {
+      std::fprintf(stderr, "[afl] adjoint_run: %d steps reversed, %d forward steps\n", st.adjoint_steps,
+                   st.forward_steps);
+      std::fflush(stderr);
+    }
     return AFL_OK;
   }
   int reverse(int a, int b, int slot, int s) {
@@ -1953,6 +1965,10 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
                             lam[nx][0].p, lam[nx][1].p, lam[nx][2].p);
       cur = nx;
       ++as.adjoint_steps;
+      if (op->verbose) {
+        std::fprintf(stderr, "[afl] adjoint_run: %d steps reversed\n", as.adjoint_steps);
+        std::fflush(stderr);
+      }
     }
     if (!rc) {
       vec_copy(st, N, lam[cur][0].p, rho_bar_0);

@@ -81,6 +81,7 @@ def load_library():
               "afl_vmult_schur_tangent_transpose"):
         getattr(L, n).argtypes = [vp, dp, dp, dp]
     L.afl_mg_transfer.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.afl_set_verbose.argtypes = [vp, C.c_int]
     for n in ("afl_section_timer_start", "afl_section_timer_stop"):
         getattr(L, n).argtypes = [vp]
     L.afl_section_timer_result.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
@@ -323,6 +324,8 @@ class EulerOperator:
         return w
 
     # ---- benchmark instrumentation -------------------------------------------------------------
+    def set_verbose(self, level): self._chk(self.L.afl_set_verbose(self.h, int(level)))
+
     SECTIONS = ("Update density", "Fixed point pressure", "Update velocity", "Update pressure")
 
     def section_timer_start(self): self._chk(self.L.afl_section_timer_start(self.h))

@@ -70,9 +70,10 @@ def c2_workload():
     return mesh, st
 
 
-def params(precond, eps):
+def params(precond, eps, restart=28):
     from atmospheric_flows_amd import make_solver_params
-    return make_solver_params(gmres_eps=eps, gmres_restart=28, fp_tol=1e-10, fp_max_iter=100, schur_precond=precond)
+    return make_solver_params(gmres_eps=eps, gmres_restart=restart, fp_tol=1e-10, fp_max_iter=100,
+                              schur_precond=precond)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -351,6 +352,11 @@ def main():
         log(f"adjoint leg: {args.adjoint_steps} steps")
         res["adjoint"] = adjoint_leg(args, torch, EulerOperator, A)
 
+    # ---- one GPU's share of the 3D 192x192x64, k=4 configuration: forward steps and adjoint gradient ----
+    if "c5" in legs:
+        log("c5 shard leg: 24x192x64 cells, k=4")
+        res["c5_shard"] = c5_shard_leg(torch, EulerOperator, A, log)
+
     if not args.no_cpu:
         log("CPU baseline")
         res["cpu_baseline"] = run_cpu_subprocess()
@@ -401,6 +407,101 @@ def adjoint_leg(args, torch, EulerOperator, A):
 
 
 A_STORE_ALL = -1
+
+
+def c5_shard_leg(torch, EulerOperator, A, log, nsteps=20, npts=4096, snapshots=20):
+    """The 3D 192x192x64-cell, k=4 configuration (288 x 288 x 25 km, hill h0 = 600 m, a = 2 km) split over
+    eight GPUs along x gives each a 24x192x64-cell slab (36 x 288 x 25 km, 184 M DoF); this runs one slab at
+    its real shape with periodic x (the neighbours' halo faces replaced by the slab's own): 20 FP32-MG steps
+    timed, the FP64-MG path over the same steps for the theta' and w errors, and the gradient of the
+    final-time theta' misfit at 4096 seeded points of the slab by binomial checkpointing."""
+    import numpy as np
+    from atmospheric_flows_amd import make_mesh, make_hill_state
+    half = 24 * 1500.0 / 2
+    mesh = make_mesh(dim=3, degree=4, n_cells=(24, 192, 64), x_range=(-half, half), y_range=(-144e3, 144e3),
+                     z_top=25e3, hill_height=600.0, hill_width=2000.0, periodic=(1, 1))
+    st = make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01, seed=2024)
+    rng = np.random.default_rng(4096)
+    pts = np.stack([rng.uniform(-half, half, npts), rng.uniform(-143e3, 143e3, npts),
+                    rng.uniform(500.0, 20e3, npts)], axis=1)
+    states, times, iters = {}, {}, {}
+    free0 = torch.cuda.mem_get_info()[0]
+    for name, pc in (("fp64_mg", A.AFL_SCHUR_MG_FP64), ("fp32_mg", A.AFL_SCHUR_MG_FP32)):
+        log(f"c5 shard: {nsteps} {name} steps")
+        o = EulerOperator(mesh, mach=1.0, froude=FR, dt=1.0)
+        o.set_solver_params(params(pc, 1e-8, restart=8))
+        x = o.init_hill_state(st)
+        o.imex_step(*tuple(t.clone() for t in x))  # set-up (multigrid levels, eigenvalue estimates)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = [o.imex_step(*x) for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / nsteps
+        states[name] = x
+        iters[name] = {"fp": [list(i.fp_iters) for i in infos], "gmres": [list(i.gmres_iters) for i in infos]}
+        if name == "fp64_mg":
+            del o
+    used = free0 - torch.cuda.mem_get_info()[0]
+    N = A.n_per_field(mesh)
+    x32, x64 = states["fp32_mg"], states["fp64_mg"]
+
+    def rl2(a, b):
+        e, r = o.l2_difference(1, a.contiguous(), b.contiguous())
+        return e / r
+
+    def theta_prime(rho, p):
+        _, _, th_bar = o.background(st)
+        return p / (rho * R_GAS) * torch.pow(st.p0 / p, R_GAS / CP) - th_bar
+
+    errs = {"theta_prime": rl2(theta_prime(x32[0], x32[2]), theta_prime(x64[0], x64[2])),
+            "w": rl2(x32[1][2 * N:], x64[1][2 * N:]), "rho": rl2(x32[0], x64[0]), "p": rl2(x32[2], x64[2])}
+    del states, x32, x64
+    # adjoint: obs from a seed-7 perturbed initial state, gradient of J at the seed-2024 state
+    log("c5 shard: adjoint gradient")
+    o.set_verbose(1)  # the reversal reports each reversed step
+    o.set_observation_points(pts)
+    truth = o.init_hill_state(make_hill_state(theta0=280.0, brunt=0.01, p0=1e5, u0=10.0, theta_sigma=0.01, seed=7))
+    for _ in range(nsteps):
+        o.imex_step(*truth)
+    obs = o.observe_theta(st, truth[0], truth[2])
+    del truth
+    log("c5 shard: observations ready; timed forward run")
+    x0 = o.init_hill_state(st)
+    x = tuple(t.clone() for t in x0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        o.imex_step(*x)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    d = o.observe_theta(st, x[0], x[2]) - obs
+    rb, pb = o.observe_theta_transpose(st, x[0], x[2], d)
+    free1 = torch.cuda.mem_get_info()[0]
+    # a two-step reversal first (its cost per step bounds the full run's duration)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    o.adjoint_run(*x0, 2, snapshots, rb, torch.zeros_like(x[1]), pb)
+    torch.cuda.synchronize()
+    log(f"c5 shard: two-step reversal {time.perf_counter() - t0:.1f} s; {nsteps}-step reversal")
+    t0 = time.perf_counter()
+    g = o.adjoint_run(*x0, nsteps, snapshots, rb, torch.zeros_like(x[1]), pb)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    log(f"c5 shard: reversal {t_adj:.1f} s")
+    used_adj = free1 - torch.cuda.mem_get_info()[0]
+    dofs = 5 * N
+    return {"workload": "3D hill 192x192x64 cells, k=4 (288x288x25 km, h0=600 m, a=2 km), one GPU's 24x192x64-cell "
+                        "x-slab at its real shape (periodic x), FP32-MG FGMRES (restart 8, true residual 1e-8)",
+            "n_per_field": N, "dof": dofs, "steps": nsteps,
+            "fp32_mg_ms_per_step": times["fp32_mg"] * 1e3, "value_fp32_mg": dofs / times["fp32_mg"],
+            "fp64_mg_ms_per_step": times["fp64_mg"] * 1e3,
+            "iters_per_step": iters,
+            "rel_l2_fp32mg_vs_fp64mg_after_steps": errs,
+            "adjoint": {"observation_points": npts, "snapshots": snapshots, "forward_run_s": t_fwd,
+                        "adjoint_s": t_adj, "cost_vs_forward_run": t_adj / t_fwd, "schedule": g[3],
+                        "gradient_finite": bool(torch.isfinite(g[2]).all() and torch.isfinite(g[0]).all())},
+            "device_memory_gb": {"solver_paths": used / 1e9, "adjoint_run_extra": used_adj / 1e9,
+                                 "full_mesh_estimate_x8": 8 * (used + used_adj) / 1e9, "hbm": 288 * 1.073741824}}
 
 
 def profiled_traffic(kernel):

@@ -185,6 +185,10 @@ int afl_observe_theta_transpose(afl_operator* op, const afl_hill_state* bg, cons
  * duration over every timed kernel; afl_kernel_timer_result then gives one kernel's launches, summed
  * duration and summed algorithmic bytes (each distinct input and output field once), restricted to
  * one IMEX stage (1..3; 0: every stage). */
+/* Progress lines on stderr (the reference's `verbose` argument of EulerSolver::run and its verbose_cout,
+ * Atmospheric_Flow.cc:1101-1102): level >= 1 reports every reversed step of
+ * afl_adjoint_run. 0 (the default) prints nothing. */
+int afl_set_verbose(afl_operator* op, int level);
 /* Phase timing of afl_imex_step, the sections of the reference's TimerOutput time_table
  * (Atmospheric_Flow.cc:246-249, 432, 546, 682, 790; enum afl_timed_section): HIP events on the
  * operator's stream around every phase of every step between start and stop (start clears earlier
