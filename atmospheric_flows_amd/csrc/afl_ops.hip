@@ -1322,34 +1322,55 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   }
 }
 
-// sinv[c] = 1 / sum_q rho(q) JxW(q) of cell c (one thread per cell)
+// sinv[c] = 1 / sum_q rho(q) JxW(q) of cell c (one thread per cell). JxW(q) = F(qx, qy) W(qz) on the
+// terrain-following map (F: the line's horizontal factor), so the sum factorises:
+//   sum_(qx, qy) F(qx, qy) sum_(ix, iy) S[qx][ix] S[qy][iy] sum_iz wz[iz] rho(ix, iy, iz),
+// wz[iz] = sum_qz W(qz) S[qz][iz] = the integral of the 1D basis function iz.
 template <int DIM, int K, typename R, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) void k_cell_rho_mass_inv(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ rho,
                                                                R* __restrict__ sinv) {
   const auto& T = *Tp;
   using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ, N1 = C::N1, NPC = C::NPC;
+  constexpr int NH = DIM == 3 ? N1 * N1 : N1;  // horizontal node positions
+  double wz[N1];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) a += (double)T.W[q] * (double)T.S[q][i];
+    wz[i] = a;
+  }
   for (int64_t c = blockIdx.x * (int64_t)AFL_NT + threadIdx.x; c < g.ncells; c += (int64_t)gridDim.x * AFL_NT) {
     int ic[3];
     ic[0] = (int)(c % g.nc[0]);
     if (DIM == 2) { ic[1] = (int)(c / g.nc[0]); ic[2] = 0; }
     else { ic[1] = (int)((c / g.nc[0]) % g.nc[1]); ic[2] = (int)(c / ((int64_t)g.nc[0] * g.nc[1])); }
     const R* r = rho + c * NPC;
+    double rz[NH];  // rho integrated along z at each horizontal node position
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      double a = 0.0;
+#pragma unroll
+      for (int iz = 0; iz < N1; ++iz) a += wz[iz] * (double)r[iz * NH + h];
+      rz[h] = a;
+    }
     double acc = 0.0;
     for (int l = 0; l < C::LT; ++l) {
       LineGeo<DIM, K, R, Q> lg;
       lg.init(g, T, ic, l);
       const int qx = l % NQ, qy = DIM == 3 ? l / NQ : 0;
-      for (int qz = 0; qz < NQ; ++qz) {
-        double v = 0.0;
-        for (int i = 0; i < NPC; ++i) {
-          const int ix = i % N1, iy = DIM == 3 ? (i / N1) % N1 : 0, iz = i / (DIM == 3 ? N1 * N1 : N1);
-          double phi = (double)T.S[qx][ix] * (double)T.S[qz][iz];
-          if (DIM == 3) phi *= (double)T.S[qy][iy];
-          v += phi * (double)r[i];
-        }
-        acc += v * (double)lg.jxw(g, T, qz);
+      const double F = DIM == 3 ? (double)lg.hx * (double)lg.hy * (double)lg.c * (double)lg.wxy
+                                : (double)lg.hx * (double)lg.c * (double)lg.wxy;
+      double v = 0.0;
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int ix = h % N1, iy = h / N1;
+        double phi = (double)T.S[qx][ix];
+        if (DIM == 3) phi *= (double)T.S[qy][iy];
+        v += phi * rz[h];
       }
+      acc += F * v;
     }
     sinv[c] = (R)(1.0 / acc);
   }
