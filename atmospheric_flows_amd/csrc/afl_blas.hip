@@ -25,6 +25,10 @@ __global__ void k_sub_mul(int64_t n, const double* __restrict__ b, const double*
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     o[i] = (b[i] - y[i]) * d[i];
 }
+__global__ void k_sub(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    o[i] = a[i] - b[i];
+}
 __global__ void k_mul(int64_t n, const double* __restrict__ x, const double* __restrict__ d, double* __restrict__ o) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     o[i] = x[i] * d[i];
@@ -123,6 +127,9 @@ void vec_axpby(hipStream_t st, int64_t n, double a, const double* x, double b, d
 }
 void vec_sub_mul(hipStream_t st, int64_t n, const double* b, const double* y, const double* d, double* out) {
   k_sub_mul<<<dim3(grid_1d(n)), dim3(AFL_NT), 0, st>>>(n, b, y, d, out);
+}
+void vec_sub(hipStream_t st, int64_t n, const double* a, const double* b, double* out) {
+  k_sub<<<dim3(grid_1d(n)), dim3(AFL_NT), 0, st>>>(n, a, b, out);
 }
 void vec_mul(hipStream_t st, int64_t n, const double* x, const double* d, double* out) {
   k_mul<<<dim3(grid_1d(n)), dim3(AFL_NT), 0, st>>>(n, x, d, out);

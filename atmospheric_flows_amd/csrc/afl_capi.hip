@@ -214,6 +214,7 @@ struct afl_operator {
   // solve (adjoint variables change little from one step to the previous)
   DevBuf adj_lam_prev[2];
   bool adj_lam_valid[2] = {false, false};
+  bool adj_keep_stages = true;  // afl_adjoint_run keeps stage sets (afl_set_adjoint_stage_sets)
   // observation points (device copies of their location) and point scratch
   ObsDev obs;
   DevBuf obs_f;
@@ -377,9 +378,10 @@ int ensure_dinv(afl_operator* op) {
 // (The reference's left-preconditioned deal.II GMRES compares the preconditioned residual
 // with eps*||rhs||; the Arnoldi estimate here is the true residual ||b - S x||.)
 // The operator is a callable apply(in, out); what names the solve in errors.
+// Sx0 (optional): apply(x) already computed by the caller, used for the first residual.
 template <class Apply>
 int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double tol, int max_it, int m, int* iters,
-             double* res_out, const char* what) {
+             double* res_out, const char* what, const double* Sx0 = nullptr) {
   int rc;
   const int64_t N = op->N();
   hipStream_t st = op->o.stream;
@@ -396,9 +398,11 @@ int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double
   int it = 0;
   double res = 0.0;
   while (true) {
-    apply(x, w);
-    vec_copy(st, N, b, V);
-    vec_axpby(st, N, -1.0, w, 1.0, V);
+    const double* Ax = w;
+    if (Sx0) Ax = Sx0;
+    else apply(x, w);
+    Sx0 = nullptr;
+    vec_sub(st, N, b, Ax, V);
     vec_norm2(st, N, V, op->red.p, sc + kScNrm);
     if ((rc = fetch(op, kScNrm, 1))) return rc;
     const double beta = hs[kScNrm];
@@ -469,10 +473,11 @@ int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double
   return check_device("gmres");
 }
 
-int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out,
+          const double* Sx0) {
   return gmres_op(
       op, [op](const double* in, double* out) { schur_apply(op, in, out); }, x, b, tol, max_it, m, iters, res_out,
-      "Schur GMRES did not converge");
+      "Schur GMRES did not converge", Sx0);
 }
 
 // p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
@@ -535,9 +540,10 @@ int ensure_mg(afl_operator* op) {
 // z_j = MG(v_j) is stored, x += Z y. Outer arithmetic, Arnoldi basis and residual in FP64.
 // The operator is a callable apply(in, out); the preconditioner is the multigrid V-cycle of the
 // current solver parameters.
+// Sx0 (optional): apply(x) already computed by the caller, used for the first residual.
 template <class Apply>
 int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double tol, int max_it, int m, int* iters,
-              double* res_out, const char* what) {
+              double* res_out, const char* what, const double* Sx0 = nullptr) {
   int rc;
   const int64_t N = op->N();
   hipStream_t st = op->o.stream;
@@ -556,9 +562,11 @@ int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, doubl
   int it = 0;
   double res = 0.0;
   while (true) {
-    apply(x, w);
-    vec_copy(st, N, b, V);
-    vec_axpby(st, N, -1.0, w, 1.0, V);
+    const double* Ax = w;
+    if (Sx0) Ax = Sx0;
+    else apply(x, w);
+    Sx0 = nullptr;
+    vec_sub(st, N, b, Ax, V);
     vec_norm2(st, N, V, op->red.p, sc + kScNrm);
     if ((rc = fetch(op, kScNrm, 1))) return rc;
     const double beta = hs[kScNrm];
@@ -628,10 +636,11 @@ int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, doubl
   return check_device("fgmres");
 }
 
-int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out) {
+int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out,
+           const double* Sx0) {
   return fgmres_op(
       op, [op](const double* in, double* out) { schur_apply(op, in, out); }, x, b, tol, max_it, m, iters, res_out,
-      "Schur FGMRES did not converge");
+      "Schur FGMRES did not converge", Sx0);
 }
 
 bool valid_op(afl_operator* op) { return op != nullptr; }
@@ -1407,8 +1416,9 @@ int afl_solve_mass(afl_operator* op, double* x, const double* b) {
   return check_device("solve_mass");
 }
 
-int afl_schur_solve(afl_operator* op, double* p, const double* rhs, int* iters, double* residual) {
-  if (!op || !p || !rhs) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+// Schur solve S p = rhs from the initial guess in p; Sp0 (optional) holds S p already.
+static int schur_solve(afl_operator* op, double* p, const double* rhs, const double* Sp0, int* iters,
+                       double* residual) {
   int rc;
   if ((rc = ensure_common(op))) return rc;
   vec_norm2(op->o.stream, op->o.N, rhs, op->red.p, op->dsc.p + kScA);
@@ -1417,12 +1427,17 @@ int afl_schur_solve(afl_operator* op, double* p, const double* rhs, int* iters, 
   int it = 0;
   double res = 0.0;
   if (op->sp.schur_precond == AFL_SCHUR_JACOBI)
-    rc = gmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res);
+    rc = gmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res, Sp0);
   else
-    rc = fgmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res);
+    rc = fgmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res, Sp0);
   if (iters) *iters = it;
   if (residual) *residual = res;
   return rc;
+}
+
+int afl_schur_solve(afl_operator* op, double* p, const double* rhs, int* iters, double* residual) {
+  if (!op || !p || !rhs) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  return schur_solve(op, p, rhs, nullptr, iters, residual);
 }
 
 int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_info* info) {
@@ -1461,6 +1476,11 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
       launch_rhs_u(o, 2, su_, op->rhs_u0.p);
     }
     launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->rhs_u0.p, op->tmp1.p);
+    // Bp holds w = A^{-1} B pfix_old from the second iteration on: the velocity update computes it
+    // (u_fix = A^{-1} E2 - w, A^{-1} being linear), and the next Schur solve starts from
+    // S pfix_old = D pfix_old - C(p_fix; w) without a second B apply and cell solve (C's coefficient
+    // p_fix changes between iterations, w does not).
+    bool have_w = false;
     int it = 0, conv = 0, gm_total = 0;
     for (; it < op->sp.fp_max_iter; ++it) {
       // pressure_fixed_point, Atmospheric_Flow.cc:545-673
@@ -1471,16 +1491,24 @@ int afl_imex_step(afl_operator* op, double* rho, double* u, double* p, afl_step_
       launch_apply_C(o, op->p_fix.p, op->tmp1.p, nullptr, op->tmp2.p);
       vec_axpby(st, N, -1.0, op->tmp2.p, 1.0, op->rhs_p.p);
       vec_copy(st, N, op->pfix_old.p, op->pfix.p);
+      if (have_w) launch_apply_C(o, op->p_fix.p, op->Bp.p, op->pfix.p, op->tmp2.p);  // S pfix_old
       int gi = 0;
       double gr = 0.0;
-      const int r = afl_schur_solve(op, op->pfix.p, op->rhs_p.p, &gi, &gr);
+      const int r = schur_solve(op, op->pfix.p, op->rhs_p.p, have_w ? op->tmp2.p : nullptr, &gi, &gr);
       gm_total += gi;
       inf.schur_residual = gr;
       if (r) return r;
-      // update_velocity, Atmospheric_Flow.cc:681-781
-      launch_apply_B(o, op->pfix.p, op->Bp.p);
-      vec_axpby(st, NU, 1.0, op->rhs_u0.p, -1.0, op->Bp.p);  // E2 - B p
-      launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, op->Bp.p, op->ufix.p);
+      if (have_w && gi == 0 && op->sp.fp_tol > 0.0) {
+        // the solve kept pfix = pfix_old: the velocity update would reproduce u_fix and the
+        // pressure change is exactly zero, so the fixed point has converged
+        conv = 1;
+        ++it;
+        break;
+      }
+      // update_velocity, Atmospheric_Flow.cc:681-781: u_fix = A^{-1} (E2 - B p)
+      launch_schur_w(o, op->rho_fix.p, op->pfix.p, op->Bp.p);
+      vec_sub(st, NU, op->tmp1.p, op->Bp.p, op->ufix.p);
+      have_w = true;
       // relative inf-norm change of the nodal pressure, Atmospheric_Flow.cc:1140-1150
       vec_linf(st, N, op->pfix.p, op->red.p, op->dsc.p + kScA);
       vec_linf_diff(st, N, op->pfix.p, op->pfix_old.p, op->red.p, op->dsc.p + kScB);
@@ -1827,6 +1855,12 @@ struct Schedule {
   }
 };
 
+// Stage sets: with keep_stages, an advance that ends at a split point keeps the stage vectors of its
+// last step (rho2, u2, p2, rho3, u3, p3, rho_c, u_c). Every range of the reversal ends at the step
+// count or at a split point, so the reversal of a step whose set is held restores nothing: it runs
+// step_vjp_reverse on the set directly (a set lives from its advance to that step's reversal; at most
+// snapshots + 1 are held). Steps without a set (the last step, and the ranges reversed without
+// snapshots) recompute their forward step as afl_imex_step_vjp does.
 struct AdjRun {
   afl_operator* op;
   int64_t N, NU;
@@ -1837,6 +1871,39 @@ struct AdjRun {
   std::vector<int> free_slots;
   Schedule sch;
   afl_adjoint_stats st{};
+  bool keep_stages = true;
+  static constexpr int kStg = 8;
+  std::vector<DevBuf> sets;     // kStg buffers per stage set
+  std::vector<int> set_step;    // step whose stages the set holds, -1 when free
+  int sets_held = 0;
+  // keeps the stage vectors of the step just taken (step index j) in a free set
+  int keep(int j) {
+    int k = -1;
+    for (int i = 0; i < (int)set_step.size(); ++i)
+      if (set_step[i] < 0) { k = i; break; }
+    if (k < 0) {
+      k = (int)set_step.size();
+      set_step.push_back(-1);
+      sets.resize(sets.size() + kStg);
+      const bool vec[kStg] = {false, true, false, false, true, false, false, true};
+      for (int f = 0; f < kStg; ++f) {
+        int rc;
+        if ((rc = sets[(size_t)kStg * k + f].alloc(vec[f] ? NU : N))) return rc;
+      }
+    }
+    const bool vec[kStg] = {false, true, false, false, true, false, false, true};
+    const double* sv[kStg] = {op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p, op->rhoc.p, op->uc.p};
+    for (int f = 0; f < kStg; ++f) vec_copy(stream(), vec[f] ? NU : N, sv[f], sets[(size_t)kStg * k + f].p);
+    set_step[k] = j;
+    ++sets_held;
+    st.stage_sets_used = std::max(st.stage_sets_used, sets_held);
+    return AFL_OK;
+  }
+  int find_set(int j) const {
+    for (int i = 0; i < (int)set_step.size(); ++i)
+      if (set_step[i] == j) return i;
+    return -1;
+  }
   hipStream_t stream() const { return op->o.stream; }
   double* sr(int k) { return snap[3 * k].p; }
   double* su(int k) { return snap[3 * k + 1].p; }
@@ -1848,8 +1915,9 @@ struct AdjRun {
     return k;
   }
   void give(int k) { free_slots.push_back(k); --used; }
-  // working state <- state at slot k advanced by n steps
-  int advance(int k, int n) {
+  // working state <- state at slot k (step a) advanced by n steps; with keep_set the stage vectors
+  // of the last of them (step a + n - 1) are kept
+  int advance(int k, int a, int n, bool keep_set) {
     vec_copy(stream(), N, sr(k), wr.p);
     vec_copy(stream(), NU, su(k), wu.p);
     vec_copy(stream(), N, sp(k), wp.p);
@@ -1857,13 +1925,27 @@ struct AdjRun {
       int rc;
       if ((rc = afl_imex_step(op, wr.p, wu.p, wp.p, nullptr))) return rc;
       ++st.forward_steps;
+      if (keep_set && keep_stages && i == n - 1 && (rc = keep(a + i))) return rc;
     }
     return AFL_OK;
   }
-  int vjp_from_working() {
+  // reverses step j from the working state (the state at step j)
+  int vjp_from_working(int j) {
     const int nx = 1 - cur;
-    int rc = afl_imex_step_vjp(op, wr.p, wu.p, wp.p, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p, lam[nx][0].p,
-                               lam[nx][1].p, lam[nx][2].p, nullptr);
+    const int k = find_set(j);
+    int rc;
+    if (k >= 0) {
+      const double* stg[kStg];
+      for (int f = 0; f < kStg; ++f) stg[f] = sets[(size_t)kStg * k + f].p;
+      rc = step_vjp_reverse(op, wr.p, wu.p, wp.p, stg, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p, lam[nx][0].p,
+                            lam[nx][1].p, lam[nx][2].p);
+      set_step[k] = -1;
+      --sets_held;
+    } else {
+      rc = afl_imex_step_vjp(op, wr.p, wu.p, wp.p, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p, lam[nx][0].p,
+                             lam[nx][1].p, lam[nx][2].p, nullptr);
+      ++st.recomputed_steps;
+    }
     if (rc) return rc;
     cur = nx;
     ++st.adjoint_steps;
@@ -1878,18 +1960,18 @@ struct AdjRun {
     int rc;
     const int t = b - a;
     if (t == 1) {
-      if ((rc = advance(slot, 0))) return rc;
-      return vjp_from_working();
+      if ((rc = advance(slot, a, 0, false))) return rc;
+      return vjp_from_working(a);
     }
     if (s == 0) {
       for (int k = b - 1; k >= a; --k) {
-        if ((rc = advance(slot, k - a))) return rc;
-        if ((rc = vjp_from_working())) return rc;
+        if ((rc = advance(slot, a, k - a, false))) return rc;
+        if ((rc = vjp_from_working(k))) return rc;
       }
       return AFL_OK;
     }
     const int mm = sch.m(t, s);
-    if ((rc = advance(slot, mm))) return rc;
+    if ((rc = advance(slot, a, mm, true))) return rc;
     const int k = take();
     vec_copy(stream(), N, wr.p, sr(k));
     vec_copy(stream(), NU, wu.p, su(k));
@@ -1902,6 +1984,12 @@ struct AdjRun {
 
 }  // namespace
 }  // extern "C++"
+
+int afl_set_adjoint_stage_sets(afl_operator* op, int keep) {
+  if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null argument");
+  op->adj_keep_stages = keep != 0;
+  return AFL_OK;
+}
 
 int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, const double* p0, int nsteps,
                     int snapshots, const double* rho_bar_T, const double* u_bar_T, const double* p_bar_T,
@@ -1985,6 +2073,7 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
   const int s = std::min(snapshots, nsteps - 1);
   AdjRun R;
   R.op = op; R.N = N; R.NU = NU;
+  R.keep_stages = op->adj_keep_stages;
   R.snap.resize((size_t)3 * (s + 1));
   int rc;
   for (int k = 0; k <= s; ++k)
@@ -2014,6 +2103,7 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
   R.st.snapshots_used = R.max_used;
   if (stats) *stats = R.st;
   for (DevBuf& b : R.snap) b.release();
+  for (DevBuf& b : R.sets) b.release();
   for (DevBuf* b : {&R.wr, &R.wu, &R.wp}) b->release();
   for (auto& l : R.lam)
     for (DevBuf& b : l) b.release();

@@ -171,6 +171,7 @@ int64_t l2_blocks(const OpCtx& o, int64_t max_blocks);
 void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);
 void vec_axpby(hipStream_t st, int64_t n, double a, const double* x, double b, double* y);  // y = a x + b y
 void vec_sub_mul(hipStream_t st, int64_t n, const double* b, const double* y, const double* d, double* out);  // (b-y)*d
+void vec_sub(hipStream_t st, int64_t n, const double* a, const double* b, double* out);  // out = a - b
 void vec_mul(hipStream_t st, int64_t n, const double* x, const double* d, double* out);
 void vec_addmul(hipStream_t st, int64_t n, const double* d, const double* z, double* x);  // x += d*z
 void vec_scale_by_inv(hipStream_t st, int64_t n, const double* s, double* x);  // x /= *s

@@ -75,6 +75,7 @@ def load_library():
     L.afl_observe_theta.argtypes = [vp, C.POINTER(HillState), dp, dp, dp]
     L.afl_observe_theta_transpose.argtypes = [vp, C.POINTER(HillState), dp, dp, dp, dp, dp]
     L.afl_adjoint_run.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(AdjointStats)]
+    L.afl_set_adjoint_stage_sets.argtypes = [vp, C.c_int]
     L.afl_imex_step_vjp.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.POINTER(StepInfo)]
     for n in ("afl_vmult_enthalpy_coefficient_transpose", "afl_vjp_rho_for_fixed",
               "afl_vmult_enthalpy_tangent", "afl_vmult_enthalpy_tangent_transpose", "afl_vmult_schur_tangent",
@@ -252,7 +253,12 @@ class EulerOperator:
                                          self._v(ub), self._s(pb),
                                          C.byref(st)))
         return rb, ub, pb, {"forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
-                            "snapshots_used": st.snapshots_used}
+                            "snapshots_used": st.snapshots_used, "recomputed_steps": st.recomputed_steps,
+                            "stage_sets_used": st.stage_sets_used}
+
+    def set_adjoint_stage_sets(self, keep):
+        """Whether adjoint_run keeps the stage vectors of each advance's last step (default True)."""
+        self._chk(self.L.afl_set_adjoint_stage_sets(self.h, 1 if keep else 0))
 
     # ---- observation operator -------------------------------------------------------------------
     def set_observation_points(self, xyz):

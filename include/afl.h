@@ -156,11 +156,14 @@ int afl_imex_step_vjp(afl_operator* op, const double* rho, const double* u, cons
  * (rho_bar_0, ...) = (d x_T / d x_0)^T (rho_bar_T, ...), x_T = nsteps steps of afl_imex_step from
  * (rho0, u0, p0). States are kept in at most `snapshots` device snapshots besides the initial one
  * (each 2+dim fields) and recomputed from them with binomial checkpointing (Griewank), with the
- * split points that minimise the forward steps for that memory; each reverse step recomputes its
- * own step to restore the stage vectors. snapshots = AFL_ADJOINT_STORE_ALL instead keeps every
- * step's state and stage vectors from one forward sweep (no recomputation; fails with
- * AFL_ERR_DEVICE when they do not fit). stats may be NULL. No reference counterpart (the
- * reference has no adjoint). */
+ * split points that minimise the forward steps for that memory. Each advance to a split point also
+ * keeps the stage vectors of its last step (a stage set: 5 + 3 dim fields; at most snapshots + 1
+ * held), so that step is reversed without recomputing it; the other reverse steps recompute their
+ * own step to restore the stage vectors (afl_set_adjoint_stage_sets(op, 0): every reverse step
+ * does, and no stage set is held). snapshots = AFL_ADJOINT_STORE_ALL instead keeps every step's
+ * state and stage vectors from one forward sweep (no recomputation; fails with AFL_ERR_DEVICE when
+ * they do not fit). stats may be NULL. No reference counterpart (the reference has no adjoint). */
+int afl_set_adjoint_stage_sets(afl_operator* op, int keep);
 int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, const double* p0, int nsteps,
                     int snapshots, const double* rho_bar_T, const double* u_bar_T, const double* p_bar_T,
                     double* rho_bar_0, double* u_bar_0, double* p_bar_0, afl_adjoint_stats* stats);

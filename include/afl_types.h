@@ -80,9 +80,11 @@ typedef struct afl_step_info {
 
 /* Counters of an adjoint run with binomial checkpointing (afl_adjoint_run). */
 typedef struct afl_adjoint_stats {
-  int32_t forward_steps;   /* IMEX steps taken to reach checkpoints and re-reach states */
-  int32_t adjoint_steps;   /* afl_imex_step_vjp calls (each also recomputes its forward step) */
-  int32_t snapshots_used;  /* most snapshots held at once (including the initial state) */
+  int32_t forward_steps;     /* IMEX steps taken to reach checkpoints and re-reach states */
+  int32_t adjoint_steps;     /* steps reversed */
+  int32_t snapshots_used;    /* most snapshots held at once (including the initial state) */
+  int32_t recomputed_steps;  /* reversed steps that recomputed their forward step (no stage set held) */
+  int32_t stage_sets_used;   /* most stage sets held at once (8 fields each: 5 scalar + 3 velocity) */
   int32_t pad_;
 } afl_adjoint_stats;
 

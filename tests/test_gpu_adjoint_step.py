@@ -181,13 +181,49 @@ def _binomial_cost(t, s, memo={}):
     return memo[key]
 
 
-@pytest.mark.parametrize("snapshots", [0, 1, 2, 7, -1])
-def test_adjoint_run_checkpointing_matches_stored_states(snapshots):
+def _reversal_recomputes(t, s):
+    """Replays the reversal order of the checkpointed run (split = the first optimal m) with a stage
+    set kept at the end of every advance to a split point: returns (steps reversed without a set,
+    most sets held at once). The reversal must visit the steps in reverse order."""
+    held, order, most, recomputed = set(), [], [0], [0]
+
+    def split(t, s):
+        return min(range(1, t), key=lambda m: m + _binomial_cost(t - m, s - 1) + _binomial_cost(m, s))
+
+    def leaf(j):
+        order.append(j)
+        if j in held:
+            held.remove(j)
+        else:
+            recomputed[0] += 1
+
+    def rev(a, b, s):
+        if b - a == 1:
+            return leaf(a)
+        if s == 0:
+            for k in range(b - 1, a - 1, -1):
+                leaf(k)
+            return
+        m = split(b - a, s)
+        held.add(a + m - 1)
+        most[0] = max(most[0], len(held))
+        rev(a + m, b, s - 1)
+        rev(a, a + m, s)
+
+    rev(0, t, s)
+    assert order == list(range(t - 1, -1, -1))
+    return recomputed[0], most[0]
+
+
+@pytest.mark.parametrize("snapshots,keep", [(0, True), (1, True), (2, True), (7, True), (2, False), (7, False),
+                                            (-1, True)])
+def test_adjoint_run_checkpointing_matches_stored_states(snapshots, keep):
     """Seven steps reversed with binomial checkpointing (or with every step's stages kept, -1) give
     the gradient of the chain of step adjoints over stored states (to the solver tolerance), with
-    the optimal number of recomputed forward steps."""
+    the optimal number of recomputed forward steps; with stage sets kept, only the steps reversed
+    without one recompute their forward step."""
     import torch
-    say(f"[gpu adjoint run] 7 steps, {snapshots} snapshots, hill 32x12 k=2")
+    say(f"[gpu adjoint run] 7 steps, {snapshots} snapshots, stage sets {keep}, hill 32x12 k=2")
     mesh = make_mesh(dim=2, degree=2, n_cells=(32, 1, 12), x_range=(-36e3, 36e3), z_top=12e3,
                      hill_height=400.0, hill_width=1000.0, periodic=(1, 1))
     op = EulerOperator(mesh, mach=1.0, froude=FR, dt=2.0)
@@ -207,6 +243,7 @@ def test_adjoint_run_checkpointing_matches_stored_states(snapshots):
     lam = c
     for n in reversed(range(nsteps)):
         lam = op.imex_step_vjp(*xs[n], *lam)[:3]
+    op.set_adjoint_stage_sets(keep)
     rb, ub, pb, stats = op.adjoint_run(rho, u, p, nsteps, snapshots, *c)
     say(f"  stats {stats}")
     # the adjoint stage solves start from the previous call's multiplier, so the two runs agree to
@@ -222,6 +259,12 @@ def test_adjoint_run_checkpointing_matches_stored_states(snapshots):
         s = min(snapshots, nsteps - 1)
         assert stats["forward_steps"] == _binomial_cost(nsteps, s)
         assert stats["snapshots_used"] <= s + 1
+        if keep:
+            recomputed, most = _reversal_recomputes(nsteps, s)
+            assert (stats["recomputed_steps"], stats["stage_sets_used"]) == (recomputed, most)
+            assert most <= s + 1
+        else:
+            assert stats["recomputed_steps"] == nsteps and stats["stage_sets_used"] == 0
 
 
 def test_step_vjp_with_fp32_multigrid_solves(oracle_mod):
