@@ -8,6 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libafl_hip.so")
+PROBE_SRC = os.path.join(HERE, "probe", "afl_probe.hip")
+PROBE_LIB = os.path.join(HERE, "libafl_probe.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AFL_OFFLOAD_ARCH", "gfx950")
 
@@ -39,8 +41,14 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(LIB + ".tmp", LIB)
+    # the measurement probe (bench.py's mfma leg): a library of its own, not part of the operator ABI
+    cmd = [HIPCC, *FLAGS, "-shared", PROBE_SRC, "-o", PROBE_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {PROBE_SRC}:\n{r.stderr[-6000:]}")
+    os.replace(PROBE_LIB + ".tmp", PROBE_LIB)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PROBE_LIB}")
     return LIB
 
 

@@ -335,7 +335,9 @@ int current_device_scratch(double** out) {
 
 constexpr int kMaxRestart = 60;
 // device scalar slots: [0,64) first CGS pass, [64,128) second pass, 128 norm, 129.. misc
-constexpr int kScH1 = 0, kScH2 = 64, kScNrm = 128, kScA = 129, kScB = 130, kScSize = 160;
+// device/pinned scalar slots: CGS2 columns, norms, reduction results, and the Krylov coefficients y
+// (written by the host into pinned memory; the stream reads them before any later fetch returns)
+constexpr int kScH1 = 0, kScH2 = 64, kScNrm = 128, kScA = 129, kScB = 130, kScY = 160, kScSize = 224;
 
 int ensure_common(afl_operator* op) {
   int rc;
@@ -381,7 +383,7 @@ int ensure_dinv(afl_operator* op) {
 // Sx0 (optional): apply(x) already computed by the caller, used for the first residual.
 template <class Apply>
 int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double tol, int max_it, int m, int* iters,
-             double* res_out, const char* what, const double* Sx0 = nullptr) {
+             double* res_out, const char* what, const double* Sx0 = nullptr, bool tol_rel = false) {
   int rc;
   const int64_t N = op->N();
   hipStream_t st = op->o.stream;
@@ -404,7 +406,12 @@ int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double
     Sx0 = nullptr;
     vec_sub(st, N, b, Ax, V);
     vec_norm2(st, N, V, op->red.p, sc + kScNrm);
-    if ((rc = fetch(op, kScNrm, 1))) return rc;
+    // tol_rel: tol is relative to the norm the caller left in slot kScA, fetched with this one
+    if ((rc = fetch(op, kScNrm, tol_rel ? 2 : 1))) return rc;
+    if (tol_rel) {
+      tol *= hs[kScA];
+      tol_rel = false;
+    }
     const double beta = hs[kScNrm];
     res = beta;
     if (!(beta > tol)) break;
@@ -454,13 +461,14 @@ int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double
       y[i] = v / H[(size_t)i * m + i];
     }
     if (j > 0) {
-      if (hipMemcpyAsync(op->ydev.p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+      // y goes through pinned memory, which the host rewrites only after a later fetch has waited
+      // for the stream, so the copy needs no wait of its own
+      std::copy(y.begin(), y.begin() + j, hs + kScY);
+      if (hipMemcpyAsync(op->ydev.p, hs + kScY, j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(AFL_ERR_DEVICE, "hipMemcpyAsync failed");
       vec_fill(st, N, 0.0, z);
       vec_maxpy(st, N, V, N, j, op->ydev.p, 1.0, z);
       vec_addmul(st, N, op->dinv_D.p, z, x);
-      // y must stay alive until the copy has been consumed
-      if (hipStreamSynchronize(st) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
     }
     if (conv) break;
     if (it >= max_it) {
@@ -474,19 +482,22 @@ int gmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double
 }
 
 int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out,
-          const double* Sx0) {
+          const double* Sx0, bool tol_rel) {
   return gmres_op(
       op, [op](const double* in, double* out) { schur_apply(op, in, out); }, x, b, tol, max_it, m, iters, res_out,
-      "Schur GMRES did not converge", Sx0);
+      "Schur GMRES did not converge", Sx0, tol_rel);
 }
 
 // p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
-// (coarsest-level Chebyshev degree: 16 in 2D, 5 in 3D unless set; on the 3D hill at 48x48x20,
-// k = 3 the solves take one FGMRES iteration from degree 5 up, so a higher degree only costs)
+// (coarsest-level Chebyshev degree unless set: 5 in 3D, where the hill solves at 48x48x20, k = 3 take
+// one FGMRES iteration from degree 5 up; in 2D 16 at k = 4 (512x100 hill: 16 fastest of 10..32) and
+// 8 at k <= 3 (256x60, k = 3 hill: forward steps within 4 % of the fastest degree, 12, and the
+// adjoint stage solves, which reduce their residual by ten orders from the previous step's
+// multiplier, cheapest from 6 to 10))
 MgConfig mg_config(const afl_solver_params& sp, int dim, int k) {
   MgConfig c;
   if (sp.mg_smoother_degree > 0) c.smoother_degree = sp.mg_smoother_degree;
-  c.coarse_degree = sp.mg_coarse_degree > 0 ? sp.mg_coarse_degree : (dim == 2 ? 16 : 5);
+  c.coarse_degree = sp.mg_coarse_degree > 0 ? sp.mg_coarse_degree : (dim == 2 ? (k >= 4 ? 16 : 8) : 5);
   if (sp.mg_levels > 0) c.max_levels = sp.mg_levels;
   if (sp.mg_smoothing_range > 1.0) c.smoothing_range = sp.mg_smoothing_range;
   if (sp.mg_cell_tol > 0.0) c.cell_tol = sp.mg_cell_tol;
@@ -543,7 +554,7 @@ int ensure_mg(afl_operator* op) {
 // Sx0 (optional): apply(x) already computed by the caller, used for the first residual.
 template <class Apply>
 int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, double tol, int max_it, int m, int* iters,
-              double* res_out, const char* what, const double* Sx0 = nullptr) {
+              double* res_out, const char* what, const double* Sx0 = nullptr, bool tol_rel = false) {
   int rc;
   const int64_t N = op->N();
   hipStream_t st = op->o.stream;
@@ -568,7 +579,12 @@ int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, doubl
     Sx0 = nullptr;
     vec_sub(st, N, b, Ax, V);
     vec_norm2(st, N, V, op->red.p, sc + kScNrm);
-    if ((rc = fetch(op, kScNrm, 1))) return rc;
+    // tol_rel: tol is relative to the norm the caller left in slot kScA, fetched with this one
+    if ((rc = fetch(op, kScNrm, tol_rel ? 2 : 1))) return rc;
+    if (tol_rel) {
+      tol *= hs[kScA];
+      tol_rel = false;
+    }
     const double beta = hs[kScNrm];
     res = beta;
     if (!(beta > tol)) break;
@@ -620,10 +636,11 @@ int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, doubl
       y[i] = v / H[(size_t)i * m + i];
     }
     if (j > 0) {
-      if (hipMemcpyAsync(op->ydev.p, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+      // y through pinned memory, as in gmres_op
+      std::copy(y.begin(), y.begin() + j, hs + kScY);
+      if (hipMemcpyAsync(op->ydev.p, hs + kScY, j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(AFL_ERR_DEVICE, "hipMemcpyAsync failed");
       vec_maxpy(st, N, Z, N, j, op->ydev.p, 1.0, x);
-      if (hipStreamSynchronize(st) != hipSuccess) return fail(AFL_ERR_DEVICE, "hipStreamSynchronize failed");
     }
     if (conv) break;
     if (it >= max_it) {
@@ -637,10 +654,10 @@ int fgmres_op(afl_operator* op, Apply&& apply, double* x, const double* b, doubl
 }
 
 int fgmres(afl_operator* op, double* x, const double* b, double tol, int max_it, int m, int* iters, double* res_out,
-           const double* Sx0) {
+           const double* Sx0, bool tol_rel) {
   return fgmres_op(
       op, [op](const double* in, double* out) { schur_apply(op, in, out); }, x, b, tol, max_it, m, iters, res_out,
-      "Schur FGMRES did not converge", Sx0);
+      "Schur FGMRES did not converge", Sx0, tol_rel);
 }
 
 bool valid_op(afl_operator* op) { return op != nullptr; }
@@ -1421,15 +1438,14 @@ static int schur_solve(afl_operator* op, double* p, const double* rhs, const dou
                        double* residual) {
   int rc;
   if ((rc = ensure_common(op))) return rc;
+  // the tolerance eps * |rhs|: the norm is fetched together with the first residual norm
   vec_norm2(op->o.stream, op->o.N, rhs, op->red.p, op->dsc.p + kScA);
-  if ((rc = fetch(op, kScA, 1))) return rc;
-  const double tol = op->sp.gmres_eps * op->host_sc[kScA];
   int it = 0;
   double res = 0.0;
   if (op->sp.schur_precond == AFL_SCHUR_JACOBI)
-    rc = gmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res, Sp0);
+    rc = gmres(op, p, rhs, op->sp.gmres_eps, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res, Sp0, true);
   else
-    rc = fgmres(op, p, rhs, tol, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res, Sp0);
+    rc = fgmres(op, p, rhs, op->sp.gmres_eps, op->sp.gmres_max_iter, op->sp.gmres_restart, &it, &res, Sp0, true);
   if (iters) *iters = it;
   if (residual) *residual = res;
   return rc;
@@ -1642,12 +1658,11 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
     vec_axpby(st, NU, -1.0, t2, 1.0, t1);
     launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, t2);
     launch_apply_Bt(o, t2, out);
+    // out = D in - out - C_pf(w_B)^T in + C_pf(w_E)^T in (the VJP output buffers are free here)
     launch_mass_apply(o, nullptr, 1.0 / (kGamma - 1.0), 1, in, t3);
-    vec_axpby(st, N, 1.0, t3, -1.0, out);
-    launch_apply_C_coef_t(o, wB, in, t3);
-    vec_axpby(st, N, -1.0, t3, 1.0, out);
-    launch_apply_C_coef_t(o, wE, in, t3);
-    vec_axpby(st, N, 1.0, t3, 1.0, out);
+    launch_apply_C_coef_t(o, wB, in, outs[0]);
+    launch_apply_C_coef_t(o, wE, in, outs[1]);
+    vec_comb3(st, N, t3, outs[0], outs[1], out);
   };
   int rc;
   if ((rc = ensure_common(op))) return rc;
@@ -1682,7 +1697,11 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
   vec_copy(st, NU, lamu, B[29]);
   vec_axpby(st, NU, -1.0, t2, 1.0, B[29]);
   launch_vjp_rhs_u(o, stage, B[29], e2s, outs);
-  for (int i = 0; i < n2; ++i) S.acc(tgt[i], outs[i], i % 3 == 1 ? NU : N);
+  {
+    AccList L;
+    for (int i = 0; i < n2; ++i) L.add(tgt[i], outs[i], i % 3 == 1 ? NU : N);
+    vec_acc_multi(st, L);
+  }
   // rho_new through A(rho_new): + int phi (A^{-1} C'(w_E)^T lam_p) . w_E - int phi lam_u . u*
   launch_dot_mass(o, t2, wE, t3);
   S.acc(tgt[rnew_pos], t3, N);
@@ -1695,9 +1714,13 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
   vec_axpby(st, N, -1.0, t3, 1.0, tgt[rnew_pos]);
   // E3 sources except u_fix: VJP of lam_p
   launch_vjp_rhs_p(o, stage, lamp, e3s, outs);
-  for (int i = 0; i < n3 - 1; ++i) {
-    const bool vel = i < 3 * stage ? (i % 3 == 1) : false;
-    S.acc(tgt[i], outs[i], vel ? NU : N);
+  {
+    AccList L;
+    for (int i = 0; i < n3 - 1; ++i) {
+      const bool vel = i < 3 * stage ? (i % 3 == 1) : false;
+      L.add(tgt[i], outs[i], vel ? NU : N);
+    }
+    vec_acc_multi(st, L);
   }
   return AFL_OK;
 }
@@ -1737,7 +1760,6 @@ int step_vjp_reverse(afl_operator* op, const double* rho, const double* u, const
     const double zero[3] = {0.0, 0.0, 0.0};
     launch_sponge(o, op->sponge.z_start, op->sponge.alpha_max * o.dt, zero, ubc);
   }
-  auto acc = [&](double* dst, const double* src, int64_t n) { S.acc(dst, src, n); };
   // ---- stage 3: pc = D^{-1} E3, uc = A(rho_c)^{-1} E2, rho_c = M^{-1} E1
   o.hyp = 3;
   vec_copy(st, N, rc_, op->rho_fix.p);
@@ -1748,7 +1770,9 @@ int step_vjp_reverse(afl_operator* op, const double* rho, const double* u, const
     const double* s11[11] = {rho, u, p, r2, u2, p2, r3, u3, p3, rc_, uc};
     launch_vjp_rhs_p(o, 3, e, s11, outs);
     double* tg[11] = {rho_bar, u_bar, p_bar, rb2, ub2, pb2, rb3, ub3, pb3, rbc, ubc};
-    for (int i = 0; i < 11; ++i) acc(tg[i], outs[i], (i % 3 == 1 && i < 9) || i == 10 ? NU : N);
+    AccList L;
+    for (int i = 0; i < 11; ++i) L.add(tg[i], outs[i], (i % 3 == 1 && i < 9) || i == 10 ? NU : N);
+    vec_acc_multi(st, L);
   }
   launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, ubc, eU);
   launch_dot_mass(o, eU, uc, t3);
@@ -1757,14 +1781,18 @@ int step_vjp_reverse(afl_operator* op, const double* rho, const double* u, const
     const double* s9[9] = {rho, u, p, r2, u2, p2, r3, u3, p3};
     launch_vjp_rhs_u(o, 3, eU, s9, outs);
     double* tg[9] = {rho_bar, u_bar, p_bar, rb2, ub2, pb2, rb3, ub3, pb3};
-    for (int i = 0; i < 9; ++i) acc(tg[i], outs[i], i % 3 == 1 ? NU : N);
+    AccList L;
+    for (int i = 0; i < 9; ++i) L.add(tg[i], outs[i], i % 3 == 1 ? NU : N);
+    vec_acc_multi(st, L);
   }
   launch_mass_solve(o, nullptr, 1.0, 1, rbc, e);
   {
     const double* s6[6] = {rho, u, r2, u2, r3, u3};
     launch_vjp_rhs_rho(o, 3, e, s6, outs);
     double* tg[6] = {rho_bar, u_bar, rb2, ub2, rb3, ub3};
-    for (int i = 0; i < 6; ++i) acc(tg[i], outs[i], i % 2 == 1 ? NU : N);
+    AccList L;
+    for (int i = 0; i < 6; ++i) L.add(tg[i], outs[i], i % 2 == 1 ? NU : N);
+    vec_acc_multi(st, L);
   }
   // ---- stage 2: fixed point (p3, u3) with rho_new = rho3, then rho3 = M^{-1} E1
   {
@@ -1779,7 +1807,9 @@ int step_vjp_reverse(afl_operator* op, const double* rho, const double* u, const
     const double* s4[4] = {rho, u, r2, u2};
     launch_vjp_rhs_rho(o, 2, e, s4, outs);
     double* tg[4] = {rho_bar, u_bar, rb2, ub2};
-    for (int i = 0; i < 4; ++i) acc(tg[i], outs[i], i % 2 == 1 ? NU : N);
+    AccList L;
+    for (int i = 0; i < 4; ++i) L.add(tg[i], outs[i], i % 2 == 1 ? NU : N);
+    vec_acc_multi(st, L);
   }
   // ---- stage 1: fixed point (p2, u2) with rho_new = rho2, then rho2 = M^{-1} E1
   {
@@ -1793,8 +1823,10 @@ int step_vjp_reverse(afl_operator* op, const double* rho, const double* u, const
   {
     const double* s2[2] = {rho, u};
     launch_vjp_rhs_rho(o, 1, e, s2, outs);
-    acc(rho_bar, outs[0], N);
-    acc(u_bar, outs[1], NU);
+    AccList L;
+    L.add(rho_bar, outs[0], N);
+    L.add(u_bar, outs[1], NU);
+    vec_acc_multi(st, L);
   }
   return AFL_OK;
 }

@@ -67,7 +67,9 @@ struct OpCtx {
   unsigned hyp = 1;       // HYPERBOLIC_stage
   TableauC tab{};
   hipStream_t stream = nullptr;
-  double cell_tol = 1e-15;  // relative tolerance of the cell-local mass-type solves
+  // relative tolerance of the cell-local mass-type solves (the reference's inner A solve: CG to 1e-12,
+  // euler_operator.h:2391); on the 3D hill it takes 3 Chebyshev steps in most cells, 4 at 1e-15
+  double cell_tol = 1e-13;
   int cell_maxit = 40;
   KernelTimer* timer = nullptr;  // set while afl_kernel_timer_* is active
 };
@@ -172,6 +174,22 @@ void vec_copy(hipStream_t st, int64_t n, const double* x, double* y);
 void vec_axpby(hipStream_t st, int64_t n, double a, const double* x, double b, double* y);  // y = a x + b y
 void vec_sub_mul(hipStream_t st, int64_t n, const double* b, const double* y, const double* d, double* out);  // (b-y)*d
 void vec_sub(hipStream_t st, int64_t n, const double* a, const double* b, double* out);  // out = a - b
+// dst[p][0..n[p]) += src[p][...] for p < count, in one launch (distinct dst buffers)
+struct AccList {
+  static constexpr int kMax = 12;
+  double* dst[kMax];
+  const double* src[kMax];
+  int64_t n[kMax];
+  int count = 0;
+  void add(double* d, const double* s, int64_t len) {
+    dst[count] = d;
+    src[count] = s;
+    n[count] = len;
+    ++count;
+  }
+};
+void vec_acc_multi(hipStream_t st, const AccList& L);
+void vec_comb3(hipStream_t st, int64_t n, const double* a, const double* b, const double* c, double* y);  // y = a - y - b + c
 void vec_mul(hipStream_t st, int64_t n, const double* x, const double* d, double* out);
 void vec_addmul(hipStream_t st, int64_t n, const double* d, const double* z, double* x);  // x += d*z
 void vec_scale_by_inv(hipStream_t st, int64_t n, const double* s, double* x);  // x /= *s

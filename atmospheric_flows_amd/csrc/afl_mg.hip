@@ -206,6 +206,14 @@ __global__ void k_vec_scale(int64_t n, R a, const R* __restrict__ x, R* __restri
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = a * x[i];
 }
+// y = x / s[0] (x unchanged when s[0] is not positive), the norm s read on the device
+template <typename R>
+__global__ void k_vec_scale_inv_dev(int64_t n, const double* __restrict__ s, const R* __restrict__ x,
+                                    R* __restrict__ y) {
+  const R a = (R)(1.0 / (s[0] > 0 ? s[0] : 1.0));
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
 template <typename R>
 __global__ void k_vec_fill(int64_t n, R v, R* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -520,7 +528,6 @@ int MgT<R>::estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, d
     } else {
       its = warm_eig_iters < iters ? warm_eig_iters : iters;
     }
-    double lam = 1.0;
     for (int it = 0; it < its; ++it) {
       level_schur(L, fine, L.ev, L.t);
       level_pinv(L, L.t, L.z);
@@ -529,12 +536,14 @@ int MgT<R>::estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, d
       vec_norm2(L.stream, L.N, work_d, work_d + L.N, scal_d);
       k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.ev, work_d);
       vec_norm2(L.stream, L.N, work_d, work_d + L.N, scal_d + 1);
+      // x = z / |z| with the norm read on the device: the host waits only for the last quotient
+      k_vec_scale_inv_dev<R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, scal_d, L.z, L.ev);
+    }
+    double lam = 1.0;
+    if (its > 0) {
       if (hipMemcpyAsync(scal_h, scal_d, 2 * sizeof(double), hipMemcpyDeviceToHost, L.stream) != hipSuccess) return 3;
       if (hipStreamSynchronize(L.stream) != hipSuccess) return 3;
       lam = scal_h[1] > 0 ? scal_h[0] / scal_h[1] : 1.0;
-      // x = z / |z|
-      k_vec_scale<R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, (R)(1.0 / (scal_h[0] > 0 ? scal_h[0] : 1.0)),
-                                                                      L.z, L.ev);
     }
     L.lam_max = lam > 1.0 ? lam : 1.0;
     L.ev_valid = true;

@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5, help="timed IMEX steps per path")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reps", type=int, default=20, help="isolated-apply repetitions")
-    ap.add_argument("--legs", default="c3,applies,kernels,c2,adjoint",
-                    help="comma list of c3 (headline, always run), applies, kernels, c2, adjoint")
+    ap.add_argument("--legs", default="c3,applies,kernels,c2,adjoint,mfma",
+                    help="comma list of c3 (headline, always run), applies, kernels, c2, adjoint, mfma, c5")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-snapshots", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -357,6 +357,10 @@ def main():
         log("c5 shard leg: 24x192x64 cells, k=4")
         res["c5_shard"] = c5_shard_leg(torch, EulerOperator, A, log)
 
+    if "mfma" in legs:
+        log("MFMA vs VALU contraction probe")
+        res["mfma_probe"] = mfma_probe_leg(torch)
+
     if not args.no_cpu:
         log("CPU baseline")
         res["cpu_baseline"] = run_cpu_subprocess()
@@ -407,6 +411,72 @@ def adjoint_leg(args, torch, EulerOperator, A):
 
 
 A_STORE_ALL = -1
+
+
+def mfma_probe_leg(torch, reps=10000, waves=8192):
+    """FP64 throughput of the 3D degree-3 sum-factorisation contraction (the 7x4 value and derivative
+    tables on Gauss-Lobatto nodes -> 7 Gauss points, applied to the 4 nodal values of a line) on the
+    vector ALUs (v_fma_f64, lane = line) and on the matrix cores (v_mfma_f64_16x16x4f64, [S; D] as the
+    16x4 A operand with 14 rows used, 16 lines per instruction), from libafl_probe.so. The MFMA output
+    placement is checked on known operands first."""
+    import ctypes as C
+    import numpy as np
+    lib = C.CDLL(os.path.join(ROOT, "atmospheric_flows_amd", "libafl_probe.so"))
+    vp = C.c_void_p
+    lib.afl_probe_set_tables.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.afl_probe_contraction.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, C.POINTER(C.c_float)]
+    lib.afl_probe_mfma_layout.argtypes = [vp, vp, vp]
+    # k = 3 tables: Lagrange basis on the Gauss-Lobatto nodes of [0, 1], values and derivatives at the
+    # 7 Gauss points
+    xn = np.array([0.0, 0.5 - 0.5 / 5 ** 0.5, 0.5 + 0.5 / 5 ** 0.5, 1.0])
+    xq = 0.5 * (np.polynomial.legendre.leggauss(7)[0] + 1.0)
+    S = np.zeros((7, 4))
+    D = np.zeros((7, 4))
+    for i in range(4):
+        c = np.poly1d([1.0])
+        for j in range(4):
+            if j != i:
+                c = c * np.poly1d([1.0, -xn[j]]) / (xn[i] - xn[j])
+        S[:, i], D[:, i] = c(xq), c.deriv()(xq)
+    S, D = np.ascontiguousarray(S), np.ascontiguousarray(D)
+    assert lib.afl_probe_set_tables(S.ctypes.data_as(C.POINTER(C.c_double)), D.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    # output placement of v_mfma_f64_16x16x4f64 on known operands: every (lane, register) output is
+    # matched to the entry of A B it equals, and the placement must be a bijection onto the 16x16 result
+    rng = np.random.default_rng(5)
+    a, b = rng.standard_normal((16, 4)), rng.standard_normal((4, 16))
+    da, db = torch.from_numpy(a.ravel()).cuda(), torch.from_numpy(b.ravel()).cuda()
+    dd = torch.zeros(256, dtype=torch.float64, device="cuda")
+    assert lib.afl_probe_mfma_layout(da.data_ptr(), db.data_ptr(), dd.data_ptr()) == 0
+    got = dd.cpu().numpy().reshape(64, 4)
+    ref = a @ b
+    place, layout_err = {}, 0.0
+    for lane in range(64):
+        for j in range(4):
+            k = int(np.argmin(np.abs(ref - got[lane, j])))
+            layout_err = max(layout_err, float(np.abs(ref.flat[k] - got[lane, j])))
+            place[(lane, j)] = divmod(k, 16)
+    bijective = len(set(place.values())) == 256
+    rows_rule = all(place[(l, j)] == (l // 16 + 4 * j, l % 16) for l in range(64) for j in range(4))
+    tab = torch.from_numpy(np.concatenate([S.ravel(), D.ravel(), S.ravel(), D.ravel()])).cuda()
+    lines = waves * 64
+    x = torch.from_numpy(rng.uniform(-1.0, 1.0, lines * 4)).cuda()
+    out = torch.empty(lines, dtype=torch.float64, device="cuda")
+    res = {"workload": f"{lines} lines x {reps} repetitions of the k=3 value+derivative contraction (56 FMAs per line)",
+           "mfma_layout": {"max_err": layout_err, "bijective": bijective,
+                           "lane_l_reg_j_holds_row_l/16+4j_col_l%16": rows_rule,
+                           "sample": {f"lane{l}": [place[(l, j)] for j in range(4)] for l in (0, 1, 16, 17, 63)}}}
+    for mode, name in ((0, "valu"), (1, "mfma")):
+        ms = C.c_float()
+        assert lib.afl_probe_contraction(mode, 10, waves, tab.data_ptr(), x.data_ptr(), out.data_ptr(), C.byref(ms)) == 0
+        assert lib.afl_probe_contraction(mode, reps, waves, tab.data_ptr(), x.data_ptr(), out.data_ptr(), C.byref(ms)) == 0
+        useful = 2.0 * 56 * lines * reps
+        issued = 2.0 * (56 if mode == 0 else 64) * lines * reps
+        res[name] = {"ms": ms.value, "useful_tflops": useful / (ms.value * 1e-3) / 1e12,
+                     "issued_tflops": issued / (ms.value * 1e-3) / 1e12,
+                     "frac_of_fp64_peak_useful": useful / (ms.value * 1e-3) / 1e12 / FP64_PEAK_TFS,
+                     "checksum_finite": bool(torch.isfinite(out).all())}
+    res["mfma_over_valu_useful"] = res["mfma"]["useful_tflops"] / res["valu"]["useful_tflops"]
+    return res
 
 
 def c5_shard_leg(torch, EulerOperator, A, log, nsteps=20, npts=4096, snapshots=20):

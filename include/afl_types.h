@@ -56,7 +56,8 @@ typedef struct afl_solver_params {
    * Chebyshev smoothers run in FP32 / FP64. Non-positive MG fields take the defaults below. */
   int32_t schur_precond;
   int32_t mg_smoother_degree;  /* Chebyshev degree of pre/post smoothing [3] */
-  int32_t mg_coarse_degree;    /* Chebyshev degree on the coarsest level [16 in 2D, 5 in 3D] */
+  int32_t mg_coarse_degree;    /* Chebyshev degree on the coarsest level [2D: 16 at k = 4, 8 at k <= 3;
+                                * 3D: 5] */
   int32_t mg_levels;           /* maximum number of degree levels k, k/2, ..., 1 [1]; with one level the
                                 * V-cycle is the Chebyshev polynomial preconditioner on the fine level */
   double mg_smoothing_range;   /* lambda_max / lambda_min of the smoothed interval [15] */
