@@ -42,7 +42,9 @@ def build(verbose=False, jobs=None):
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(LIB + ".tmp", LIB)
     # the measurement probe (bench.py's mfma leg): a library of its own, not part of the operator ABI
-    cmd = [HIPCC, *FLAGS, "-shared", PROBE_SRC, "-o", PROBE_LIB + ".tmp"]
+    # MFMA accumulators in VGPRs (gfx950 reads and writes them there): without it the loop-carried
+    # accumulators are copied to and from AGPRs around every MFMA
+    cmd = [HIPCC, *FLAGS, "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-shared", PROBE_SRC, "-o", PROBE_LIB + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {PROBE_SRC}:\n{r.stderr[-6000:]}")

@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reps", type=int, default=20, help="isolated-apply repetitions")
     ap.add_argument("--legs", default="c3,applies,kernels,c2,adjoint,mfma",
-                    help="comma list of c3 (headline, always run), applies, kernels, c2, adjoint, mfma, c5")
+                    help="comma list of c3 (the headline; applies and kernels need it), applies, kernels, c2, adjoint, mfma, c5")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-snapshots", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -182,135 +182,136 @@ def main():
         return e / r
 
     res = {}
-    # ---- headline: 3D 96x96x40, k=3, FP32-MG FGMRES vs FP64 Jacobi GMRES vs FP64-MG FGMRES -------------
-    mesh, st = c3_workload()
-    N = A.n_per_field(mesh)
-    dim, dofs = 3, 5 * N
-    if args.calibrate:  # reads and writes exactly 4 N doubles (755 MB, beyond the 256 MB Infinity Cache)
-        buf = torch.ones(4 * N, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        buf.mul_(1.0)
-        torch.cuda.synchronize()
-        del buf
-    paths = {}
-    for name, pc in (("fp32_mg", A.AFL_SCHUR_MG_FP32), ("fp64_jacobi", A.AFL_SCHUR_JACOBI),
-                     ("fp64_mg", A.AFL_SCHUR_MG_FP64)):
-        log(f"c3 path {name}: {args.warmup} warm-up + {args.steps} timed steps")
-        t, infos, state, o = run_path(mesh, st, pc, 1e-8, args.steps, args.warmup)
-        paths[name] = {"t": t, "infos": infos, "state": state, "op": o}
-        if name != "fp32_mg":
-            del o
-            paths[name]["op"] = None
-    o32 = paths["fp32_mg"]["op"]
-    ref = paths["fp64_jacobi"]["state"]
-    th_ref = theta_prime(o32, st, ref[0], ref[2])
-    errs = {}
-    for name in ("fp32_mg", "fp64_mg"):
-        s = paths[name]["state"]
-        th = theta_prime(o32, st, s[0], s[2])
-        errs[name] = {"theta_prime": rel_l2(o32, 1, th, th_ref),
-                      "w": rel_l2(o32, 1, s[1][2 * N:].contiguous(), ref[1][2 * N:].contiguous()),
-                      "rho": rel_l2(o32, 1, s[0], ref[0]), "u": rel_l2(o32, dim, s[1], ref[1]),
-                      "p": rel_l2(o32, 1, s[2], ref[2])}
-    step_s = paths["fp32_mg"]["t"]
-    res.update({
-        "metric": "IMEX step wall-time, FP32-MG vs FP64 path at equal residual; DoF/s, % HBM peak",
-        "value": dofs / step_s,
-        "unit": "DoF/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": step_s * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "3D Agnesi hill 96x96x40 cells (144x144x25 km), k=3, FP64 state and explicit stages, "
-                               "FP64 flexible GMRES (true residual 1e-8) with an FP32 p-multigrid Chebyshev V-cycle, "
-                               "dt=1 s, seeded N(0, 0.01 K) theta noise",
-                   "n_per_field": N, "dof": dofs, "solver": "fgmres+fp32-mg"},
-        "fp_iters_per_step": [list(i.fp_iters) for i in paths["fp32_mg"]["infos"]],
-        "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp32_mg"]["infos"]],
-        "fp64_plain_path": {"ms_per_step": paths["fp64_jacobi"]["t"] * 1e3, "value": dofs / paths["fp64_jacobi"]["t"],
-                            "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp64_jacobi"]["infos"]]},
-        "fp64_mg_path": {"ms_per_step": paths["fp64_mg"]["t"] * 1e3, "value": dofs / paths["fp64_mg"]["t"],
-                         "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp64_mg"]["infos"]]},
-        "speedup_fp32mg_vs_fp64": paths["fp64_jacobi"]["t"] / step_s,
-        "speedup_fp32mg_vs_fp64mg": paths["fp64_mg"]["t"] / step_s,
-        "rel_l2_vs_fp64_path_after_steps": {**errs, "steps": args.steps + args.warmup},
-    })
-    for name in ("fp64_jacobi", "fp64_mg"):
-        paths[name]["state"] = None
+    if "c3" in legs:  # the headline (omitted only for runs that profile other legs)
+        # ---- headline: 3D 96x96x40, k=3, FP32-MG FGMRES vs FP64 Jacobi GMRES vs FP64-MG FGMRES -------------
+        mesh, st = c3_workload()
+        N = A.n_per_field(mesh)
+        dim, dofs = 3, 5 * N
+        if args.calibrate:  # reads and writes exactly 4 N doubles (755 MB, beyond the 256 MB Infinity Cache)
+            buf = torch.ones(4 * N, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            buf.mul_(1.0)
+            torch.cuda.synchronize()
+            del buf
+        paths = {}
+        for name, pc in (("fp32_mg", A.AFL_SCHUR_MG_FP32), ("fp64_jacobi", A.AFL_SCHUR_JACOBI),
+                         ("fp64_mg", A.AFL_SCHUR_MG_FP64)):
+            log(f"c3 path {name}: {args.warmup} warm-up + {args.steps} timed steps")
+            t, infos, state, o = run_path(mesh, st, pc, 1e-8, args.steps, args.warmup)
+            paths[name] = {"t": t, "infos": infos, "state": state, "op": o}
+            if name != "fp32_mg":
+                del o
+                paths[name]["op"] = None
+        o32 = paths["fp32_mg"]["op"]
+        ref = paths["fp64_jacobi"]["state"]
+        th_ref = theta_prime(o32, st, ref[0], ref[2])
+        errs = {}
+        for name in ("fp32_mg", "fp64_mg"):
+            s = paths[name]["state"]
+            th = theta_prime(o32, st, s[0], s[2])
+            errs[name] = {"theta_prime": rel_l2(o32, 1, th, th_ref),
+                          "w": rel_l2(o32, 1, s[1][2 * N:].contiguous(), ref[1][2 * N:].contiguous()),
+                          "rho": rel_l2(o32, 1, s[0], ref[0]), "u": rel_l2(o32, dim, s[1], ref[1]),
+                          "p": rel_l2(o32, 1, s[2], ref[2])}
+        step_s = paths["fp32_mg"]["t"]
+        res.update({
+            "metric": "IMEX step wall-time, FP32-MG vs FP64 path at equal residual; DoF/s, % HBM peak",
+            "value": dofs / step_s,
+            "unit": "DoF/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": step_s * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "3D Agnesi hill 96x96x40 cells (144x144x25 km), k=3, FP64 state and explicit stages, "
+                                   "FP64 flexible GMRES (true residual 1e-8) with an FP32 p-multigrid Chebyshev V-cycle, "
+                                   "dt=1 s, seeded N(0, 0.01 K) theta noise",
+                       "n_per_field": N, "dof": dofs, "solver": "fgmres+fp32-mg"},
+            "fp_iters_per_step": [list(i.fp_iters) for i in paths["fp32_mg"]["infos"]],
+            "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp32_mg"]["infos"]],
+            "fp64_plain_path": {"ms_per_step": paths["fp64_jacobi"]["t"] * 1e3, "value": dofs / paths["fp64_jacobi"]["t"],
+                                "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp64_jacobi"]["infos"]]},
+            "fp64_mg_path": {"ms_per_step": paths["fp64_mg"]["t"] * 1e3, "value": dofs / paths["fp64_mg"]["t"],
+                             "gmres_iters_per_step": [list(i.gmres_iters) for i in paths["fp64_mg"]["infos"]]},
+            "speedup_fp32mg_vs_fp64": paths["fp64_jacobi"]["t"] / step_s,
+            "speedup_fp32mg_vs_fp64mg": paths["fp64_mg"]["t"] / step_s,
+            "rel_l2_vs_fp64_path_after_steps": {**errs, "steps": args.steps + args.warmup},
+        })
+        for name in ("fp64_jacobi", "fp64_mg"):
+            paths[name]["state"] = None
 
-    # ---- isolated applies on the headline mesh (every working set is far beyond the Infinity Cache) ----
-    if "applies" in legs:
-        log("c3 isolated applies")
-        rho, u, p = paths["fp32_mg"]["state"]
-        o = o32
-        dst_s, dst_v = o.new_scalar(), o.new_vector()
-        o.set_HYPERBOLIC_stage(3)
-        r2, u2, p2 = rho.clone(), u.clone(), p.clone()
-        src_u = [rho, u, p, r2, u2, p2, rho, u, p]
-        ap = {}
-        fb = N * 8
-        ap["E2_stage3"] = (events_time(lambda i=0: o.vmult_rhs_velocity_fixed(dst_v, src_u), args.reps), 18 * fb)
-        src_p = src_u + [r2, u2]
-        ap["E3_stage3"] = (events_time(lambda i=0: o.vmult_rhs_pressure(dst_s, src_p), args.reps), 20 * fb)
-        src_r = [rho, u, r2, u2, rho, u]
-        ap["E1_stage3"] = (events_time(lambda i=0: o.vmult_rhs_rho_update(dst_s, src_r), args.reps), 13 * fb)
-        o.set_HYPERBOLIC_stage(1)
-        o.set_rho_for_fixed(rho)
-        o.set_pres_fixed(p)
-        o.set_NS_stage(2)
-        ap["schur_vmult_fp64"] = (events_time(lambda i=0: o.vmult(dst_s, p), args.reps), 11 * fb)
-        ap["schur_w_fp64"] = (events_time(lambda i=0: o.vmult_schur_velocity(dst_v, p), args.reps), 5 * fb)
-        ap["apply_B"] = (events_time(lambda i=0: o.vmult_pressure(dst_v, p), args.reps), 4 * fb)
-        ap["apply_C"] = (events_time(lambda i=0: o.vmult_enthalpy(dst_s, u), args.reps), 5 * fb)
-        o.set_NS_stage(3)
-        ap["A_solve"] = (events_time(lambda i=0: o.solve_mass(dst_v, u), args.reps), 7 * fb)
-        p32, w32 = p.float(), torch.empty(dim * N, device="cuda", dtype=torch.float32)
-        ap["schur_w_fp32_level"] = (events_time(lambda i=0: o.vmult_schur_velocity_f32(w32, p32), args.reps), 5 * N * 4)
-        o.set_NS_stage(2)
-        ap["fp32_vcycle"] = (events_time(lambda i=0: o.mg_vmult(dst_s, p), max(2, args.reps // 4)), None)
-        res["isolated_applies_c3"] = {k: {"us": t * 1e6, **({"GBs": b / t / 1e9, "hbm_frac": b / t / 1e9 / HBM_PEAK_GBS}
-                                                             if b else {})} for k, (t, b) in ap.items()}
-        del dst_s, dst_v, r2, u2, p2, p32, w32
-        torch.cuda.synchronize()
+        # ---- isolated applies on the headline mesh (every working set is far beyond the Infinity Cache) ----
+        if "applies" in legs:
+            log("c3 isolated applies")
+            rho, u, p = paths["fp32_mg"]["state"]
+            o = o32
+            dst_s, dst_v = o.new_scalar(), o.new_vector()
+            o.set_HYPERBOLIC_stage(3)
+            r2, u2, p2 = rho.clone(), u.clone(), p.clone()
+            src_u = [rho, u, p, r2, u2, p2, rho, u, p]
+            ap = {}
+            fb = N * 8
+            ap["E2_stage3"] = (events_time(lambda i=0: o.vmult_rhs_velocity_fixed(dst_v, src_u), args.reps), 18 * fb)
+            src_p = src_u + [r2, u2]
+            ap["E3_stage3"] = (events_time(lambda i=0: o.vmult_rhs_pressure(dst_s, src_p), args.reps), 20 * fb)
+            src_r = [rho, u, r2, u2, rho, u]
+            ap["E1_stage3"] = (events_time(lambda i=0: o.vmult_rhs_rho_update(dst_s, src_r), args.reps), 13 * fb)
+            o.set_HYPERBOLIC_stage(1)
+            o.set_rho_for_fixed(rho)
+            o.set_pres_fixed(p)
+            o.set_NS_stage(2)
+            ap["schur_vmult_fp64"] = (events_time(lambda i=0: o.vmult(dst_s, p), args.reps), 11 * fb)
+            ap["schur_w_fp64"] = (events_time(lambda i=0: o.vmult_schur_velocity(dst_v, p), args.reps), 5 * fb)
+            ap["apply_B"] = (events_time(lambda i=0: o.vmult_pressure(dst_v, p), args.reps), 4 * fb)
+            ap["apply_C"] = (events_time(lambda i=0: o.vmult_enthalpy(dst_s, u), args.reps), 5 * fb)
+            o.set_NS_stage(3)
+            ap["A_solve"] = (events_time(lambda i=0: o.solve_mass(dst_v, u), args.reps), 7 * fb)
+            p32, w32 = p.float(), torch.empty(dim * N, device="cuda", dtype=torch.float32)
+            ap["schur_w_fp32_level"] = (events_time(lambda i=0: o.vmult_schur_velocity_f32(w32, p32), args.reps), 5 * N * 4)
+            o.set_NS_stage(2)
+            ap["fp32_vcycle"] = (events_time(lambda i=0: o.mg_vmult(dst_s, p), max(2, args.reps // 4)), None)
+            res["isolated_applies_c3"] = {k: {"us": t * 1e6, **({"GBs": b / t / 1e9, "hbm_frac": b / t / 1e9 / HBM_PEAK_GBS}
+                                                                 if b else {})} for k, (t, b) in ap.items()}
+            del dst_s, dst_v, r2, u2, p2, p32, w32
+            torch.cuda.synchronize()
 
-    # ---- per-kernel times inside the FP32-preconditioned steps; roofline of the dominant kernel --------
-    if "kernels" in legs:
-        log("c3 per-kernel timing inside FP32-MG steps")
-        state = paths["fp32_mg"]["state"]
-        mask = 0
-        for f in A.TIMED_KERNELS:
-            mask |= f
-        o32.kernel_timer_start(mask, 2000 + 1000 * args.steps)
-        for _ in range(args.steps):
-            o32.imex_step(*state)
-        o32.kernel_timer_stop()
-        kern = {}
-        for f, name in A.TIMED_KERNELS.items():
-            n, ms, by = o32.kernel_timer_result(f, 0)
-            if n:
-                kern[name] = {"launches_per_step": n / args.steps, "ms_per_step": ms / args.steps,
-                              "avg_us": 1e3 * ms / n, "bytes_per_launch": by / n,
-                              "GBs": by / (ms * 1e-3) / 1e9}
-                for stg in (1, 2, 3):
-                    ns_, mss, bys = o32.kernel_timer_result(f, stg)
-                    if ns_:
-                        kern[name][f"stage{stg}"] = {"launches": ns_, "avg_us": 1e3 * mss / ns_,
-                                                     "bytes_per_launch": bys / ns_, "GBs": bys / (mss * 1e-3) / 1e9}
-        res["kernels_in_fp32_step"] = kern
-        dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
-        d = kern[dom]
-        res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "bytes_per_launch": d["bytes_per_launch"],
-                           "avg_launch_us": d["avg_us"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
-                           "fp64_vector": profiled_fp64(dom)}
-    res["fp32_step_state_finite"] = bool(torch.isfinite(paths["fp32_mg"]["state"][2]).all())
-    del paths, o32
-    torch.cuda.synchronize()
+        # ---- per-kernel times inside the FP32-preconditioned steps; roofline of the dominant kernel --------
+        if "kernels" in legs:
+            log("c3 per-kernel timing inside FP32-MG steps")
+            state = paths["fp32_mg"]["state"]
+            mask = 0
+            for f in A.TIMED_KERNELS:
+                mask |= f
+            o32.kernel_timer_start(mask, 2000 + 1000 * args.steps)
+            for _ in range(args.steps):
+                o32.imex_step(*state)
+            o32.kernel_timer_stop()
+            kern = {}
+            for f, name in A.TIMED_KERNELS.items():
+                n, ms, by = o32.kernel_timer_result(f, 0)
+                if n:
+                    kern[name] = {"launches_per_step": n / args.steps, "ms_per_step": ms / args.steps,
+                                  "avg_us": 1e3 * ms / n, "bytes_per_launch": by / n,
+                                  "GBs": by / (ms * 1e-3) / 1e9}
+                    for stg in (1, 2, 3):
+                        ns_, mss, bys = o32.kernel_timer_result(f, stg)
+                        if ns_:
+                            kern[name][f"stage{stg}"] = {"launches": ns_, "avg_us": 1e3 * mss / ns_,
+                                                         "bytes_per_launch": bys / ns_, "GBs": bys / (mss * 1e-3) / 1e9}
+            res["kernels_in_fp32_step"] = kern
+            dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+            d = kern[dom]
+            res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "bytes_per_launch": d["bytes_per_launch"],
+                               "avg_launch_us": d["avg_us"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(dom),
+                               "fp64_vector": profiled_fp64(dom)}
+        res["fp32_step_state_finite"] = bool(torch.isfinite(paths["fp32_mg"]["state"][2]).all())
+        del paths, o32
+        torch.cuda.synchronize()
 
     # ---- 2D 512x100, k=4 (c2): steps and isolated applies over rotated buffer sets ------------------------
     if "c2" in legs:
