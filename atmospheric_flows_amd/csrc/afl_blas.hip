@@ -103,6 +103,18 @@ __global__ __launch_bounds__(AFL_NT) void k_norm_partial(int64_t n, const double
   if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
+// FP32 vector, FP64 accumulation: partial sums of x . x
+__global__ __launch_bounds__(AFL_NT) void k_norm_partial_f32(int64_t n, const float* __restrict__ x,
+                                                             double* __restrict__ part) {
+  double a = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)AFL_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * AFL_NT) {
+    const double v = (double)x[i];
+    a += v * v;
+  }
+  a = block_reduce<false>(a);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
 // out[j] = fin(sum_b part[j*kRedBlocks + b]) for j < m
 template <int MODE>  // 0 sum, 1 sqrt(sum), 2 max
 __global__ __launch_bounds__(AFL_NT) void k_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
@@ -176,6 +188,11 @@ void vec_maxpy(hipStream_t st, int64_t n, const double* V, int64_t ldv, int m, c
 void vec_norm2(hipStream_t st, int64_t n, const double* x, double* work, double* out) {
   const int nb = red_blocks(n);
   k_norm_partial<false, false><<<dim3(nb), dim3(AFL_NT), 0, st>>>(n, x, nullptr, work);
+  k_finish<1><<<dim3(1), dim3(AFL_NT), 0, st>>>(work, nb, out);
+}
+void vec_norm2_f32(hipStream_t st, int64_t n, const float* x, double* work, double* out) {
+  const int nb = red_blocks(n);
+  k_norm_partial_f32<<<dim3(nb), dim3(AFL_NT), 0, st>>>(n, x, work);
   k_finish<1><<<dim3(1), dim3(AFL_NT), 0, st>>>(work, nb, out);
 }
 void vec_linf(hipStream_t st, int64_t n, const double* x, double* work, double* out) {

@@ -199,6 +199,7 @@ void vec_mdot(hipStream_t st, int64_t n, const double* V, int64_t ldv, int m, co
 // w -= sum_j h[j] V_j  (h on device);  x += sum_j y[j] V_j with y on device
 void vec_maxpy(hipStream_t st, int64_t n, const double* V, int64_t ldv, int m, const double* h, double sign, double* w);
 void vec_norm2(hipStream_t st, int64_t n, const double* x, double* work, double* out);  // sqrt(<x,x>)
+void vec_norm2_f32(hipStream_t st, int64_t n, const float* x, double* work, double* out);  // FP64 accumulation
 void vec_linf(hipStream_t st, int64_t n, const double* x, double* work, double* out);
 void vec_linf_diff(hipStream_t st, int64_t n, const double* x, const double* y, double* work, double* out);
 int64_t red_work_size(int m);

@@ -47,7 +47,9 @@ template <typename R>
 struct MgT {
   std::vector<MgLevelT<R>> levels;
   int smoother_degree = 3, coarse_degree = 8;
-  int warm_eig_iters = 4;  // power iterations when restarting from the previous eigenvector
+  // power iterations when restarting from the previous eigenvector (rho_for_fixed changes by about
+  // 1e-4 between stages; the smoothed interval reaches 1.2 lambda_max)
+  int warm_eig_iters = 2;
   KernelTimer* timer = nullptr;  // optional launch timing of the level kernels
   double smoothing_range = 15.0;
   // tabs_by_k: 1D tables per degree with the operator's 2k + 1 Gauss points; tabs_rq_by_k: the same

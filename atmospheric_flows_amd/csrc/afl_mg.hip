@@ -531,11 +531,14 @@ int MgT<R>::estimate_eigenvalues(const OpCtx& fine, int iters, double* work_d, d
     for (int it = 0; it < its; ++it) {
       level_schur(L, fine, L.ev, L.t);
       level_pinv(L, L.t, L.z);
-      // |z| / |x| (norms in double through the shared reduction kernels)
-      k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.z, work_d);
-      vec_norm2(L.stream, L.N, work_d, work_d + L.N, scal_d);
-      k_convert<R, double><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, L.ev, work_d);
-      vec_norm2(L.stream, L.N, work_d, work_d + L.N, scal_d + 1);
+      // |z| / |x| (norms accumulated in double)
+      if constexpr (sizeof(R) == 4) {
+        vec_norm2_f32(L.stream, L.N, (const float*)L.z, work_d + L.N, scal_d);
+        vec_norm2_f32(L.stream, L.N, (const float*)L.ev, work_d + L.N, scal_d + 1);
+      } else {
+        vec_norm2(L.stream, L.N, (const double*)L.z, work_d + L.N, scal_d);
+        vec_norm2(L.stream, L.N, (const double*)L.ev, work_d + L.N, scal_d + 1);
+      }
       // x = z / |z| with the norm read on the device: the host waits only for the last quotient
       k_vec_scale_inv_dev<R><<<dim3(grid_n(L.N)), dim3(AFL_NT), 0, L.stream>>>(L.N, scal_d, L.z, L.ev);
     }
