@@ -596,7 +596,7 @@ __device__ __attribute__((always_inline)) inline void apply_C_rows(const Geo& g,
                                     const R* rows_a = nullptr, const R* rows_b = nullptr, int sel = -1) {
   using C = Cfg<DIM, K, Q>;
   constexpr int NQ = C::NQ;
-  constexpr bool BATCH = sizeof(R) == 4 && AFL_C_BATCH_F32;
+  constexpr bool BATCH = sizeof(R) == 4 && AFL_C_BATCH_F32 && DIM == 2;
   using LC = LdsC<DIM, K, WITH_D, BATCH, Q>;
   R* t1 = sh;
   R* i1 = t1;
@@ -701,7 +701,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __rest
                                                      const R* __restrict__ w, const R* __restrict__ pD, R scaleD,
                                                      R* __restrict__ dst) {
   const auto& T = *Tp;
-  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4 && AFL_C_BATCH_F32, Q>;
+  using LC = LdsC<DIM, K, WITH_D, sizeof(R) == 4 && AFL_C_BATCH_F32 && DIM == 2, Q>;
   __shared__ R shw[Cfg<DIM, K, Q>::CPB * LC::WF];
   Slot<DIM, K, Q> s(g);
   const int cbs = s.cb < Cfg<DIM, K, Q>::CPB ? s.cb : 0;
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
   const auto& T = *Tp;
-  using LC = LdsC<DIM, K, true, sizeof(R) == 4 && AFL_C_BATCH_F32, Q>;
+  using LC = LdsC<DIM, K, true, sizeof(R) == 4 && AFL_C_BATCH_F32 && DIM == 2, Q>;
   __shared__ R shw[Cfg<DIM, K, Q>::CPB * LC::WF];
   using C = Cfg<DIM, K, Q>;
   constexpr int N1 = C::N1, NPC = C::NPC, ROWS = C::ROWS;
