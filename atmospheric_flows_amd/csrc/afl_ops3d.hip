@@ -101,6 +101,94 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
 // B: vmult_pressure (euler_operator.h:1510-1617) at node `lane`: volume -f p div v, faces +f {p} n.v
 // (walls: p n.v), f = coeff dt / Ma^2. p staged at nod; its neighbour face nodes are loaded here,
 // one direction ahead (nbf: Sz::NF2 values).
+// Work-area values of integrate_B_volume: five z-swept line arrays (the four y-swept arrays reuse
+// the same area).
+template <int K, int Q>
+struct BVol {
+  static constexpr int N1 = K + 1, A1 = N1 * Q * Q, A2 = N1 * N1 * Q, SIZE = 5 * A1;
+  static_assert(4 * A2 <= 5 * A1, "y-swept arrays fit the line arrays' area");
+};
+
+// The volume term of B, -f int p div(v) (euler_operator.h:1510-1617), for the three components at once.
+// On the terrain-following map only the zeta row of J^{-1} is full, so with g = -f p JxW the reference
+// integrands are g/hx d_xi (x), g/hy d_eta (y) and g J^{-1}_{zeta,a} d_zeta (a = x, y, z); after the
+// vertical sweep on the lane's line they are five arrays of k + 1 sums (ps: S W p, pd: D W p, pdz:
+// D W p (1 - zeta / H), each times the line's horizontal factors):
+//   out_x = Sx Sy a0 + Dx Sy b0,  out_y = Sx Sy a1 + Sx Dy b1,  out_z = Sx Sy c2.
+// The y sweep makes four arrays (Sy a0, Sy b0, Sy a1 + Dy b1, Sy c2), the x sweep the three results at
+// node `lane`. The y-swept values stay in registers across a wavefront barrier and are then stored
+// over the line arrays. wk: BVol::SIZE values.
+template <int K, int Q>
+__device__ __attribute__((always_inline)) inline void integrate_B_volume(const Cell<K, Q>& c, const Tabs<K, Q>& tb,
+                                                                          double* wk, const double (&a0)[K + 1],
+                                                                          const double (&b0)[K + 1], const double (&a1)[K + 1],
+                                                                          const double (&b1)[K + 1], const double (&c2)[K + 1],
+                                                                          double (&out)[3]) {
+  using Z = Sz<K, Q>;
+  using BV = BVol<K, Q>;
+  constexpr int N1 = Z::N1, GQ = Z::GQ, NPC = Z::NPC, A1 = BV::A1, A2 = BV::A2;
+  constexpr int NR = Z::cdiv(N1 * N1, GQ);  // rounds of the y sweep
+  double* t = wk;  // [5][N1 iz][Q qy][Q qx]
+  double* u = wk;  // [4][N1 iz][N1 iy][Q qx], written once every line array has been read
+  if (c.line) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      const int o = (i * Q + c.qb) * Q + c.qa;
+      t[o] = a0[i];
+      t[A1 + o] = b0[i];
+      t[2 * A1 + o] = a1[i];
+      t[3 * A1 + o] = b1[i];
+      t[4 * A1 + o] = c2[i];
+    }
+  }
+  wsync();
+  double sv[NR][4];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int pr = c.qb + GQ * r;  // (iz, iy)
+    sv[r][0] = sv[r][1] = sv[r][2] = sv[r][3] = 0.0;
+    if (c.grp_ok && pr < N1 * N1) {
+      const int iz = pr / N1, iy = pr % N1;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int o = (iz * Q + q) * Q + c.qa;
+        const double sy = tb.S[q * N1 + iy], dy = tb.D[q * N1 + iy];
+        sv[r][0] += sy * t[o];
+        sv[r][1] += sy * t[A1 + o];
+        sv[r][2] += sy * t[2 * A1 + o] + dy * t[3 * A1 + o];
+        sv[r][3] += sy * t[4 * A1 + o];
+      }
+    }
+  }
+  wsync();
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int pr = c.qb + GQ * r;
+    if (c.grp_ok && pr < N1 * N1) {
+      const int o = pr * Q + c.qa;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) u[a * A2 + o] = sv[r][a];
+    }
+  }
+  wsync();
+  out[0] = out[1] = out[2] = 0.0;
+  if (c.lane < NPC) {
+    const int ix = c.lane % N1, row = c.lane / N1;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double sx = tb.S[q * N1 + ix], dx = tb.D[q * N1 + ix];
+      const int o = row * Q + q;
+      out[0] += sx * u[o] + dx * u[A2 + o];
+      out[1] += sx * u[2 * A2 + o];
+      out[2] += sx * u[3 * A2 + o];
+    }
+  }
+  wsync();
+}
+
+// B p at the cell's nodes (euler_operator.h:1510-1617): the face terms direction by direction (the x
+// and y faces have the constant normals +-e_x, +-e_y, so only that component receives their
+// integral), then the volume term from the vertical sums of p on the lane's line.
 template <int K>
 __device__ __attribute__((always_inline)) inline void apply_B_nodes(const Cell<K>& c, const Geo& g, const Tab& T,
                                                                      const Tabs<K, 2 * K + 1>& tb,
@@ -109,6 +197,7 @@ __device__ __attribute__((always_inline)) inline void apply_B_nodes(const Cell<K
                                                                      double* nbf, double* wk, double (&out)[3]) {
   constexpr int Q = 2 * K + 1;
   using Z = Sz<K, Q>;
+  constexpr int N1 = K + 1;
   double fadd[3] = {0.0, 0.0, 0.0};
   double pn = c.nbr(p, 0);
 #pragma unroll
@@ -118,38 +207,71 @@ __device__ __attribute__((always_inline)) inline void apply_B_nodes(const Cell<K
     wsync();
     double trp[1][2][2];
     traces<K, Q, 1>(c, tb, dir, nod, Z::NPC, nbf, Z::NF2, wk, trp);
-    double Fa[3][2] = {};
-    if (c.line) {
-      double n[2][3], jxw[2];
-      dir_geo<K>(g, T, c.ic, lg, dir, c.lane, n, jxw);
+    double n[2][3] = {}, jxw[2] = {};
+    if (c.line) dir_geo<K>(g, T, c.ic, lg, dir, c.lane, n, jxw);
+    if (dir < 2) {
+      double Fa[1][2] = {};
+      if (c.line)
 #pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        const double avg = 0.5 * (trp[0][side][0] + trp[0][side][1]);
+        for (int side = 0; side < 2; ++side)
+          Fa[0][side] = f * 0.5 * (trp[0][side][0] + trp[0][side][1]) * n[side][dir] * jxw[side];
+      double add1[1] = {0.0};
+      face_integrate<K, Q, 1>(c, tb, dir, wk, Fa, add1);
+      fadd[dir] += add1[0];
+    } else {
+      double Fa[3][2] = {};
+      if (c.line)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) Fa[a][side] = f * avg * n[side][a] * jxw[side];
-      }
+        for (int side = 0; side < 2; ++side) {
+          const double avg = 0.5 * (trp[0][side][0] + trp[0][side][1]);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) Fa[a][side] = f * avg * n[side][a] * jxw[side];
+        }
+      face_integrate<K, Q, 3>(c, tb, dir, wk, Fa, fadd);
     }
-    face_integrate<K, Q, 3>(c, tb, dir, wk, Fa, fadd);
   }
   double pq[1][Q];
   eval<K, Q, 1>(c, T, nod, Z::NPC, wk, pq);
+  double a0[N1], b0[N1], a1[N1], b1[N1], c2[N1];
+  {
+    double ps[N1], pd[N1], pdz[N1];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double V[1][Q] = {}, G[1][3][Q], o[1];
+    for (int i = 0; i < N1; ++i) ps[i] = pd[i] = pdz[i] = 0.0;
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-      to_ref<K>(lg, g, T, c.ic, q, a == 0 ? -f * pq[0][q] : 0.0, a == 1 ? -f * pq[0][q] : 0.0,
-                a == 2 ? -f * pq[0][q] : 0.0, G[0][0][q], G[0][1][q], G[0][2][q]);
-    integrate<K, Q, 1, false, true>(c, T, tb, wk, V, G, o);
-    out[a] = o[0] + fadd[a];
+    for (int q = 0; q < Q; ++q) {
+      const double fz = 1.0 - (lg.zeta0 + T.QP[q] * lg.hz) * lg.iH;
+      const double wp = T.W[q] * pq[0][q];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        ps[i] += T.S[q][i] * wp;
+        pd[i] += T.D[q][i] * wp;
+        pdz[i] += T.D[q][i] * (wp * fz);
+      }
+    }
+    // the line's factors of gref (LineGeo) for the gradient fluxes -f p e_a
+    const double fw = f * lg.wxy, hxy = lg.hx * lg.hy;
+    const double ca0 = fw * hxy * lg.dhx, cb0 = -fw * lg.hy * lg.c, ca1 = fw * hxy * lg.dhy, cb1 = -fw * lg.hx * lg.c,
+                 cc2 = -fw * hxy;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      a0[i] = ca0 * pdz[i];
+      b0[i] = cb0 * ps[i];
+      a1[i] = ca1 * pdz[i];
+      b1[i] = cb1 * ps[i];
+      c2[i] = cc2 * pd[i];
+    }
   }
+  double vol[3];
+  integrate_B_volume<K, Q>(c, tb, wk, a0, b0, a1, b1, c2, vol);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[a] = vol[a] + fadd[a];
 }
 
 template <int K>
 __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) void k3_apply_B(
     Geo g, const Tab* __restrict__ Tp, double f, int64_t N, const double* __restrict__ p, double* __restrict__ dst) {
   constexpr int PER_ = Sz<K, 2 * K + 1>::NPC + Sz<K, 2 * K + 1>::NF2 +
-                       Max4<Sz<K, 2 * K + 1>::ING, Sz<K, 2 * K + 1>::TR, 3 * Sz<K, 2 * K + 1>::FI, Sz<K, 2 * K + 1>::EV>::v;
+                       Max4<BVol<K, 2 * K + 1>::SIZE, Sz<K, 2 * K + 1>::TR, 3 * Sz<K, 2 * K + 1>::FI, Sz<K, 2 * K + 1>::EV>::v;
   W3_PROLOGUE(PER_)
   double* nod = wbase;
   double* nbf = nod + NPC;
