@@ -33,6 +33,7 @@ struct Sz {
   static constexpr int TR = 4 * N1 * Q;                            // traces of one field on one direction's faces
   static constexpr int INV = N1 * Q * Q + N1 * N1 * Q;             // values-only integration, one field
   static constexpr int ING = 3 * N1 * Q * Q + 2 * N1 * N1 * Q;     // value + gradient integration, one field
+  static constexpr int INGZ = 3 * N1 * Q * Q;                      // integrate_from_z (second pass in place)
   static constexpr int FI = 2 * LT + 2 * N1 * Q + 2 * NFN;         // one direction's face integrals, one component
   static constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 };
@@ -407,7 +408,8 @@ __device__ __attribute__((always_inline)) inline void integrate(const Cell<K, Q>
 // The y and x sweeps of integrate for one component, from the z sweep already applied on the lane's
 // line: sa[i] = sum_q S[q][i] V[q] + D[q][i] Gz[q], sx[i] = sum_q S[q][i] Gx[q], sy[i] likewise
 // (vertical node i). Lets a kernel accumulate the z-swept sums state by state instead of holding
-// every quadrature-point contribution. Returns the result at node `lane`. wk: Sz::ING.
+// every quadrature-point contribution. Returns the result at node `lane`. The y-swept values stay in
+// registers across a wavefront barrier and then overwrite the line arrays. wk: Sz::INGZ.
 template <int K, int Q>
 __device__ __attribute__((always_inline)) inline double integrate_from_z(const Cell<K, Q>& c, const Tabs<K, Q>& tb,
                                                                           double* wk, const double (&sa)[K + 1],
@@ -415,10 +417,12 @@ __device__ __attribute__((always_inline)) inline double integrate_from_z(const C
                                                                           const double (&sy)[K + 1]) {
   using Z = Sz<K, Q>;
   constexpr int N1 = Z::N1, GQ = Z::GQ, NPC = Z::NPC, A1 = N1 * Q * Q, A2 = N1 * N1 * Q;
+  constexpr int NR = Z::cdiv(N1 * N1, GQ);  // rounds of the y sweep
+  static_assert(2 * A2 <= 3 * A1, "y-swept arrays fit the line arrays' area");
   double* ta = wk;
   double* tx = ta + A1;
   double* ty = tx + A1;
-  double* ua = ty + A1;
+  double* ua = wk;  // written once every line array has been read
   double* ux = ua + A2;
   if (c.line) {
 #pragma unroll
@@ -430,22 +434,28 @@ __device__ __attribute__((always_inline)) inline double integrate_from_z(const C
     }
   }
   wsync();
-  if (c.grp_ok) {
-#pragma unroll 1
-    for (int r = 0; r < Z::cdiv(N1 * N1, GQ); ++r) {
-      const int pr = c.qb + GQ * r;  // (iz, iy)
-      if (pr < N1 * N1) {
-        const int iz = pr / N1, iy = pr % N1;
-        double s0 = 0.0, s1 = 0.0;
+  double sv[NR][2];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          const int o = (iz * Q + q) * Q + c.qa;
-          s0 += tb.S[q * N1 + iy] * ta[o] + tb.D[q * N1 + iy] * ty[o];
-          s1 += tb.S[q * N1 + iy] * tx[o];
-        }
-        ua[pr * Q + c.qa] = s0;
-        ux[pr * Q + c.qa] = s1;
+  for (int r = 0; r < NR; ++r) {
+    const int pr = c.qb + GQ * r;  // (iz, iy)
+    sv[r][0] = sv[r][1] = 0.0;
+    if (c.grp_ok && pr < N1 * N1) {
+      const int iz = pr / N1, iy = pr % N1;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int o = (iz * Q + q) * Q + c.qa;
+        sv[r][0] += tb.S[q * N1 + iy] * ta[o] + tb.D[q * N1 + iy] * ty[o];
+        sv[r][1] += tb.S[q * N1 + iy] * tx[o];
       }
+    }
+  }
+  wsync();
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int pr = c.qb + GQ * r;
+    if (c.grp_ok && pr < N1 * N1) {
+      ua[pr * Q + c.qa] = sv[r][0];
+      ux[pr * Q + c.qa] = sv[r][1];
     }
   }
   wsync();

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     const double* __restrict__ w, const double* __restrict__ pD, double scaleD, double* __restrict__ dst) {
   constexpr int NO = WITH_D ? 5 : 4;  // staged own fields: pfix, w0, w1, w2 [, pD]
   constexpr int PER_ = NO * Sz<K, 2 * K + 1>::NPC + 4 * Sz<K, 2 * K + 1>::NF2 +
-                       Max4<Sz<K, 2 * K + 1>::ING, 4 * Sz<K, 2 * K + 1>::TR, 4 * Sz<K, 2 * K + 1>::EV,
+                       Max4<Sz<K, 2 * K + 1>::INGZ, 4 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
                             Sz<K, 2 * K + 1>::FI>::v;
   W3_PROLOGUE(PER_)
   double* nod = wbase;
@@ -351,23 +351,45 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     }
     face_integrate<K, Q, 1>(c, tb, dir, wk, Fo, fadd);
   }
-  double v[4][Q];
-  eval<K, Q, 4>(c, T, nod, NPC, wk, v);
-  double V[1][Q], G[1][3][Q];
+  // volume: the fields on the lane's line after the horizontal sweeps (two at a time), the flux
+  // G = s w with s = -cdt gamma/(gamma-1) p_fix mapped to reference coordinates as LineGeo::gref does
+  // and swept vertically into the three line sums integrate_from_z takes
+  constexpr int N1 = K + 1;
+  double zp[2][N1], zw[2][N1];
+  eval_lines<K, Q, 2>(c, nod, NPC, wk, zp);            // p_fix, w0
+  eval_lines<K, Q, 2>(c, nod + 2 * NPC, NPC, wk, zw);  // w1, w2
+  double zd[1][N1] = {};
+  if (WITH_D) eval_lines<K, Q, 1>(c, nod + 4 * NPC, NPC, wk, zd);
+  double sa[N1], sx[N1], sy[N1];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const double s = sgn * (-cdt * gr * v[0][q]);
-    to_ref<K>(lg, g, T, c.ic, q, s * v[1][q], s * v[2][q], s * v[3][q], G[0][0][q], G[0][1][q], G[0][2][q]);
-  }
-  if (WITH_D) {
-    double pdq[1][Q];
-    eval<K, Q, 1>(c, T, nod + (NO - 1) * NPC, NPC, wk, pdq);
+  for (int i = 0; i < N1; ++i) sa[i] = sx[i] = sy[i] = 0.0;
+  if (c.line) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) V[0][q] = scaleD * pdq[0][q] * lg.jxw(g, T, q);
+    for (int q = 0; q < Q; ++q) {
+      const double wq = T.W[q];
+      const double s = sgn * (-cdt * gr * line_value<K>(T, zp[0], q)) * wq;
+      const double g0 = s * line_value<K>(T, zp[1], q), g1 = s * line_value<K>(T, zw[0], q),
+                   g2 = s * line_value<K>(T, zw[1], q);
+      const double fz = 1.0 - (lg.zeta0 + T.QP[q] * lg.hz) * lg.iH;
+      const double gz = g2 - fz * (lg.dhx * g0 + lg.dhy * g1);
+      const double vd = WITH_D ? scaleD * lg.c * wq * line_value<K>(T, zd[0], q) : 0.0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        sx[i] += T.S[q][i] * g0;
+        sy[i] += T.S[q][i] * g1;
+        sa[i] += T.D[q][i] * gz + T.S[q][i] * vd;
+      }
+    }
+    const double hxy = lg.hx * lg.hy, fx = lg.hy * lg.c * lg.wxy, fy = lg.hx * lg.c * lg.wxy, fa = hxy * lg.wxy;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      sx[i] *= fx;
+      sy[i] *= fy;
+      sa[i] *= fa;
+    }
   }
-  double o[1];
-  integrate<K, Q, 1, WITH_D, true>(c, T, tb, wk, V, G, o);
-  c.store(dst, o[0] + fadd[0]);
+  const double o = integrate_from_z<K, Q>(c, tb, wk, sa, sx, sy);
+  c.store(dst, o + fadd[0]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -378,7 +400,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     Geo g, const Tab* __restrict__ Tp, Srcs src, double dt, double w0, double w1, double w2, int64_t N,
     double* __restrict__ dst) {
   constexpr int PER_ = 4 * NS * Sz<K, 2 * K + 1>::NPC + 4 * NS * Sz<K, 2 * K + 1>::NF2 +
-                       Max4<Sz<K, 2 * K + 1>::ING, 4 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
+                       Max4<Sz<K, 2 * K + 1>::INGZ, 4 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
                             Sz<K, 2 * K + 1>::FI>::v;
   W3_PROLOGUE(PER_)
   double* nod = wbase;                 // [4 NS][NPC]: rho_j, u_j (3) per state
@@ -447,30 +469,48 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
       for (int side = 0; side < 2; ++side) Fo[0][side] = -F[side] * fj[side];
     face_integrate<K, Q, 1>(c, tb, dir, wk, Fo, fadd);
   }
-  double V[1][Q] = {}, Gp[3][Q] = {};
+  // volume: rho^n psi + dt sum_j w_j (rho u)_j . grad psi. Each state's fields on the lane's line after
+  // the horizontal sweeps; the flux mapped to reference coordinates as LineGeo::gref does and swept
+  // vertically into the three line sums of integrate_from_z
+  constexpr int N1 = K + 1;
+  double sa[N1], sx[N1], sy[N1];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) sa[i] = sx[i] = sy[i] = 0.0;
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    double v[2][Q], u2[2][Q];
-    eval<K, Q, 2>(c, T, nod + 4 * j * NPC, NPC, wk, v);         // rho_j, u_j0
-    eval<K, Q, 2>(c, T, nod + (4 * j + 2) * NPC, NPC, wk, u2);  // u_j1, u_j2
+    double zr[2][N1], zu[2][N1];
+    eval_lines<K, Q, 2>(c, nod + 4 * j * NPC, NPC, wk, zr);         // rho_j, u_j0
+    eval_lines<K, Q, 2>(c, nod + (4 * j + 2) * NPC, NPC, wk, zu);  // u_j1, u_j2
+    if (c.line) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      if (j == 0) V[0][q] = v[0][q];
-      const double s = wj[j] * dt * v[0][q];
-      Gp[0][q] += s * v[1][q];
-      Gp[1][q] += s * u2[0][q];
-      Gp[2][q] += s * u2[1][q];
+      for (int q = 0; q < Q; ++q) {
+        const double wq = T.W[q];
+        const double r = line_value<K>(T, zr[0], q);
+        const double s = wj[j] * dt * r * wq;
+        const double g0 = s * line_value<K>(T, zr[1], q), g1 = s * line_value<K>(T, zu[0], q),
+                     g2 = s * line_value<K>(T, zu[1], q);
+        const double fz = 1.0 - (lg.zeta0 + T.QP[q] * lg.hz) * lg.iH;
+        const double gz = g2 - fz * (lg.dhx * g0 + lg.dhy * g1);
+        const double vm = j == 0 ? r * lg.c * wq : 0.0;  // rho^n JxW / (hx hy wxy)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          sx[i] += T.S[q][i] * g0;
+          sy[i] += T.S[q][i] * g1;
+          sa[i] += T.D[q][i] * gz + T.S[q][i] * vm;
+        }
+      }
     }
   }
-  double G[1][3][Q];
+  if (c.line) {
+    const double hxy = lg.hx * lg.hy, fx = lg.hy * lg.c * lg.wxy, fy = lg.hx * lg.c * lg.wxy, fa = hxy * lg.wxy;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    to_ref<K>(lg, g, T, c.ic, q, Gp[0][q], Gp[1][q], Gp[2][q], G[0][0][q], G[0][1][q], G[0][2][q]);
-    V[0][q] *= lg.jxw(g, T, q);
+    for (int i = 0; i < N1; ++i) {
+      sx[i] *= fx;
+      sy[i] *= fy;
+      sa[i] *= fa;
+    }
   }
-  double o[1];
-  integrate<K, Q, 1, true, true>(c, T, tb, wk, V, G, o);
-  c.store(dst, o[0] + fadd[0]);
+  c.store(dst, integrate_from_z<K, Q>(c, tb, wk, sa, sx, sy) + fadd[0]);
 }
 
 // ---------------------------------------------------------------------------------------
