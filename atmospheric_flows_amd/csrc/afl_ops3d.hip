@@ -619,25 +619,34 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
         zl[3][i] = z2[0][i]; zl[4][i] = z2[1][i];
       }
     }
+    // the flux tensor mapped to reference coordinates as LineGeo::gref does, with the line's
+    // horizontal factors applied to the z-swept sums once, after the last state
 #pragma unroll 1
     for (int q = 0; q < Q; ++q) {
+      const double wq = T.W[q];
       const double r = line_value<K>(T, zl[0], q);
       const double u[3] = {line_value<K>(T, zl[1], q), line_value<K>(T, zl[2], q), line_value<K>(T, zl[3], q)};
-      const double pw = wpj * dt * iMa2 * line_value<K>(T, zl[4], q);
-      const double jxw = lg.jxw(g, T, q);
+      const double pw = wpj * dt * iMa2 * line_value<K>(T, zl[4], q) * wq;
+      const double fz = 1.0 - (lg.zeta0 + T.QP[q] * lg.hz) * lg.iH;
+      const double cw = lg.c * wq;  // JxW / (hx hy wxy)
       double V[3] = {0.0, 0.0, 0.0};
       if (j == 0)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) V[d] = r * u[d];
-      if (j < NG) V[2] -= cf.gw[j] * dt * iFr2 * r;
-      const double ru = wj * dt * r;
-      double gx, gy, gz;
-      to_ref<K>(lg, g, T, c.ic, q, ru * u[0] * u[0] + pw, ru * u[0] * u[1], ru * u[0] * u[2], gx, gy, gz);
-      zsweep_acc<K>(T, q, V[0] * jxw, gx, gy, gz, sa0, sx0, sy0);
-      to_ref<K>(lg, g, T, c.ic, q, ru * u[1] * u[0], ru * u[1] * u[1] + pw, ru * u[1] * u[2], gx, gy, gz);
-      zsweep_acc<K>(T, q, V[1] * jxw, gx, gy, gz, sa1, sx1, sy1);
-      to_ref<K>(lg, g, T, c.ic, q, ru * u[2] * u[0], ru * u[2] * u[1], ru * u[2] * u[2] + pw, gx, gy, gz);
-      zsweep_acc<K>(T, q, V[2] * jxw, gx, gy, gz, sa2, sx2, sy2);
+        for (int d = 0; d < 3; ++d) V[d] = r * u[d] * cw;
+      if (j < NG) V[2] -= cf.gw[j] * dt * iFr2 * r * cw;
+      const double ru = wj * dt * r * wq;
+      {
+        const double g0 = ru * u[0] * u[0] + pw, g1 = ru * u[0] * u[1], g2 = ru * u[0] * u[2];
+        zsweep_acc<K>(T, q, V[0], g0, g1, g2 - fz * (lg.dhx * g0 + lg.dhy * g1), sa0, sx0, sy0);
+      }
+      {
+        const double g0 = ru * u[1] * u[0], g1 = ru * u[1] * u[1] + pw, g2 = ru * u[1] * u[2];
+        zsweep_acc<K>(T, q, V[1], g0, g1, g2 - fz * (lg.dhx * g0 + lg.dhy * g1), sa1, sx1, sy1);
+      }
+      {
+        const double g0 = ru * u[2] * u[0], g1 = ru * u[2] * u[1], g2 = ru * u[2] * u[2] + pw;
+        zsweep_acc<K>(T, q, V[2], g0, g1, g2 - fz * (lg.dhx * g0 + lg.dhy * g1), sa2, sx2, sy2);
+      }
     }
   }
   if (XG) {
@@ -645,8 +654,17 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2))) 
     eval_lines<K, Q, 1>(c, nod + 5 * NE * NPC, NPC, wk, zx);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
-      zsweep_acc<K>(T, q, -cf.gw[NG - 1] * dt * iFr2 * line_value<K>(T, zx[0], q) * lg.jxw(g, T, q), 0.0, 0.0, 0.0,
+      zsweep_acc<K>(T, q, -cf.gw[NG - 1] * dt * iFr2 * line_value<K>(T, zx[0], q) * lg.c * T.W[q], 0.0, 0.0, 0.0,
                     sa2, sx2, sy2);
+  }
+  {
+    const double fx = lg.hy * lg.c * lg.wxy, fy = lg.hx * lg.c * lg.wxy, fa = lg.hx * lg.hy * lg.wxy;
+#pragma unroll
+    for (int i = 0; i <= K; ++i) {
+      sx0[i] *= fx; sy0[i] *= fy; sa0[i] *= fa;
+      sx1[i] *= fx; sy1[i] *= fy; sa1[i] *= fa;
+      sx2[i] *= fx; sy2[i] *= fy; sa2[i] *= fa;
+    }
   }
   c.store(dst, integrate_from_z<K, Q>(c, tb, wk, sa0, sx0, sy0) + fadd[0]);
   c.store(dst + N, integrate_from_z<K, Q>(c, tb, wk, sa1, sx1, sy1) + fadd[1]);
@@ -666,7 +684,7 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(STAGE
   constexpr bool KC = FULL || STAGE == 3;  // kinetic correction (and the extra gravity pair) present
   constexpr int NF = 5 * NE + (KC ? 4 : 0);  // staged states [, correction pair rho, u]
   constexpr int PER_ = NF * Sz<K, 2 * K + 1>::NPC + 5 * NE * Sz<K, 2 * K + 1>::NF2 +
-                       Max4<Sz<K, 2 * K + 1>::ING, 5 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
+                       Max4<Sz<K, 2 * K + 1>::INGZ, 5 * Sz<K, 2 * K + 1>::TR, 2 * Sz<K, 2 * K + 1>::EV,
                             Sz<K, 2 * K + 1>::FI>::v;
   W3_PROLOGUE(PER_)
   constexpr double gm1 = kGamma - 1.0;
@@ -768,7 +786,13 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(STAGE
       for (int side = 0; side < 2; ++side) Fo[0][side] = -F[side] * fj[side];
     face_integrate<K, Q, 1>(c, tb, dir, wk, Fo, fadd);
   }
-  double V[1][Q] = {}, Gp[3][Q] = {};
+  // volume: the terms of every state accumulated straight into the three vertical line sums of
+  // integrate_from_z (fluxes mapped to reference coordinates as LineGeo::gref does; the line's
+  // horizontal factors applied once at the end)
+  constexpr int N1 = K + 1;
+  double sa[N1], sx[N1], sy[N1];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) sa[i] = sx[i] = sy[i] = 0.0;
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
     const double wj = cf.w[j], wpj = cf.wt[j];
@@ -784,18 +808,25 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(STAGE
         zl[3][i] = zb[1][i]; zl[4][i] = zc[0][i];
       }
     }
+    if (c.line) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const double r = line_value<K>(T, zl[0], q), p = line_value<K>(T, zl[4], q);
-      const double u[3] = {line_value<K>(T, zl[1], q), line_value<K>(T, zl[2], q), line_value<K>(T, zl[3], q)};
-      const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-      const double E = 1.0 / gm1 * (p / r) + 0.5 * Ma2 * uu;
-      if (j == 0) V[0][q] += r * E;
-      if (j < NG) V[0][q] -= cf.gw[j] * dt * Ma2 * iFr2 * r * u[2];  // gravity work of state j
-      const double kin = 0.5 * wj * dt * Ma2 * uu * r;
-      const double ent = wpj * dt * (r * (E - 0.5 * Ma2 * uu) + p);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) Gp[d][q] += (kin + ent) * u[d];
+      for (int q = 0; q < Q; ++q) {
+        const double wq = T.W[q];
+        const double r = line_value<K>(T, zl[0], q), p = line_value<K>(T, zl[4], q);
+        const double u[3] = {line_value<K>(T, zl[1], q), line_value<K>(T, zl[2], q), line_value<K>(T, zl[3], q)};
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double E = 1.0 / gm1 * (p / r) + 0.5 * Ma2 * uu;
+        double v = 0.0;
+        if (j == 0) v += r * E;
+        if (j < NG) v -= cf.gw[j] * dt * Ma2 * iFr2 * r * u[2];  // gravity work of state j
+        v *= lg.c * wq;                                           // JxW / (hx hy wxy)
+        const double kin = 0.5 * wj * dt * Ma2 * uu * r;
+        const double ent = wpj * dt * (r * (E - 0.5 * Ma2 * uu) + p);
+        const double f = (kin + ent) * wq;
+        const double g0 = f * u[0], g1 = f * u[1], g2 = f * u[2];
+        const double fz = 1.0 - (lg.zeta0 + T.QP[q] * lg.hz) * lg.iH;
+        zsweep_acc<K>(T, q, v, g0, g1, g2 - fz * (lg.dhx * g0 + lg.dhy * g1), sa, sx, sy);
+      }
     }
   }
   if (KC) {
@@ -808,23 +839,27 @@ __global__ __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(STAGE
 #pragma unroll
       for (int i = 0; i <= K; ++i) { zk[0][i] = za[0][i]; zk[1][i] = za[1][i]; zk[2][i] = zb[0][i]; zk[3][i] = zb[1][i]; }
     }
+    if (c.line) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const double r = line_value<K>(T, zk[0], q), w2 = line_value<K>(T, zk[3], q);
-      const double u0 = line_value<K>(T, zk[1], q), u1 = line_value<K>(T, zk[2], q);
-      V[0][q] -= 0.5 * r * Ma2 * (u0 * u0 + u1 * u1 + w2 * w2);
-      if (NG > NE) V[0][q] -= cf.gw[NG - 1] * dt * Ma2 * iFr2 * r * w2;
+      for (int q = 0; q < Q; ++q) {
+        const double r = line_value<K>(T, zk[0], q), w2 = line_value<K>(T, zk[3], q);
+        const double u0 = line_value<K>(T, zk[1], q), u1 = line_value<K>(T, zk[2], q);
+        double v = -0.5 * r * Ma2 * (u0 * u0 + u1 * u1 + w2 * w2);
+        if (NG > NE) v -= cf.gw[NG - 1] * dt * Ma2 * iFr2 * r * w2;
+        zsweep_acc<K>(T, q, v * lg.c * T.W[q], 0.0, 0.0, 0.0, sa, sx, sy);
+      }
     }
   }
-  double G[1][3][Q];
+  if (c.line) {
+    const double fx = lg.hy * lg.c * lg.wxy, fy = lg.hx * lg.c * lg.wxy, fa = lg.hx * lg.hy * lg.wxy;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    to_ref<K>(lg, g, T, c.ic, q, Gp[0][q], Gp[1][q], Gp[2][q], G[0][0][q], G[0][1][q], G[0][2][q]);
-    V[0][q] *= lg.jxw(g, T, q);
+    for (int i = 0; i < N1; ++i) {
+      sx[i] *= fx;
+      sy[i] *= fy;
+      sa[i] *= fa;
+    }
   }
-  double o[1];
-  integrate<K, Q, 1, true, true>(c, T, tb, wk, V, G, o);
-  c.store(dst, o[0] + fadd[0]);
+  c.store(dst, integrate_from_z<K, Q>(c, tb, wk, sa, sx, sy) + fadd[0]);
 }
 
 // The u_fixed terms of E3 at an implicit stage (see k_rhs_p_ufix): dst = base + int psi V,
