@@ -1,6 +1,8 @@
 // Host/device shared internals of libafl_hip.so (not part of the C-ABI).
 #pragma once
 
+#include <cstdlib>
+
 #include <hip/hip_ext.h>
 
 #include <vector>
@@ -182,6 +184,7 @@ struct AccList {
   int64_t n[kMax];
   int count = 0;
   void add(double* d, const double* s, int64_t len) {
+    if (count >= kMax) std::abort();  // the callers add at most 11 pairs
     dst[count] = d;
     src[count] = s;
     n[count] = len;
