@@ -28,11 +28,11 @@ class SolverParams(C.Structure):
 AFL_SCHUR_JACOBI, AFL_SCHUR_MG_FP32, AFL_SCHUR_MG_FP64 = 0, 1, 2
 AFL_TIME_LEVEL_W, AFL_TIME_LEVEL_C_CHEB = 1, 2
 AFL_TIME_RHS_RHO, AFL_TIME_RHS_U, AFL_TIME_RHS_P, AFL_TIME_RHS_P_UFIX = 4, 8, 16, 32
-AFL_TIME_SCHUR_W, AFL_TIME_APPLY_C, AFL_TIME_MASS_SOLVE, AFL_TIME_APPLY_B = 64, 128, 256, 512
+AFL_TIME_SCHUR_W, AFL_TIME_APPLY_C, AFL_TIME_MASS_SOLVE, AFL_TIME_APPLY_B, AFL_TIME_A_SOLVE = 64, 128, 256, 512, 1024
 TIMED_KERNELS = {AFL_TIME_LEVEL_W: "level_w", AFL_TIME_LEVEL_C_CHEB: "level_c_cheb", AFL_TIME_RHS_RHO: "rhs_rho",
                  AFL_TIME_RHS_U: "rhs_u", AFL_TIME_RHS_P: "rhs_p", AFL_TIME_RHS_P_UFIX: "rhs_p_ufix",
                  AFL_TIME_SCHUR_W: "schur_w", AFL_TIME_APPLY_C: "apply_C", AFL_TIME_MASS_SOLVE: "mass_solve",
-                 AFL_TIME_APPLY_B: "apply_B"}
+                 AFL_TIME_APPLY_B: "apply_B", AFL_TIME_A_SOLVE: "A_solve"}
 
 
 class StepInfo(C.Structure):

@@ -1101,7 +1101,7 @@ int afl_kernel_timer_start(afl_operator* op, int kernels, int max_launches) {
   if (!op) return fail(AFL_ERR_INVALID_ARGUMENT, "null handle");
   constexpr int kAll = AFL_TIME_LEVEL_W | AFL_TIME_LEVEL_C_CHEB | AFL_TIME_RHS_RHO | AFL_TIME_RHS_U | AFL_TIME_RHS_P |
                        AFL_TIME_RHS_P_UFIX | AFL_TIME_SCHUR_W | AFL_TIME_APPLY_C | AFL_TIME_MASS_SOLVE |
-                       AFL_TIME_APPLY_B;
+                       AFL_TIME_APPLY_B | AFL_TIME_A_SOLVE;
   if (kernels <= 0 || (kernels & ~kAll)) return fail(AFL_ERR_INVALID_ARGUMENT, "unknown timed kernel");
   if (max_launches < 1 || max_launches > (1 << 20)) return fail(AFL_ERR_INVALID_ARGUMENT, "max_launches out of range");
   KernelTimer& t = op->timer;
@@ -2036,21 +2036,28 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
     // with no recomputation
     constexpr int kF = 11;  // rho, u, p, rho2, u2, p2, rho3, u3, p3, rho_c, u_c
     const bool vec[kF] = {false, true, false, false, true, false, false, true, false, false, true};
-    std::vector<DevBuf> store((size_t)kF * nsteps);
+    // one allocation for every step's fields (thousands of separate allocations cost a visible
+    // fraction of a forward run on small meshes); field f of step k at slot(k, f)
+    int64_t per_step = 0;
+    int64_t off[kF];
+    for (int f = 0; f < kF; ++f) {
+      off[f] = per_step;
+      per_step += vec[f] ? NU : N;
+    }
+    DevBuf all;
     DevBuf x[3], lam[2][3];
     int rc;
     auto release = [&]() {
-      for (DevBuf& b : store) b.release();
+      all.release();
       for (DevBuf& b : x) b.release();
       for (auto& l : lam)
         for (DevBuf& b : l) b.release();
     };
-    for (int k = 0; k < nsteps; ++k)
-      for (int f = 0; f < kF; ++f)
-        if ((rc = store[(size_t)kF * k + f].alloc(vec[f] ? NU : N))) {
-          release();
-          return fail(AFL_ERR_DEVICE, "adjoint_run: not enough device memory to store every step; use snapshots");
-        }
+    if ((rc = all.alloc(per_step * nsteps))) {
+      release();
+      return fail(AFL_ERR_DEVICE, "adjoint_run: not enough device memory to store every step; use snapshots");
+    }
+    auto slot = [&](int k, int f) { return all.p + per_step * k + off[f]; };
     for (int f = 0; f < 3; ++f)
       if ((rc = x[f].alloc(f == 1 ? NU : N)) || (rc = lam[0][f].alloc(f == 1 ? NU : N)) ||
           (rc = lam[1][f].alloc(f == 1 ? NU : N))) {
@@ -2063,11 +2070,10 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
     vec_copy(st, NU, u0, x[1].p);
     vec_copy(st, N, p0, x[2].p);
     for (int k = 0; k < nsteps && !rc; ++k) {
-      DevBuf* S = &store[(size_t)kF * k];
-      for (int f = 0; f < 3; ++f) vec_copy(st, f == 1 ? NU : N, x[f].p, S[f].p);
+      for (int f = 0; f < 3; ++f) vec_copy(st, f == 1 ? NU : N, x[f].p, slot(k, f));
       if ((rc = afl_imex_step(op, x[0].p, x[1].p, x[2].p, nullptr))) break;
       const double* sv[8] = {op->rho2.p, op->u2.p, op->p2.p, op->rho3.p, op->u3.p, op->p3.p, op->rhoc.p, op->uc.p};
-      for (int f = 0; f < 8; ++f) vec_copy(st, vec[3 + f] ? NU : N, sv[f], S[3 + f].p);
+      for (int f = 0; f < 8; ++f) vec_copy(st, vec[3 + f] ? NU : N, sv[f], slot(k, 3 + f));
       ++as.forward_steps;
     }
     int cur = 0;
@@ -2077,11 +2083,10 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
       vec_copy(st, N, p_bar_T, lam[0][2].p);
     }
     for (int k = nsteps - 1; k >= 0 && !rc; --k) {
-      DevBuf* S = &store[(size_t)kF * k];
       const double* stg[8];
-      for (int f = 0; f < 8; ++f) stg[f] = S[3 + f].p;
+      for (int f = 0; f < 8; ++f) stg[f] = slot(k, 3 + f);
       const int nx = 1 - cur;
-      rc = step_vjp_reverse(op, S[0].p, S[1].p, S[2].p, stg, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p,
+      rc = step_vjp_reverse(op, slot(k, 0), slot(k, 1), slot(k, 2), stg, lam[cur][0].p, lam[cur][1].p, lam[cur][2].p,
                             lam[nx][0].p, lam[nx][1].p, lam[nx][2].p);
       cur = nx;
       ++as.adjoint_steps;

@@ -40,7 +40,8 @@ struct KernelTimer {
   }
 };
 enum KernelId { kTimeLevelW = 1, kTimeLevelCCheb = 2, kTimeRhsRho = 4, kTimeRhsU = 8, kTimeRhsP = 16,
-                kTimeRhsPUfix = 32, kTimeSchurW = 64, kTimeApplyC = 128, kTimeMassSolve = 256, kTimeApplyB = 512 };
+                kTimeRhsPUfix = 32, kTimeSchurW = 64, kTimeApplyC = 128, kTimeMassSolve = 256, kTimeApplyB = 512,
+                kTimeASolve = 1024 };
 
 // Positional sources of an explicit-stage apply (kernel argument by value).
 struct Srcs { const double* p[12]; };

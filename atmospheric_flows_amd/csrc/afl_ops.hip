@@ -1580,7 +1580,7 @@ void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncom
       launch_timed(o, kTimeMassSolve, (int)o.hyp, by, k_mass_solve<D_, K_, 1>, gr, bl, o.geo, o.dT, rho, scale, o.N, b, x,
                    tol2, o.cell_maxit);
     else
-      launch_timed(o, kTimeMassSolve, (int)o.hyp, by, k_mass_solve<D_, K_, D_>, gr, bl, o.geo, o.dT, rho, scale, o.N, b,
+      launch_timed(o, kTimeASolve, (int)o.hyp, by, k_mass_solve<D_, K_, D_>, gr, bl, o.geo, o.dT, rho, scale, o.N, b,
                    x, tol2, o.cell_maxit);
   });
 }

@@ -922,7 +922,7 @@ void launch3_mass_solve(const OpCtx& o, double by, const double* rho, double sca
       launch_timed(o, kTimeMassSolve, (int)o.hyp, by, w3::k3_mass_solve<K_, 1>, w3_grid(o.geo.ncells), w3_block(),
                    o.geo, o.dT, rho, scale, o.N, b, x, o.cell_tol, o.cell_maxit);
     else
-      launch_timed(o, kTimeMassSolve, (int)o.hyp, by, w3::k3_mass_solve<K_, 3>, w3_grid(o.geo.ncells), w3_block(),
+      launch_timed(o, kTimeASolve, (int)o.hyp, by, w3::k3_mass_solve<K_, 3>, w3_grid(o.geo.ncells), w3_block(),
                    o.geo, o.dT, rho, scale, o.N, b, x, o.cell_tol, o.cell_maxit);
   });
 }
@@ -943,7 +943,7 @@ void launch3_schur_w(const OpCtx& o, double by, double f, const double* rho, con
   W3_DISPATCH(o.k, {
     launch_timed(o, kTimeApplyB, (int)o.hyp, sz * (1 + 3), w3::k3_apply_B<K_>, w3_grid(o.geo.ncells), w3_block(), o.geo,
                  o.dT, f, o.N, p, w);
-    launch_timed(o, kTimeMassSolve, (int)o.hyp, sz * (1 + 3 + 3), w3::k3_mass_solve<K_, 3>, w3_grid(o.geo.ncells),
+    launch_timed(o, kTimeASolve, (int)o.hyp, sz * (1 + 3 + 3), w3::k3_mass_solve<K_, 3>, w3_grid(o.geo.ncells),
                  w3_block(), o.geo, o.dT, rho, 1.0, o.N, (const double*)w, w, o.cell_tol, o.cell_maxit);
   });
 }

@@ -118,8 +118,9 @@ enum afl_timed_kernel {
   AFL_TIME_RHS_P_UFIX = 32,    /* E3's u_fixed terms inside the fixed point */
   AFL_TIME_SCHUR_W = 64,       /* FP64 Schur first half w = A^-1 B p */
   AFL_TIME_APPLY_C = 128,      /* FP64 C apply (with D p: the Schur second half) */
-  AFL_TIME_MASS_SOLVE = 256,   /* FP64 cell-local M / A / D solves */
-  AFL_TIME_APPLY_B = 512       /* FP64 B apply */
+  AFL_TIME_MASS_SOLVE = 256,   /* FP64 cell-local M and D solves (one component) */
+  AFL_TIME_APPLY_B = 512,      /* FP64 B apply */
+  AFL_TIME_A_SOLVE = 1024      /* FP64 cell-local rho-weighted A solves (dim components) */
 };
 
 /* Phases of afl_imex_step timed by afl_section_timer_*: the reference's TimerOutput sections
