@@ -126,7 +126,7 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_rho(Geo g, const Tab* __rest
   eval_line_faces<DIM, K>(rbar, s, nb, t1, t2, fn, T, R, trR);
   grad_jxw<DIM, K>(rbar, s, t1, T, g, lg, P);
   const double G0[DIM][NQ] = {};
-#pragma unroll 1
+#pragma unroll
   for (int j = 0; j < NS; ++j) {
     const double cj = (j == 0 ? w0 : (j == 1 ? w1 : w2)) * dt;
     const double* rs = in_at(io, 2 * j);
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_u(Geo g, const Tab* __restri
 #pragma unroll
   for (int q = 0; q < NQ; ++q) jx[q] = lg.jxw(g, T, q);
   const double G0[DIM][NQ] = {};
-#pragma unroll 1
+#pragma unroll
   for (int j = 0; j < NE; ++j) {
     const double cj = pick<3>(cf.w, j) * dt, cpj = pick<3>(cf.wp, j) * dt * iMa2;
     const double cg = (j < NG ? pick<3>(cf.gw, j) : 0.0) * dt * iFr2;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_u(Geo g, const Tab* __restri
     finish_field<DIM, K, true, false>(s, i1, i2, Fq, T, V, G0, Fv, out_at(io, 3 * j + 2));
   }
   // densities that enter through gravity only
-#pragma unroll 1
+#pragma unroll
   for (int gi = NE; gi < NG; ++gi) {
     const double cg = pick<3>(cf.gw, gi) * dt * iFr2;
     double V[NQ], Fv[2 * DIM];
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_p(Geo g, const Tab* __restri
     for (int f = 0; f < 2 * DIM; ++f)
       for (int d = 0; d < DIM; ++d) Q0[f][d] = 0.0;
   }
-#pragma unroll 1
+#pragma unroll
   for (int j = 0; j < NE; ++j) {
     const double c1 = pick<3>(cf.w, j) * dt * Ma2, c2 = pick<3>(cf.wt, j) * dt;
     const double cg = (j < NG ? pick<3>(cf.gw, j) : 0.0) * dt * Ma2 * iFr2;
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_Bt(Geo g, const Tab* __restric
   for (int q = 0; q < NQ; ++q) V[q] = 0.0;
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f) Fv[f] = 0.0;
-#pragma unroll 1
+#pragma unroll
   for (int a = 0; a < DIM; ++a) {
     double ya[NQ], tr[2 * DIM][2], P[DIM][NQ];
     eval_line_faces<DIM, K>(y + a * N, s, nb, t1, t2, fn, T, ya, tr);
@@ -500,7 +500,7 @@ __device__ __attribute__((always_inline)) inline void normal_traces(const double
                                                                     double (&wn)[2 * DIM][2]) {
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f) wn[f][0] = wn[f][1] = 0.0;
-#pragma unroll 1
+#pragma unroll
   for (int d = 0; d < DIM; ++d) {
     double v[2 * K + 1], tr[2 * DIM][2];
     eval_line_faces<DIM, K>(w + d * N, s, nb, t1, t2, fn, T, v, tr);
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_tangent_t(Geo g, const Tab* 
                          : 0.0;
   }
   const double G0[DIM][NQ] = {};
-#pragma unroll 1
+#pragma unroll
   for (int d = 0; d < DIM; ++d) {
     double V[NQ], Fv[2 * DIM];
 #pragma unroll
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_coef_t(Geo g, const Tab* __r
   double wn[2 * DIM][2];
 #pragma unroll
   for (int f = 0; f < 2 * DIM; ++f) wn[f][0] = wn[f][1] = 0.0;
-#pragma unroll 1
+#pragma unroll
   for (int d = 0; d < DIM; ++d) {
     double v[NQ], tr[2 * DIM][2];
     eval_line_faces<DIM, K>(w + d * N, s, nb, t1, t2, fn, T, v, tr);
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(AFL_NT) void k_cell_products(Geo g, const Tab* __re
   if constexpr (MODE == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) V[q] = 0.0;
-#pragma unroll 1
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       double x[NQ], y[NQ];
       eval_line<DIM, K>(a + d * N, s, t1, t2, T, x);
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(AFL_NT) void k_cell_products(Geo g, const Tab* __re
     double x[NQ], r[NQ];
     eval_line<DIM, K>(a, s, t1, t2, T, x);
     eval_line<DIM, K>(b, s, t1, t2, T, r);
-#pragma unroll 1
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       double u[NQ];
       eval_line<DIM, K>(c + d * N, s, t1, t2, T, u);
