@@ -567,6 +567,42 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, const Tab* __restrict
   for (int a = 0; a < DIM; ++a) store_row<DIM, K>(dst + a * N, s, out[a]);
 }
 
+// One component A of B p (3D, k = 4: a launch per component holds one face-flux block and one
+// result row, two waves per SIMD instead of one).
+template <int DIM, int K, int A>
+__global__ __launch_bounds__(AFL_NT) void k_apply_B1(Geo g, const Tab* __restrict__ Tp, double f, int64_t N,
+                                                      const double* __restrict__ p, double* __restrict__ dst) {
+  const auto& T = *Tp;
+  __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
+  __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
+  AFL_SLOT_PTRS
+  double* Fq = shF + cbs * Lds<DIM, K>::F;
+  LineGeo<DIM, K> lg;
+  lg.init(g, T, s.ic, s.l);
+  const Nbrs<DIM> nb(g, s.ic);
+  const FacePts<DIM, K> fp(g, T, s);
+  double pq[NQ], trp[2 * DIM][2];
+  eval_line_faces<DIM, K>(p, s, nb, t1, t2, fn, T, pq, trp);
+  if (s.act) {
+#pragma unroll
+    for (int fc = 0; fc < 2 * DIM; ++fc)
+      Fq[fc * C::NFQ + s.l] = f * 0.5 * (trp[fc][0] + trp[fc][1]) * fp.n[fc][A] * fp.jxw[fc];
+  }
+  cell_sync<DIM, K>();
+  double V[NQ], G[DIM][NQ], out[N1];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double Gp[3] = {0, 0, 0}, Rg[3];
+    Gp[A] = -f * pq[q];
+    lg.gref(g, T, s.ic, q, Gp, Rg);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) G[d][q] = Rg[d];
+  }
+  integrate_line<DIM, K, false, true>(s, i1, i2, T, V, G, out);
+  add_faces<DIM, K>(s, Fq, T, out, 0xffu);
+  store_row<DIM, K>(dst + A * N, s, out);
+}
+
 // ---------------------------------------------------------------------------------------
 // C: vmult_enthalpy (euler_operator.h:1411-1507); nonlinear in w through lambda = max|w.n|.
 // LDS per cell of the C apply. FP32 (BATCH): all fields (p_fix, w, and p for the D term) are
@@ -1136,10 +1172,11 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
   }
 }
 
+// b and x may be the same vector: every row is read by its owner thread before the cell's solve
+// and written by that thread after it.
 template <int DIM, int K, int NC>
 __global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho, double scale,
-                                                        int64_t N, const double* __restrict__ b,
-                                                        double* __restrict__ x, double tol, int maxit) {
+                                                        int64_t N, const double* b, double* x, double tol, int maxit) {
   const auto& T = *Tp;
   constexpr int PC = Lds<DIM, K>::W > SolvLds<DIM, K>::size(NC) ? Lds<DIM, K>::W : SolvLds<DIM, K>::size(NC);
   __shared__ double shw[Cfg<DIM, K>::CPB * PC];
@@ -1542,6 +1579,14 @@ void launch_apply_B(const OpCtx& o, const double* p, double* dst) {
     launch3_apply_B(o, by, f, p, dst);
     return;
   }
+  if (o.dim == 3 && o.k == 4) {
+    const dim3 gr = grid_for<3, 4>(o.geo.ncells), bl(Cfg<3, 4>::NT);
+    const double byc = 8.0 * (double)o.N * 2;
+    launch_timed(o, kTimeApplyB, (int)o.hyp, byc, k_apply_B1<3, 4, 0>, gr, bl, o.geo, o.dT, f, o.N, p, dst);
+    launch_timed(o, kTimeApplyB, (int)o.hyp, byc, k_apply_B1<3, 4, 1>, gr, bl, o.geo, o.dT, f, o.N, p, dst);
+    launch_timed(o, kTimeApplyB, (int)o.hyp, byc, k_apply_B1<3, 4, 2>, gr, bl, o.geo, o.dT, f, o.N, p, dst);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     launch_timed(o, kTimeApplyB, (int)o.hyp, by, k_apply_B<D_, K_>, grid_for<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT),
                  o.geo, o.dT, f, o.N, p, dst);
@@ -1574,6 +1619,17 @@ void launch_mass_solve(const OpCtx& o, const double* rho, double scale, int ncom
     launch3_mass_solve(o, by, rho, scale, ncomp, b, x);
     return;
   }
+  if (o.dim == 3 && o.k == 4 && ncomp > 1) {
+    // one launch per component: the one-component kernel runs at three waves per SIMD, the
+    // three-component one at one (its per-thread iterates triple), which more than pays for
+    // reading rho and forming the cell's preconditioner three times
+    const dim3 gr = grid_for<3, 4>(o.geo.ncells), bl(Cfg<3, 4>::NT);
+    const double byc = 8.0 * (double)o.N * (2 + (rho ? 1 : 0));
+    for (int c = 0; c < ncomp; ++c)
+      launch_timed(o, kTimeASolve, (int)o.hyp, byc, k_mass_solve<3, 4, 1>, gr, bl, o.geo, o.dT, rho, scale, o.N,
+                   b + c * o.N, x + c * o.N, tol2, o.cell_maxit);
+    return;
+  }
   AFL_DISPATCH(o.dim, o.k, {
     const dim3 gr = grid_for<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
     if (ncomp == 1)
@@ -1592,6 +1648,13 @@ void launch_schur_w(const OpCtx& o, const double* rho, const double* p, double* 
   const double by = 8.0 * (double)o.N * (2 + o.dim);  // p, rho in; w out
   if (use_w3(o)) {
     launch3_schur_w(o, by, f, rho, p, w);
+    return;
+  }
+  if (o.dim == 3 && o.k == 4) {
+    // the fused kernel holds the B apply's and the cell solve's registers at once (one wave per
+    // SIMD); as separate launches each phase runs at its own occupancy, for one extra pass over w
+    launch_apply_B(o, p, w);
+    launch_mass_solve(o, rho, 1.0, 3, w, w);
     return;
   }
   AFL_DISPATCH(o.dim, o.k, {
