@@ -110,11 +110,12 @@ __device__ __attribute__((always_inline)) inline double* out_at(const AdjSrcs& i
   return v;
 }
 
-// E1 adjoint: out[i] = (d rhs_rho / d src[i])^T rbar, src = (rho_j, u_j), j < NS.
+// E1 adjoint: out[i] = (d rhs_rho / d src[i])^T rbar, src = (rho_j, u_j), j < NS; one launch per
+// state j = jsel.
 template <int DIM, int K, int NS>
 __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_rho(Geo g, const Tab* __restrict__ Tp, AdjSrcs io,
                                                         const double* __restrict__ rbar, double dt, double w0,
-                                                        double w1, double w2, int64_t N) {
+                                                        double w1, double w2, int64_t N, int jsel) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::F];
@@ -126,8 +127,8 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_rho(Geo g, const Tab* __rest
   eval_line_faces<DIM, K>(rbar, s, nb, t1, t2, fn, T, R, trR);
   grad_jxw<DIM, K>(rbar, s, t1, T, g, lg, P);
   const double G0[DIM][NQ] = {};
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
+  {
+    const int j = jsel;
     const double cj = (j == 0 ? w0 : (j == 1 ? w1 : w2)) * dt;
     const double* rs = in_at(io, 2 * j);
     const double* us = in_at(io, 2 * j + 1);
@@ -178,10 +179,11 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_rho(Geo g, const Tab* __rest
 // gw[g], g < NG (the densities of states beyond NE enter through gravity only).
 struct AdjE2Coef { double w[3], wp[3], gw[3]; };
 
+// One launch per explicit state (jsel < NE), and one for the gravity-only densities (jsel == NE).
 template <int DIM, int K, int NE, int NG>
 __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_u(Geo g, const Tab* __restrict__ Tp, AdjSrcs io,
                                                       const double* __restrict__ rbar, AdjE2Coef cf, double dt,
-                                                      double iMa2, double iFr2, int64_t N) {
+                                                      double iMa2, double iFr2, int64_t N, int jsel) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::F];
@@ -210,8 +212,8 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_u(Geo g, const Tab* __restri
 #pragma unroll
   for (int q = 0; q < NQ; ++q) jx[q] = lg.jxw(g, T, q);
   const double G0[DIM][NQ] = {};
-#pragma unroll
-  for (int j = 0; j < NE; ++j) {
+  if (jsel < NE) {
+    const int j = jsel;
     const double cj = pick<3>(cf.w, j) * dt, cpj = pick<3>(cf.wp, j) * dt * iMa2;
     const double cg = (j < NG ? pick<3>(cf.gw, j) : 0.0) * dt * iFr2;
     double r[NQ], trr[2 * DIM][2], u[DIM][NQ], tru[DIM][2 * DIM][2];
@@ -282,6 +284,7 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_u(Geo g, const Tab* __restri
       Fv[f] = nb.c[f] >= 0 ? cpj * 0.5 * wn : -cpj * Rw[f];
     }
     finish_field<DIM, K, true, false>(s, i1, i2, Fq, T, V, G0, Fv, out_at(io, 3 * j + 2));
+    return;
   }
   // densities that enter through gravity only
 #pragma unroll
@@ -305,10 +308,12 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_u(Geo g, const Tab* __restri
 // c2 = wt dt. STAGE 2 keeps the reference's u_tmp_2 . u_old kinetic jump (euler_operator.h:1182-1183).
 struct AdjE3Coef { double w[3], wt[3], gw[3]; };
 
+// One launch per explicit state: jsel < NE differentiates state jsel, jsel == NE the kinetic-energy
+// correction (one state's inputs live at a time keeps the 3D kernels free of register spills).
 template <int DIM, int K, int NE, int NG, int STAGE>
 __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_p(Geo g, const Tab* __restrict__ Tp, AdjSrcs io,
                                                       const double* __restrict__ rbar, AdjE3Coef cf, double dt,
-                                                      double Ma2, double iFr2, int64_t N) {
+                                                      double Ma2, double iFr2, int64_t N, int jsel) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::F];
@@ -328,32 +333,37 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_p(Geo g, const Tab* __restri
 #pragma unroll
   for (int q = 0; q < NQ; ++q) jx[q] = lg.jxw(g, T, q);
   const double G0[DIM][NQ] = {};
-  // stage 2: traces of u_old for state 1's kinetic jump, and that jump's derivative with respect
-  // to u_old on the own faces (added to u_old's adjoint)
-  double tru0[DIM][2 * DIM][2], Q0[2 * DIM][DIM];
-  if constexpr (STAGE == 2) {
-    double v[NQ];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(in_at(io, 1) + d * N, s, nb, t1, t2, fn, T, v, tru0[d]);
-    double r1[NQ], trr1[2 * DIM][2], tru1[DIM][2 * DIM][2];
-    eval_line_faces<DIM, K>(in_at(io, 3), s, nb, t1, t2, fn, T, r1, trr1);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(in_at(io, 4) + d * N, s, nb, t1, t2, fn, T, v, tru1[d]);
-    const double c1 = pick<3>(cf.w, 1) * dt * Ma2;
-#pragma unroll
-    for (int f = 0; f < 2 * DIM; ++f) {
-      double a = 0.0, b = 0.0;
-      for (int d = 0; d < DIM; ++d) { a += tru1[d][f][0] * nv[f][d]; b += tru1[d][f][1] * nv[f][d]; }
-      const double lam = fmax(fabs(a), fabs(b));
-      for (int d = 0; d < DIM; ++d) Q0[f][d] = W[f] * c1 * 0.5 * lam * trr1[f][0] * 0.5 * tru1[d][f][0];
-    }
-  } else {
+  if (jsel < NE) {
+    const int j = jsel;
+    // stage 2: state 1's kinetic jump reads the traces of u_old (state 0's velocity), and that jump's
+    // derivative with respect to u_old on the own faces goes to u_old's adjoint (state 0)
+    double tru0[DIM][2 * DIM][2], Q0[2 * DIM][DIM];
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f)
-      for (int d = 0; d < DIM; ++d) Q0[f][d] = 0.0;
-  }
 #pragma unroll
-  for (int j = 0; j < NE; ++j) {
+      for (int d = 0; d < DIM; ++d) {
+        Q0[f][d] = 0.0;
+        tru0[d][f][0] = tru0[d][f][1] = 0.0;
+      }
+    if (STAGE == 2 && j == 1) {
+      double v[NQ];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(in_at(io, 1) + d * N, s, nb, t1, t2, fn, T, v, tru0[d]);
+    }
+    if (STAGE == 2 && j == 0) {
+      double v[NQ], r1[NQ], trr1[2 * DIM][2], tru1[DIM][2 * DIM][2];
+      eval_line_faces<DIM, K>(in_at(io, 3), s, nb, t1, t2, fn, T, r1, trr1);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) eval_line_faces<DIM, K>(in_at(io, 4) + d * N, s, nb, t1, t2, fn, T, v, tru1[d]);
+      const double c1 = pick<3>(cf.w, 1) * dt * Ma2;
+#pragma unroll
+      for (int f = 0; f < 2 * DIM; ++f) {
+        double a = 0.0, b = 0.0;
+        for (int d = 0; d < DIM; ++d) { a += tru1[d][f][0] * nv[f][d]; b += tru1[d][f][1] * nv[f][d]; }
+        const double lam = fmax(fabs(a), fabs(b));
+        for (int d = 0; d < DIM; ++d) Q0[f][d] = W[f] * c1 * 0.5 * lam * trr1[f][0] * 0.5 * tru1[d][f][0];
+      }
+    }
     const double c1 = pick<3>(cf.w, j) * dt * Ma2, c2 = pick<3>(cf.wt, j) * dt;
     const double cg = (j < NG ? pick<3>(cf.gw, j) : 0.0) * dt * Ma2 * iFr2;
     const bool quirk = STAGE == 2 && j == 1;
@@ -429,9 +439,8 @@ __global__ __launch_bounds__(AFL_NT) void k_vjp_rhs_p(Geo g, const Tab* __restri
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) Fv[f] = W[f] * c2 * 0.5 * (gr * una[f] + lam[f] / gm1);
     finish_field<DIM, K, true, false>(s, i1, i2, Fq, T, V, G0, Fv, out_at(io, 3 * j + 2));
-  }
-  // kinetic-energy correction (and, for stages 1 and 2, the last gravity pair, the same sources)
-  {
+  } else {
+    // kinetic-energy correction (and, for stages 1 and 2, the last gravity pair, the same sources)
     const double cg = (NG > NE ? pick<3>(cf.gw, NE) : 0.0) * dt * Ma2 * iFr2;
     double r[NQ], u[DIM][NQ];
     eval_line<DIM, K>(in_at(io, KI), s, t1, t2, T, r);
@@ -568,11 +577,15 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_tangent(Geo g, const Tab* __
   finish_field<DIM, K, false, true>(s, i1, i2, Fq, T, V0, G, Fv, dst);
 }
 
-// C'(w0)^T cb -> DIM fields.
-template <int DIM, int K>
+// C'(w0)^T cb -> DIM fields. SVM: minus the scaled velocity mass of cb in the same pass,
+// dst_d -= int phi cb r (alpha u_d + beta [d = z]) (r, u: rs, us), as k_cell_products MODE 1.
+template <int DIM, int K, bool SVM = false>
 __global__ __launch_bounds__(AFL_NT) void k_apply_C_tangent_t(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ pf,
                                                               const double* __restrict__ cb, const double* __restrict__ w0,
-                                                              double cdt, int64_t N, double* __restrict__ dst) {
+                                                              double cdt, int64_t N, double* __restrict__ dst,
+                                                              const double* __restrict__ rs = nullptr,
+                                                              const double* __restrict__ us = nullptr, double alpha = 0.0,
+                                                              double beta = 0.0) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::WF];
   __shared__ double shF[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::F];
@@ -598,11 +611,25 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_tangent_t(Geo g, const Tab* 
                          : 0.0;
   }
   const double G0[DIM][NQ] = {};
+  double cr[NQ];  // SVM: cb r JxW at the line's points
+  if constexpr (SVM) {
+    double r[NQ];
+    eval_line<DIM, K>(rs, s, t1, t2, T, r);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) cr[q] = cv[q] * r[q] * lg.jxw(g, T, q);
+  }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     double V[NQ], Fv[2 * DIM];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) V[q] = -cdt * gr * pv[q] * (d == 0 ? P[0][q] : (d == 1 ? P[1][q] : P[DIM - 1][q]));
+    if constexpr (SVM) {
+      double u[NQ];
+      eval_line<DIM, K>(us + d * N, s, t1, t2, T, u);
+      const double bz = d == DIM - 1 ? beta : 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) V[q] -= cr[q] * (alpha * u[q] + bz);
+    }
 #pragma unroll
     for (int f = 0; f < 2 * DIM; ++f) Fv[f] = Wf[f] * pick<DIM>(nv[f], d);
     finish_field<DIM, K, true, false>(s, i1, i2, Fq, T, V, G0, Fv, dst + d * N);
@@ -654,6 +681,76 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C_coef_t(Geo g, const Tab* __r
   }
   const double G0[DIM][NQ] = {};
   finish_field<DIM, K, true, false>(s, i1, i2, Fq, T, V, G0, Fv, dst);
+}
+
+// The coefficient part of the adjoint stage operator in one pass:
+//   dst = scaleD M cb + (dC(wE; pf)/dpf)^T cb - (dC(wB; pf)/dpf)^T cb - acc,
+// with cb evaluated once for both velocities. acc may be dst: each row is read by its owner thread
+// just before that thread writes it.
+template <int DIM, int K>
+__global__ __launch_bounds__(AFL_NT) void k_apply_C_coef_pair_t(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ wE,
+                                                                const double* __restrict__ wB, const double* __restrict__ cb,
+                                                                double cdt, double scaleD, const double* acc, int64_t N,
+                                                                double* dst) {
+  const auto& T = *Tp;
+  __shared__ double shw[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::WF];
+  __shared__ double shF[Cfg<DIM, K>::CPB * AdjLds<DIM, K>::F];
+  AFL_ADJ_SLOT
+  constexpr double gr = kGamma / (kGamma - 1.0), igm1 = 1.0 / (kGamma - 1.0);
+  LineGeo<DIM, K> lg;
+  lg.init(g, T, s.ic, s.l);
+  const Nbrs<DIM> nb(g, s.ic);
+  double nv[2 * DIM][DIM], jf[2 * DIM];
+  face_normals<DIM, K>(g, T, s, nv, jf);
+  double cv[NQ], trc[2 * DIM][2], P[DIM][NQ], V[NQ];
+  eval_line_faces<DIM, K>(cb, s, nb, t1, t2, fn, T, cv, trc);
+  grad_jxw<DIM, K>(cb, s, t1, T, g, lg, P);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) V[q] = scaleD * cv[q] * lg.jxw(g, T, q);
+  double wnE[2 * DIM][2], wnB[2 * DIM][2];
+#pragma unroll
+  for (int f = 0; f < 2 * DIM; ++f) wnE[f][0] = wnE[f][1] = wnB[f][0] = wnB[f][1] = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double v[NQ], tr[2 * DIM][2];
+    eval_line_faces<DIM, K>(wE + d * N, s, nb, t1, t2, fn, T, v, tr);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) V[q] -= cdt * gr * v[q] * P[d][q];
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      wnE[f][0] += tr[f][0] * nv[f][d];
+      wnE[f][1] += tr[f][1] * nv[f][d];
+    }
+    eval_line_faces<DIM, K>(wB + d * N, s, nb, t1, t2, fn, T, v, tr);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) V[q] += cdt * gr * v[q] * P[d][q];
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      wnB[f][0] += tr[f][0] * nv[f][d];
+      wnB[f][1] += tr[f][1] * nv[f][d];
+    }
+  }
+  double Fv[2 * DIM];
+#pragma unroll
+  for (int f = 0; f < 2 * DIM; ++f) {
+    const double lamE = fmax(fabs(wnE[f][0]), fabs(wnE[f][1]));
+    const double lamB = fmax(fabs(wnB[f][0]), fabs(wnB[f][1]));
+    const double jc = cdt * (trc[f][0] - trc[f][1]) * jf[f] * 0.5;
+    Fv[f] = nb.c[f] >= 0 ? jc * (gr * wnE[f][0] + lamE * igm1) - jc * (gr * wnB[f][0] + lamB * igm1) : 0.0;
+  }
+  const double G0[DIM][NQ] = {};
+  cell_sync<DIM, K>();  // the last field's face reads are done
+  if (s.act)
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) Fq[f * C::NFQ + s.l] = Fv[f];
+  double out[K + 1];
+  integrate_line<DIM, K, true, false>(s, i1, i2, T, V, G0, out);
+  add_faces<DIM, K>(s, Fq, T, out, 0xffu);
+  if (s.act && s.l < C::ROWS) {
+    const int64_t r0 = s.cell * C::NPC + s.l * C::N1;
+#pragma unroll
+    for (int i = 0; i < C::N1; ++i) dst[r0 + i] = out[i] - acc[r0 + i];
+  }
 }
 
 // Cell-local products integrated against the test functions:
@@ -734,9 +831,11 @@ void launch_vjp_rhs_rho(const OpCtx& o, int stage, const double* rbar, const dou
   const TableauC& t = o.tab;
   AFL_ADJ_DISPATCH(o.dim, o.k, {
     const dim3 gr = adj_grid<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
-    if (stage == 1) k_vjp_rhs_rho<D_, K_, 1><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, o.dt, t.a21, 0.0, 0.0, o.N);
-    else if (stage == 2) k_vjp_rhs_rho<D_, K_, 2><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, o.dt, t.a31, t.a32, 0.0, o.N);
-    else k_vjp_rhs_rho<D_, K_, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, o.dt, t.b1, t.b2, t.b3, o.N);
+    for (int j = 0; j < stage; ++j) {
+      if (stage == 1) k_vjp_rhs_rho<D_, K_, 1><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, o.dt, t.a21, 0.0, 0.0, o.N, j);
+      else if (stage == 2) k_vjp_rhs_rho<D_, K_, 2><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, o.dt, t.a31, t.a32, 0.0, o.N, j);
+      else k_vjp_rhs_rho<D_, K_, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, o.dt, t.b1, t.b2, t.b3, o.N, j);
+    }
   });
 }
 
@@ -758,9 +857,13 @@ void launch_vjp_rhs_u(const OpCtx& o, int stage, const double* rbar, const doubl
   const double iMa2 = 1.0 / (o.Ma * o.Ma), iFr2 = 1.0 / (o.Fr * o.Fr);
   AFL_ADJ_DISPATCH(o.dim, o.k, {
     const dim3 gr = adj_grid<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
-    if (stage == 1) k_vjp_rhs_u<D_, K_, 1, 2><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, iMa2, iFr2, o.N);
-    else if (stage == 2) k_vjp_rhs_u<D_, K_, 2, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, iMa2, iFr2, o.N);
-    else k_vjp_rhs_u<D_, K_, 3, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, iMa2, iFr2, o.N);
+    // states 0..NE-1, then (stages 1 and 2) the gravity-only density
+    const int nl = stage == 3 ? 3 : stage + 1;
+    for (int j = 0; j < nl; ++j) {
+      if (stage == 1) k_vjp_rhs_u<D_, K_, 1, 2><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, iMa2, iFr2, o.N, j);
+      else if (stage == 2) k_vjp_rhs_u<D_, K_, 2, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, iMa2, iFr2, o.N, j);
+      else k_vjp_rhs_u<D_, K_, 3, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, iMa2, iFr2, o.N, j);
+    }
   });
 }
 
@@ -782,9 +885,11 @@ void launch_vjp_rhs_p(const OpCtx& o, int stage, const double* rbar, const doubl
   const double Ma2 = o.Ma * o.Ma, iFr2 = 1.0 / (o.Fr * o.Fr);
   AFL_ADJ_DISPATCH(o.dim, o.k, {
     const dim3 gr = adj_grid<D_, K_>(o.geo.ncells), bl(Cfg<D_, K_>::NT);
-    if (stage == 1) k_vjp_rhs_p<D_, K_, 1, 2, 1><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, Ma2, iFr2, o.N);
-    else if (stage == 2) k_vjp_rhs_p<D_, K_, 2, 3, 2><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, Ma2, iFr2, o.N);
-    else k_vjp_rhs_p<D_, K_, 3, 3, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, Ma2, iFr2, o.N);
+    for (int j = 0; j <= stage; ++j) {
+      if (stage == 1) k_vjp_rhs_p<D_, K_, 1, 2, 1><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, Ma2, iFr2, o.N, j);
+      else if (stage == 2) k_vjp_rhs_p<D_, K_, 2, 3, 2><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, Ma2, iFr2, o.N, j);
+      else k_vjp_rhs_p<D_, K_, 3, 3, 3><<<gr, bl, 0, o.stream>>>(o.geo, o.dT, io, rbar, cf, o.dt, Ma2, iFr2, o.N, j);
+    }
   });
 }
 
@@ -801,6 +906,15 @@ void launch_apply_C_coef_t(const OpCtx& o, const double* w, const double* cb, do
   AFL_ADJ_DISPATCH(o.dim, o.k, {
     k_apply_C_coef_t<D_, K_><<<adj_grid<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, w, cb, cdt,
                                                                                                      o.N, dst);
+  });
+}
+
+void launch_apply_C_coef_pair_t(const OpCtx& o, const double* wE, const double* wB, const double* cb, double scaleD,
+                                const double* acc, double* dst) {
+  const double cdt = (o.hyp == 1 ? o.tab.at22 : o.tab.at33) * o.dt;
+  AFL_ADJ_DISPATCH(o.dim, o.k, {
+    k_apply_C_coef_pair_t<D_, K_><<<adj_grid<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(
+        o.geo, o.dT, wE, wB, cb, cdt, scaleD, acc, o.N, dst);
   });
 }
 
@@ -832,6 +946,15 @@ void launch_apply_C_tangent_t(const OpCtx& o, const double* pf, const double* cb
   AFL_ADJ_DISPATCH(o.dim, o.k, {
     k_apply_C_tangent_t<D_, K_><<<adj_grid<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(o.geo, o.dT, pf, cb, w0,
                                                                                                         cdt, o.N, dst);
+  });
+}
+
+void launch_apply_C_tangent_t_svm(const OpCtx& o, const double* pf, const double* cb, const double* w0, const double* rho,
+                                  const double* u, double alpha, double beta, double* dst) {
+  const double cdt = (o.hyp == 1 ? o.tab.at22 : o.tab.at33) * o.dt;
+  AFL_ADJ_DISPATCH(o.dim, o.k, {
+    k_apply_C_tangent_t<D_, K_, true><<<adj_grid<D_, K_>(o.geo.ncells), dim3(Cfg<D_, K_>::NT), 0, o.stream>>>(
+        o.geo, o.dT, pf, cb, w0, cdt, o.N, dst, rho, u, alpha, beta);
   });
 }
 

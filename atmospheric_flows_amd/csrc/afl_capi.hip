@@ -1653,16 +1653,11 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
   launch_apply_Bt(o, t1, r);
   vec_axpby(st, N, 1.0, pbar, -1.0, r);
   auto Z = [&](const double* in, double* out) {
-    launch_apply_C_tangent_t(o, op->p_fix.p, in, wB, t1);
-    launch_scaled_velocity_mass(o, in, rho_new, us, -Ma2, -cgz, t2);
-    vec_axpby(st, NU, -1.0, t2, 1.0, t1);
+    launch_apply_C_tangent_t_svm(o, op->p_fix.p, in, wB, rho_new, us, -Ma2, -cgz, t1);
     launch_mass_solve(o, op->rho_fix.p, 1.0, o.dim, t1, t2);
     launch_apply_Bt(o, t2, out);
-    // out = D in - out - C_pf(w_B)^T in + C_pf(w_E)^T in (the VJP output buffers are free here)
-    launch_mass_apply(o, nullptr, 1.0 / (kGamma - 1.0), 1, in, t3);
-    launch_apply_C_coef_t(o, wB, in, outs[0]);
-    launch_apply_C_coef_t(o, wE, in, outs[1]);
-    vec_comb3(st, N, t3, outs[0], outs[1], out);
+    // out = D in - out - C_pf(w_B)^T in + C_pf(w_E)^T in
+    launch_apply_C_coef_pair_t(o, wE, wB, in, 1.0 / (kGamma - 1.0), out, out);
   };
   int rc;
   if ((rc = ensure_common(op))) return rc;

@@ -136,6 +136,9 @@ void launch_vjp_rhs_u(const OpCtx& o, int stage, const double* rbar, const doubl
 void launch_vjp_rhs_p(const OpCtx& o, int stage, const double* rbar, const double* const* src, double* const* out);
 void launch_apply_Bt(const OpCtx& o, const double* y, double* dst);
 void launch_apply_C_coef_t(const OpCtx& o, const double* w, const double* cb, double* dst);
+// dst = scaleD M cb + C_coef_t(wE, cb) - C_coef_t(wB, cb) - acc (acc may be dst)
+void launch_apply_C_coef_pair_t(const OpCtx& o, const double* wE, const double* wB, const double* cb, double scaleD,
+                                const double* acc, double* dst);
 // dst = int phi (a . b) over DIM-vector fields a, b
 void launch_dot_mass(const OpCtx& o, const double* a, const double* b, double* dst);
 // dst_d = int phi lam rho (alpha u_d + beta [d = vertical])
@@ -143,6 +146,9 @@ void launch_scaled_velocity_mass(const OpCtx& o, const double* lam, const double
                                  double beta, double* dst);
 void launch_apply_C_tangent(const OpCtx& o, const double* pf, const double* dw, const double* w0, double* dst);
 void launch_apply_C_tangent_t(const OpCtx& o, const double* pf, const double* cb, const double* w0, double* dst);
+// launch_apply_C_tangent_t minus launch_scaled_velocity_mass(cb, rho, u, alpha, beta) in one pass
+void launch_apply_C_tangent_t_svm(const OpCtx& o, const double* pf, const double* cb, const double* w0, const double* rho,
+                                  const double* u, double alpha, double beta, double* dst);
 // Multigrid level operator in precision R. sinv != nullptr selects the mean-density block
 // solve (w = P^{-1} B p, sinv per cell from launch_cell_rho_mass_inv); otherwise the cell-local
 // A solve runs to tol with rho. rq selects the level quadrature: k + 1 Gauss points per direction
