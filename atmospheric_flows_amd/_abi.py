@@ -45,7 +45,8 @@ class StepInfo(C.Structure):
 
 class AdjointStats(C.Structure):
     _fields_ = [("forward_steps", C.c_int32), ("adjoint_steps", C.c_int32), ("snapshots_used", C.c_int32),
-                ("recomputed_steps", C.c_int32), ("stage_sets_used", C.c_int32), ("pad_", C.c_int32)]
+                ("recomputed_steps", C.c_int32), ("stage_sets_used", C.c_int32),
+                ("stage_solve_iterations", C.c_int32)]
 
 
 class Diagnostics(C.Structure):

@@ -214,6 +214,7 @@ struct afl_operator {
   // solve (adjoint variables change little from one step to the previous)
   DevBuf adj_lam_prev[2];
   bool adj_lam_valid[2] = {false, false};
+  int64_t adj_krylov_its = 0;  // Krylov iterations of the adjoint stage solves since the last reset
   bool adj_keep_stages = true;  // afl_adjoint_run keeps stage sets (afl_set_adjoint_stage_sets)
   // observation points (device copies of their location) and point scratch
   ObsDev obs;
@@ -489,15 +490,17 @@ int gmres(afl_operator* op, double* x, const double* b, double tol, int max_it, 
 }
 
 // p-multigrid hierarchy for the current solver parameters (built once, levels share the mesh)
-// (coarsest-level Chebyshev degree unless set: 5 in 3D, where the hill solves at 48x48x20, k = 3 take
-// one FGMRES iteration from degree 5 up; in 2D 16 at k = 4 (512x100 hill: 16 fastest of 10..32) and
+// (coarsest-level Chebyshev degree unless set: in 3D 5 at k <= 3, where the hill solves at 48x48x20,
+// k = 3 take one FGMRES iteration from degree 5 up, and 12 at k = 4 (24x48x64-cell slab of the
+// 192x192x64 hill: forward step 330 ms at 5, 285 at 8, 255 at 12, 272 at 16; the store-all adjoint
+// 2.53 s at 5, 1.89 s at 12, 1.95 s at 16); in 2D 16 at k = 4 (512x100 hill: 16 fastest of 10..32) and
 // 8 at k <= 3 (256x60, k = 3 hill: forward steps within 4 % of the fastest degree, 12, and the
 // adjoint stage solves, which reduce their residual by ten orders from the previous step's
 // multiplier, cheapest from 6 to 10))
 MgConfig mg_config(const afl_solver_params& sp, int dim, int k) {
   MgConfig c;
   if (sp.mg_smoother_degree > 0) c.smoother_degree = sp.mg_smoother_degree;
-  c.coarse_degree = sp.mg_coarse_degree > 0 ? sp.mg_coarse_degree : (dim == 2 ? (k >= 4 ? 16 : 8) : 5);
+  c.coarse_degree = sp.mg_coarse_degree > 0 ? sp.mg_coarse_degree : (dim == 2 ? (k >= 4 ? 16 : 8) : (k >= 4 ? 12 : 5));
   if (sp.mg_levels > 0) c.max_levels = sp.mg_levels;
   if (sp.mg_smoothing_range > 1.0) c.smoothing_range = sp.mg_smoothing_range;
   if (sp.mg_cell_tol > 0.0) c.cell_tol = sp.mg_cell_tol;
@@ -1680,6 +1683,7 @@ int fp_adjoint(StepAdj& S, int stage, const double* rho_new, const double* ps, c
                      "adjoint stage FGMRES did not converge");
     if (rc) return rc;
   }
+  op->adj_krylov_its += it;
   vec_copy(st, N, lamp, op->adj_lam_prev[stage - 1].p);
   op->adj_lam_valid[stage - 1] = true;
   // lam_u = A^{-1} (ubar + E3_u^T lam_p)
@@ -2026,6 +2030,7 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
   if (nsteps < 1 || snapshots < AFL_ADJOINT_STORE_ALL || snapshots > 4096)
     return fail(AFL_ERR_INVALID_ARGUMENT, "adjoint_run: nsteps >= 1 and AFL_ADJOINT_STORE_ALL <= snapshots <= 4096");
   const int64_t N = op->N(), NU = N * op->dim();
+  op->adj_krylov_its = 0;
   if (snapshots == AFL_ADJOINT_STORE_ALL) {
     // one forward sweep keeping every step's input state and stage vectors, then the reverse sweep
     // with no recomputation
@@ -2098,6 +2103,7 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
     }
     hipStreamSynchronize(st);
     as.snapshots_used = nsteps;
+    as.stage_solve_iterations = (int32_t)std::min<int64_t>(op->adj_krylov_its, INT32_MAX);
     if (stats) *stats = as;
     release();
     return rc;
@@ -2133,6 +2139,7 @@ int afl_adjoint_run(afl_operator* op, const double* rho0, const double* u0, cons
   }
   hipStreamSynchronize(st);
   R.st.snapshots_used = R.max_used;
+  R.st.stage_solve_iterations = (int32_t)std::min<int64_t>(op->adj_krylov_its, INT32_MAX);
   if (stats) *stats = R.st;
   for (DevBuf& b : R.snap) b.release();
   for (DevBuf& b : R.sets) b.release();

@@ -254,7 +254,8 @@ class EulerOperator:
                                          C.byref(st)))
         return rb, ub, pb, {"forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
                             "snapshots_used": st.snapshots_used, "recomputed_steps": st.recomputed_steps,
-                            "stage_sets_used": st.stage_sets_used}
+                            "stage_sets_used": st.stage_sets_used,
+                            "stage_solve_iterations": st.stage_solve_iterations}
 
     def set_adjoint_stage_sets(self, keep):
         """Whether adjoint_run keeps the stage vectors of each advance's last step (default True)."""

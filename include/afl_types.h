@@ -57,7 +57,7 @@ typedef struct afl_solver_params {
   int32_t schur_precond;
   int32_t mg_smoother_degree;  /* Chebyshev degree of pre/post smoothing [3] */
   int32_t mg_coarse_degree;    /* Chebyshev degree on the coarsest level [2D: 16 at k = 4, 8 at k <= 3;
-                                * 3D: 5] */
+                                * 3D: 12 at k = 4, 5 at k <= 3] */
   int32_t mg_levels;           /* maximum number of degree levels k, k/2, ..., 1 [1]; with one level the
                                 * V-cycle is the Chebyshev polynomial preconditioner on the fine level */
   double mg_smoothing_range;   /* lambda_max / lambda_min of the smoothed interval [15] */
@@ -86,7 +86,7 @@ typedef struct afl_adjoint_stats {
   int32_t snapshots_used;    /* most snapshots held at once (including the initial state) */
   int32_t recomputed_steps;  /* reversed steps that recomputed their forward step (no stage set held) */
   int32_t stage_sets_used;   /* most stage sets held at once (8 fields each: 5 scalar + 3 velocity) */
-  int32_t pad_;
+  int32_t stage_solve_iterations; /* Krylov iterations of all adjoint stage solves (two per reversed step) */
 } afl_adjoint_stats;
 
 /* Hydrostatic constant-N background plus seeded theta noise. */
