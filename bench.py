@@ -282,28 +282,8 @@ def main():
         # ---- per-kernel times inside the FP32-preconditioned steps; roofline of the dominant kernel --------
         if "kernels" in legs:
             log("c3 per-kernel timing inside FP32-MG steps")
-            state = paths["fp32_mg"]["state"]
-            mask = 0
-            for f in A.TIMED_KERNELS:
-                mask |= f
-            o32.kernel_timer_start(mask, 2000 + 1000 * args.steps)
-            for _ in range(args.steps):
-                o32.imex_step(*state)
-            o32.kernel_timer_stop()
-            kern = {}
-            for f, name in A.TIMED_KERNELS.items():
-                n, ms, by = o32.kernel_timer_result(f, 0)
-                if n:
-                    kern[name] = {"launches_per_step": n / args.steps, "ms_per_step": ms / args.steps,
-                                  "avg_us": 1e3 * ms / n, "bytes_per_launch": by / n,
-                                  "GBs": by / (ms * 1e-3) / 1e9}
-                    for stg in (1, 2, 3):
-                        ns_, mss, bys = o32.kernel_timer_result(f, stg)
-                        if ns_:
-                            kern[name][f"stage{stg}"] = {"launches": ns_, "avg_us": 1e3 * mss / ns_,
-                                                         "bytes_per_launch": bys / ns_, "GBs": bys / (mss * 1e-3) / 1e9}
+            kern, dom = kernels_in_steps(o32, A, paths["fp32_mg"]["state"], args.steps)
             res["kernels_in_fp32_step"] = kern
-            dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
             d = kern[dom]
             res["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "bytes_per_launch": d["bytes_per_launch"],
                                "avg_launch_us": d["avg_us"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -366,6 +346,31 @@ def main():
         log("CPU baseline")
         res["cpu_baseline"] = run_cpu_subprocess()
     print(json.dumps(res))
+
+
+def kernels_in_steps(o, A, state, steps):
+    """HIP-event time, launches and algorithmic bytes of every timed kernel over `steps` IMEX steps
+    of operator o from `state`; returns the table and the kernel with the most time per step."""
+    mask = 0
+    for f in A.TIMED_KERNELS:
+        mask |= f
+    o.kernel_timer_start(mask, 2000 + 1000 * steps)
+    for _ in range(steps):
+        o.imex_step(*state)
+    o.kernel_timer_stop()
+    kern = {}
+    for f, name in A.TIMED_KERNELS.items():
+        n, ms, by = o.kernel_timer_result(f, 0)
+        if n:
+            kern[name] = {"launches_per_step": n / steps, "ms_per_step": ms / steps,
+                          "avg_us": 1e3 * ms / n, "bytes_per_launch": by / n,
+                          "GBs": by / (ms * 1e-3) / 1e9}
+            for stg in (1, 2, 3):
+                ns_, mss, bys = o.kernel_timer_result(f, stg)
+                if ns_:
+                    kern[name][f"stage{stg}"] = {"launches": ns_, "avg_us": 1e3 * mss / ns_,
+                                                 "bytes_per_launch": bys / ns_, "GBs": bys / (mss * 1e-3) / 1e9}
+    return kern, max(kern, key=lambda k: kern[k]["ms_per_step"])
 
 
 def adjoint_leg(args, torch, EulerOperator, A):
@@ -484,8 +489,9 @@ def c5_shard_leg(torch, EulerOperator, A, log, nsteps=20, npts=4096, snapshots=2
     """The 3D 192x192x64-cell, k=4 configuration (288 x 288 x 25 km, hill h0 = 600 m, a = 2 km) split over
     eight GPUs along x gives each a 24x192x64-cell slab (36 x 288 x 25 km, 184 M DoF); this runs one slab at
     its real shape with periodic x (the neighbours' halo faces replaced by the slab's own): 20 FP32-MG steps
-    timed, the FP64-MG path over the same steps for the theta' and w errors, and the gradient of the
-    final-time theta' misfit at 4096 seeded points of the slab by binomial checkpointing."""
+    timed, the per-kernel HIP-event times of two more and the HBM roofline of the kernel with the most
+    time per step, the FP64-MG path over the same steps for the theta' and w errors, and the gradient of
+    the final-time theta' misfit at 4096 seeded points of the slab by binomial checkpointing."""
     import numpy as np
     from atmospheric_flows_amd import make_mesh, make_hill_state
     half = 24 * 1500.0 / 2
@@ -512,6 +518,15 @@ def c5_shard_leg(torch, EulerOperator, A, log, nsteps=20, npts=4096, snapshots=2
         iters[name] = {"fp": [list(i.fp_iters) for i in infos], "gmres": [list(i.gmres_iters) for i in infos]}
         if name == "fp64_mg":
             del o
+        else:
+            # per-kernel HIP-event times of two more steps (from a copy of the state) and the HBM
+            # roofline of the kernel with the most time per step
+            log("c5 shard: per-kernel timing of 2 FP32-MG steps")
+            kern, dom = kernels_in_steps(o, A, tuple(t.clone() for t in x), 2)
+            d = kern[dom]
+            roof = {"kernel": dom, "bound": "hbm", "achieved": d["GBs"], "bytes_per_launch": d["bytes_per_launch"],
+                    "avg_launch_us": d["avg_us"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": d["GBs"] / HBM_PEAK_GBS}
     used = free0 - torch.cuda.mem_get_info()[0]
     N = A.n_per_field(mesh)
     x32, x64 = states["fp32_mg"], states["fp64_mg"]
@@ -566,6 +581,7 @@ def c5_shard_leg(torch, EulerOperator, A, log, nsteps=20, npts=4096, snapshots=2
             "n_per_field": N, "dof": dofs, "steps": nsteps,
             "fp32_mg_ms_per_step": times["fp32_mg"] * 1e3, "value_fp32_mg": dofs / times["fp32_mg"],
             "fp64_mg_ms_per_step": times["fp64_mg"] * 1e3,
+            "kernels_in_fp32_step": kern, "roofline": roof,
             "iters_per_step": iters,
             "rel_l2_fp32mg_vs_fp64mg_after_steps": errs,
             "adjoint": {"observation_points": npts, "snapshots": snapshots, "forward_run_s": t_fwd,
