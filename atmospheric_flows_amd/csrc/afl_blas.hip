@@ -37,11 +37,6 @@ __global__ void k_acc_multi(AccList L) {
     for (int64_t i = i0; i < L.n[p]; i += step) d[i] += s[i];
   }
 }
-__global__ void k_comb3(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
-                        const double* __restrict__ c, double* __restrict__ y) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = a[i] - y[i] - b[i] + c[i];
-}
 __global__ void k_mul(int64_t n, const double* __restrict__ x, const double* __restrict__ d, double* __restrict__ o) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     o[i] = x[i] * d[i];
@@ -161,9 +156,6 @@ void vec_acc_multi(hipStream_t st, const AccList& L) {
   int64_t n = 0;
   for (int p = 0; p < L.count; ++p) n = L.n[p] > n ? L.n[p] : n;
   k_acc_multi<<<dim3(grid_1d(n)), dim3(AFL_NT), 0, st>>>(L);
-}
-void vec_comb3(hipStream_t st, int64_t n, const double* a, const double* b, const double* c, double* y) {
-  k_comb3<<<dim3(grid_1d(n)), dim3(AFL_NT), 0, st>>>(n, a, b, c, y);
 }
 void vec_mul(hipStream_t st, int64_t n, const double* x, const double* d, double* out) {
   k_mul<<<dim3(grid_1d(n)), dim3(AFL_NT), 0, st>>>(n, x, d, out);

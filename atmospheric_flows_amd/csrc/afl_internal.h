@@ -199,7 +199,6 @@ struct AccList {
   }
 };
 void vec_acc_multi(hipStream_t st, const AccList& L);
-void vec_comb3(hipStream_t st, int64_t n, const double* a, const double* b, const double* c, double* y);  // y = a - y - b + c
 void vec_mul(hipStream_t st, int64_t n, const double* x, const double* d, double* out);
 void vec_addmul(hipStream_t st, int64_t n, const double* d, const double* z, double* x);  // x += d*z
 void vec_scale_by_inv(hipStream_t st, int64_t n, const double* s, double* x);  // x /= *s
