@@ -526,17 +526,19 @@ __device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g,
   constexpr int NQ = C::NQ;
   R pq[NQ], trp[2 * DIM][2];
   eval_line_faces<DIM, K, R>(p, s, nb, t1, t2, fn, T, pq, trp);
-  if (s.act) {
-#pragma unroll
-    for (int fc = 0; fc < 2 * DIM; ++fc) {
-      const R avg = R(0.5) * (trp[fc][0] + trp[fc][1]);  // walls: trp[fc][1] == trp[fc][0] (p_D = p)
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) Fq[(a * 2 * DIM + fc) * C::NFQ + s.l] = f * avg * fp.n[fc][a] * fp.jxw[fc];
-    }
-  }
-  cell_sync<DIM, K, Q>();
+  // one face-flux block (Lds::F values per cell), refilled per component: the cell's LDS stays
+  // small enough for one more block per CU in the level operators
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
+    if (a > 0) cell_sync<DIM, K, Q>();  // the previous component's add_faces reads are done
+    if (s.act) {
+#pragma unroll
+      for (int fc = 0; fc < 2 * DIM; ++fc) {
+        const R avg = R(0.5) * (trp[fc][0] + trp[fc][1]);  // walls: trp[fc][1] == trp[fc][0] (p_D = p)
+        Fq[fc * C::NFQ + s.l] = f * avg * fp.n[fc][a] * fp.jxw[fc];
+      }
+    }
+    cell_sync<DIM, K, Q>();
     R V[NQ], G[DIM][NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -546,7 +548,7 @@ __device__ __attribute__((always_inline)) inline void apply_B_rows(const Geo& g,
       for (int d = 0; d < DIM; ++d) G[d][q] = Rg[d];
     }
     integrate_line<DIM, K, false, true, R>(s, i1, i2, T, V, G, out[a]);
-    add_faces<DIM, K, R>(s, Fq + a * 2 * DIM * C::NFQ, T, out[a], 0xffu);
+    add_faces<DIM, K, R>(s, Fq, T, out[a], 0xffu);
   }
 }
 
@@ -555,14 +557,14 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_B(Geo g, const Tab* __restrict
                                                      double* __restrict__ dst) {
   const auto& T = *Tp;
   __shared__ double shw[Cfg<DIM, K>::CPB * Lds<DIM, K>::WF];
-  __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F * DIM];
+  __shared__ double shF[Cfg<DIM, K>::CPB * Lds<DIM, K>::F];
   AFL_SLOT_PTRS
   LineGeo<DIM, K> lg;
   lg.init(g, T, s.ic, s.l);
   const Nbrs<DIM> nb(g, s.ic);
   const FacePts<DIM, K> fp(g, T, s);
   double out[DIM][N1];
-  apply_B_rows<DIM, K, double>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, shF + cbs * Lds<DIM, K>::F * DIM, out);
+  apply_B_rows<DIM, K, double>(g, T, s, lg, nb, fp, p, f, t1, t2, i1, i2, fn, shF + cbs * Lds<DIM, K>::F, out);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) store_row<DIM, K>(dst + a * N, s, out[a]);
 }
@@ -1173,9 +1175,10 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
 }
 
 // b and x may be the same vector: every row is read by its owner thread before the cell's solve
-// and written by that thread after it.
+// and written by that thread after it. 3D k = 4, one component: held to 128 VGPRs (four waves per
+// SIMD instead of three; no spills).
 template <int DIM, int K, int NC>
-__global__ __launch_bounds__(AFL_NT) void k_mass_solve(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho, double scale,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 3 && K == 4 && NC == 1 ? 4 : 1))) void k_mass_solve(Geo g, const Tab* __restrict__ Tp, const double* __restrict__ rho, double scale,
                                                         int64_t N, const double* b, double* x, double tol, int maxit) {
   const auto& T = *Tp;
   constexpr int PC = Lds<DIM, K>::W > SolvLds<DIM, K>::size(NC) ? Lds<DIM, K>::W : SolvLds<DIM, K>::size(NC);
@@ -1208,7 +1211,7 @@ template <int DIM, int K, typename R = double, int Q = 2 * K + 1>
 __global__ __launch_bounds__(AFL_NT) void k_schur_w(Geo g, const TabT<R>* __restrict__ Tp, const R* __restrict__ rho, R f, int64_t N,
                                                      const R* __restrict__ p, R* __restrict__ w, R tol, int maxit) {
   const auto& T = *Tp;
-  constexpr int PB = Lds<DIM, K, Q>::WF + Lds<DIM, K, Q>::F * DIM;  // B phase: work + face nodes + face fluxes
+  constexpr int PB = Lds<DIM, K, Q>::WF + Lds<DIM, K, Q>::F;  // B phase: work + face nodes + face fluxes
   constexpr int PS = SolvLds<DIM, K, Q>::size(DIM);             // solver phase
   constexpr int PC = PB > PS ? PB : PS;
   __shared__ R shw[Cfg<DIM, K, Q>::CPB * PC];
@@ -1333,9 +1336,12 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
                                                          const R* __restrict__ p, R* __restrict__ w) {
   const auto& T = *Tp;
   using C = Cfg<DIM, K, Q>;
-  constexpr int PB = Lds<DIM, K, Q>::WF + Lds<DIM, K, Q>::F * DIM;
+  constexpr int PB = Lds<DIM, K, Q>::WF + Lds<DIM, K, Q>::F;
   constexpr int PM = 2 * DIM * C::NPC;
   constexpr int PC = PB > PM ? PB : PM;
+  // cells spread over several wavefronts need the mass-inverse scratch [0, PM) apart from the face
+  // fluxes that slower waves' add_faces may still read; inside one wavefront LDS accesses stay in order
+  static_assert(C::WAVE_LOCAL || PM <= Lds<DIM, K, Q>::WF, "the mass-inverse scratch must not overlap the face fluxes");
   __shared__ R shw[C::CPB * PC];
   constexpr int N1 = C::N1;
   Slot<DIM, K, Q> s(g);
@@ -1349,7 +1355,6 @@ __global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2
   R bp[DIM][N1];
   apply_B_rows<DIM, K, R>(g, T, s, lg, nb, fp, p, f, base, base + C::EV1, base, base + C::IN1,
                           base + Lds<DIM, K, Q>::W, base + Lds<DIM, K, Q>::WF, bp);
-  // the mass-inverse transposes use [0, PM), disjoint from the face fluxes add_faces may still read
   cell_mi_apply<DIM, K, DIM, R>(s, T, base, bp);
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
