@@ -20,6 +20,12 @@
 #ifndef AFL_E2_WAVES
 #define AFL_E2_WAVES 2
 #endif
+// 3D FP32 level C apply + Chebyshev step at k <= 3: waves per SIMD the register budget must allow
+// (4: 128 VGPRs with a 28-byte spill at k = 3, level C 521 -> 503 us at 96x96x40; at k = 4 the
+// cap spills 68 bytes and gains nothing)
+#ifndef AFL_CHEB_WAVES_3D
+#define AFL_CHEB_WAVES_3D 4
+#endif
 
 namespace afl {
 
@@ -755,7 +761,7 @@ __global__ __launch_bounds__(AFL_NT) void k_apply_C(Geo g, const TabT<R>* __rest
 // The block preconditioner is cell-local (LDS transposes of the cell's rows); pD, d and x are
 // read only inside the thread's own cell, so d and x (and pD == d or x) update in place.
 template <int DIM, int K, typename R, int Q = 2 * K + 1>
-__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 && sizeof(R) == 4 ? AFL_CHEB_WAVES : 1))) void k_apply_C_cheb(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
+__global__ __launch_bounds__(AFL_NT) __attribute__((amdgpu_waves_per_eu(DIM == 2 && sizeof(R) == 4 ? AFL_CHEB_WAVES : (DIM == 3 && K <= 3 && sizeof(R) == 4 ? AFL_CHEB_WAVES_3D : 1)))) void k_apply_C_cheb(Geo g, const TabT<R>* __restrict__ Tp, R cdt, int64_t N, const R* __restrict__ pfix,
                                                           const R* __restrict__ w, const R* pD, R scaleD,
                                                           const R* __restrict__ jbar, const R* __restrict__ rin,
                                                           R* __restrict__ rout, R c1, R c2, R* d, R* x) {
@@ -877,7 +883,9 @@ struct SolvLds {
   static constexpr int WF = EVF > INF ? EVF : INF;  // per component work area
   static constexpr int PER = WF + C::NPC;            // + the component's dofs
   // set-up scratch: co per line, line sums (3D), marginals, per-line ratio bounds
-  static constexpr int SETUP = C::LT * C::NQ + (DIM == 3 ? C::LT : 0) + (DIM + 1) * C::NQ + 2 * C::LT;
+  // (3D: + the partial vertical marginals and partial bounds of the two-level reductions)
+  static constexpr int SETUP = C::LT * C::NQ + (DIM == 3 ? C::LT : 0) + (DIM + 1) * C::NQ + 2 * C::LT +
+                               (DIM == 3 ? C::NQ * C::NQ + 2 * C::NQ : 0);
   static constexpr int SEP = DIM * C::N1 * C::N1;    // the dim inverse 1D matrices (persist)
   static constexpr int sep_off(int nc) { return nc * PER > SETUP ? nc * PER : SETUP; }
   static constexpr int size(int nc) { return sep_off(nc) + SEP; }
@@ -922,6 +930,8 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
   R* sL = sco + LT * NQ;                         // [LT] line sums (3D)
   R* smg = sL + (DIM == 3 ? LT : 0);             // [DIM][NQ] marginals a, (b,) c
   R* srm = smg + (DIM + 1) * NQ;                 // [2][LT] per-line min / max of co / c~
+  R* part = srm + 2 * LT;                        // 3D: [NQ][NQ] partial vertical marginals
+  R* smm = part + (DIM == 3 ? NQ * NQ : 0);      // 3D: [2][NQ] partial bounds
   if (threadIdx.x == 0) nit_blk = 1;
   R lsum = R(0);
 #pragma unroll
@@ -932,10 +942,27 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
     if (DIM == 3) sL[s.l] = lsum;
   }
   cell_sync<DIM, K, Q>();
+  if constexpr (DIM == 3) {
+    // vertical marginal in two levels: thread l = qy NQ + q first sums the NQ lines of row qy at
+    // height q (the LT-term sum of one thread is a long dependent chain)
+    if (s.act) {
+      const int q = s.l % NQ, qy = s.l / NQ;
+      R c = R(0);
+#pragma unroll
+      for (int qx = 0; qx < NQ; ++qx) c += sco[(qy * NQ + qx) * NQ + q];
+      part[s.l] = c;
+    }
+    cell_sync<DIM, K, Q>();
+  }
   if (s.act && s.l < NQ) {
     // vertical marginal: sum over the lines of co at qz = l (and its reciprocal)
     R c = R(0);
-    for (int t = 0; t < LT; ++t) c += sco[t * NQ + s.l];
+    if constexpr (DIM == 3) {
+#pragma unroll
+      for (int qy = 0; qy < NQ; ++qy) c += part[qy * NQ + s.l];
+    } else {
+      for (int t = 0; t < LT; ++t) c += sco[t * NQ + s.l];
+    }
     smg[(DIM - 1) * NQ + s.l] = c;
     smg[DIM * NQ + s.l] = R(1) / c;
     if (DIM == 2) {
@@ -992,7 +1019,29 @@ __device__ __attribute__((always_inline)) inline void cell_cheb(const TabT<R>& T
   }
   cell_sync<DIM, K, Q>();
   R cmin = R(1), cmax = R(1);
-  if (s.act) {
+  if constexpr (DIM == 3) {
+    // the cell's bounds in two levels: threads l < NQ reduce the lines l, l + NQ, ...
+    if (s.act && s.l < NQ) {
+      R mn = R(1e30), mx = R(0);
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        mn = fmin(mn, srm[r * NQ + s.l]);
+        mx = fmax(mx, srm[LT + r * NQ + s.l]);
+      }
+      smm[s.l] = mn;
+      smm[NQ + s.l] = mx;
+    }
+    cell_sync<DIM, K, Q>();
+    if (s.act) {
+      R mn = R(1e30), mx = R(0);
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        mn = fmin(mn, smm[r]);
+        mx = fmax(mx, smm[NQ + r]);
+      }
+      cmin = mn; cmax = mx;
+    }
+  } else if (s.act) {
     R mn = R(1e30), mx = R(0);
     for (int t = 0; t < LT; ++t) {
       mn = fmin(mn, srm[t]);
