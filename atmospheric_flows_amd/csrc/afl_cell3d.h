@@ -550,7 +550,8 @@ template <int K, int Q, int NC>
 struct SolveWk {
   using Z = Sz<K, Q>;
   static constexpr int MI = 3 * Z::N1 * Z::N1;                               // the 1D inverses (persist)
-  static constexpr int SETUP = Z::LT * Q + Z::LT + 4 * Q;                      // co per line, line sums, marginals
+  static constexpr int SETUP = Z::LT * Q + Z::LT + 4 * Q + Z::LT;             // co per line, line sums, marginals,
+                                                                               // partial vertical marginals
   static constexpr int EVI = Z::EV > Z::INV ? Z::EV : Z::INV;
   static constexpr int RUN = 2 * NC * Z::NPC + NC * EVI;                      // staged vectors + sweeps
   static constexpr int SIZE = MI + (SETUP > RUN ? SETUP : RUN);
@@ -574,6 +575,7 @@ __device__ __attribute__((always_inline)) inline void cheb_solve(const Cell<K, Q
   double* sco = wk + SW::MI;        // [LT][Q]
   double* sL = sco + LT * Q;        // [LT] line sums
   double* smg = sL + LT;            // [4][Q]: x, y, z marginals, 1 / z marginal
+  double* part = smg + 4 * Q;       // [LT]: vertical marginal of each row of lines
   double lsum = 0.0;
 #pragma unroll
   for (int q = 0; q < Q; ++q) lsum += co[q];
@@ -583,9 +585,19 @@ __device__ __attribute__((always_inline)) inline void cheb_solve(const Cell<K, Q
     sL[c.lane] = lsum;
   }
   wsync();
+  // vertical marginal in two levels: lane qy Q + q first sums the Q lines of row qy at height q
+  if (c.line) {
+    const int q = c.lane % Q, qy = c.lane / Q;
+    double p = 0.0;
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) p += sco[(qy * Q + qx) * Q + q];
+    part[c.lane] = p;
+  }
+  wsync();
   if (c.lane < Q) {
     double cz = 0.0, ax = 0.0, by = 0.0;
-    for (int t = 0; t < LT; ++t) cz += sco[t * Q + c.lane];
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) cz += part[qy * Q + c.lane];
 #pragma unroll
     for (int t = 0; t < Q; ++t) { ax += sL[t * Q + c.lane]; by += sL[c.lane * Q + t]; }
     smg[c.lane] = ax;           // x marginal: lines (qx = lane, qy)
